@@ -1,0 +1,350 @@
+"""ctypes binding of the FP64 oracle (liborc.so) -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this module, and only
+as the checker. The product library never loads it.
+
+Every function restates a reference HOST_MEMORY routine; the C++ source (afq_oracle.cpp) cites
+the reference file:line next to each one.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liborc.so")
+
+P_WEIGHT, P_PHASE, P_PSEUDO_ELOC, P_E1, P_EXX, P_EJ, P_OVLP = 0, 1, 2, 3, 4, 5, 6
+P_LOGSCL_UP, P_LOGSCL_DN, P_IS_UNITARY, P_THETA, P_COUNT = 7, 8, 9, 10, 11
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        _lib = C.CDLL(_LIB)
+        _declare(_lib)
+    return _lib
+
+
+_vp = C.c_void_p
+_i = C.c_int
+_l = C.c_long
+_d = C.c_double
+
+
+def _declare(L):
+    sig = {
+        "orc_getrf": (_i, [_i, _vp, _vp]),
+        "orc_log_det_from_getrf": (None, [_i, _vp, _vp, _vp]),
+        "orc_getri_or_zero": (_i, [_i, _vp, _vp]),
+        "orc_inverse_logdet": (None, [_i, _i, _vp, _vp, _vp]),
+        "orc_log_overlap": (None, [_i, _i, _i, _vp, _vp, _l, _vp]),
+        "orc_mixed_density_matrix": (None, [_i, _i, _i, _vp, _vp, _l, _vp, _vp, _i]),
+        "orc_half_rotate": (None, [_i, _i, _i, _vp, _vp, _vp, _vp, _vp]),
+        "orc_exchange_v0": (None, [_i, _i, _vp, _vp]),
+        "orc_vbias": (None, [_i, _i, _i, _i, _i, _i, _d, _vp, _vp, _vp, _vp]),
+        "orc_vhs": (None, [_i, _i, _i, _d, _vp, _vp, _vp]),
+        "orc_energy": (None, [_i, _i, _i, _i, _i, _i, _d, _vp, _vp, _vp, _vp, _vp, _vp]),
+        "orc_exp_hermitian": (None, [_i, _vp, _vp]),
+        "orc_one_body_propagator": (None, [_i, _i, _d, _vp, _vp, _vp, _vp, _vp, _vp]),
+        "orc_propagate": (None, [_i, _i, _i, _vp, _l, _vp, _vp, _i]),
+        "orc_orthogonalize": (None, [_i, _i, _i, _vp, _l, _vp]),
+        "orc_construct_X": (None, [_i, _i, _d, _i, _i, _vp, _vp, _vp, _vp]),
+        "orc_meanfield_factor": (None, [_i, _i, _vp, _vp, _vp]),
+        "orc_walker_update": (None, [_i, _vp, _l, _l, _i, _i, _i, _d, _d, _d, _vp, _vp, _vp,
+                                     _vp, _d, _d, _vp]),
+        "orc_bound_weights": (None, [_i, _vp, _l, _l, _d, _d, _d, _vp]),
+        "orc_system_create": (_vp, [_i, _i, _i, _vp, _vp, _vp, _d, _d]),
+        "orc_system_destroy": (None, [_vp]),
+        "orc_system_set_params": (None, [_vp, _i, _d, _d, _d, _d, _d, _i, _i, _i]),
+        "orc_system_tensors": (None, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+        "orc_system_mixed_density_matrix": (None, [_vp, _i, _vp, _l, _vp, _vp]),
+        "orc_system_log_overlap": (None, [_vp, _i, _vp, _l, _vp]),
+        "orc_system_energy": (None, [_vp, _i, _vp, _l, _vp, _vp]),
+        "orc_system_seed_properties": (None, [_vp, _i, _vp, _l]),
+        "orc_system_step": (None, [_vp, _i, _vp, _l, _vp, _d, _i, _vp, _vp, _vp, _vp, _vp]),
+        "orc_system_orthogonalize": (None, [_vp, _i, _vp, _l]),
+        "orc_system_stats": (None, [_vp, _vp, _vp]),
+        "orc_num_threads": (_i, []),
+        "orc_set_num_threads": (None, [_i]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def _c(a, dt=np.complex128):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+# ------------------------------------------------------------------ det_ops level
+def getrf(A):
+    A = _c(A).copy()
+    n = A.shape[0]
+    ipiv = np.zeros(n, np.int32)
+    info = lib().orc_getrf(n, _p(A), _p(ipiv))
+    return A, ipiv, info
+
+
+def log_det_from_getrf(LU, ipiv, res=0j):
+    LU = _c(LU)
+    r = np.array([res], np.complex128)
+    lib().orc_log_det_from_getrf(LU.shape[0], _p(LU), _p(np.ascontiguousarray(ipiv, np.int32)),
+                                 _p(r))
+    return r[0]
+
+
+def getri_or_zero(LU, ipiv):
+    LU = _c(LU).copy()
+    info = lib().orc_getri_or_zero(LU.shape[0], _p(LU), _p(np.ascontiguousarray(ipiv, np.int32)))
+    return LU, info
+
+
+def inverse_logdet(A):
+    A = _c(A)
+    nb, n, _ = A.shape
+    inv = np.zeros_like(A)
+    ld = np.zeros(nb, np.complex128)
+    lib().orc_inverse_logdet(nb, n, _p(A), _p(inv), _p(ld))
+    return inv, ld
+
+
+def log_overlap(PsiT, Phi):
+    """det_ops::Log_Overlap: PsiT [N,M], Phi [nb,M,N] -> ov [nb] (accumulated from 0)."""
+    PsiT, Phi = _c(PsiT), _c(Phi)
+    nb, M, N = Phi.shape
+    ov = np.zeros(nb, np.complex128)
+    lib().orc_log_overlap(nb, M, N, _p(PsiT), _p(Phi), M * N, _p(ov))
+    return ov
+
+
+def mixed_density_matrix(PsiT, Phi, compact=True):
+    PsiT, Phi = _c(PsiT), _c(Phi)
+    nb, M, N = Phi.shape
+    G = np.zeros((nb, N, M) if compact else (nb, M, M), np.complex128)
+    ov = np.zeros(nb, np.complex128)
+    lib().orc_mixed_density_matrix(nb, M, N, _p(PsiT), _p(Phi), M * N, _p(G), _p(ov),
+                                   int(compact))
+    return G, ov
+
+
+def half_rotate(h, Likn, PsiT):
+    h, Likn, PsiT = _c(h, np.float64), _c(Likn, np.float64), _c(PsiT)
+    M = h.shape[0]
+    K = Likn.shape[2]
+    N = PsiT.shape[0]
+    haj = np.zeros((N, M), np.complex128)
+    Lnak = np.zeros((K, N, M), np.complex128)
+    lib().orc_half_rotate(M, N, K, _p(h), _p(Likn), _p(PsiT), _p(haj), _p(Lnak))
+    return haj, Lnak
+
+
+def exchange_v0(Likn):
+    Likn = _c(Likn, np.float64)
+    M, _, K = Likn.shape
+    v0 = np.zeros((M, M))
+    lib().orc_exchange_v0(M, K, _p(Likn), _p(v0))
+    return v0
+
+
+def vbias(G, Lnak, dt, walker_type=1, nup=None, ndn=0, Lnak_dn=None):
+    G, Lnak = _c(G), _c(Lnak)
+    W = G.shape[0]
+    K, n_up, M = Lnak.shape
+    nup = n_up if nup is None else nup
+    v = np.zeros((W, K), np.complex128)
+    lib().orc_vbias(W, M, nup, ndn, K, walker_type, dt, _p(Lnak),
+                    _p(None if Lnak_dn is None else _c(Lnak_dn)), _p(G), _p(v))
+    return v
+
+
+def vhs(X, Likn, dt):
+    X, Likn = _c(X), _c(Likn, np.float64)
+    W, K = X.shape
+    M = Likn.shape[0]
+    V = np.zeros((W, M, M), np.complex128)
+    lib().orc_vhs(W, M, K, dt, _p(Likn), _p(X), _p(V))
+    return V
+
+
+def energy(G, haj, Lnak, E0=0.0, walker_type=1, ndn=0, haj_dn=None, Lnak_dn=None):
+    G, haj, Lnak = _c(G), _c(haj), _c(Lnak)
+    W = G.shape[0]
+    K, nup, M = Lnak.shape
+    E = np.zeros((W, 3), np.complex128)
+    lib().orc_energy(W, M, nup, ndn, K, walker_type, E0, _p(haj),
+                     _p(None if haj_dn is None else _c(haj_dn)), _p(Lnak),
+                     _p(None if Lnak_dn is None else _c(Lnak_dn)), _p(G), _p(E))
+    return E
+
+
+def exp_hermitian(A):
+    A = _c(A)
+    out = np.zeros_like(A)
+    lib().orc_exp_hermitian(A.shape[0], _p(A), _p(out))
+    return out
+
+
+def propagate(Phi, P1, V, order=6):
+    Phi = _c(Phi).copy()
+    nb, M, N = Phi.shape
+    lib().orc_propagate(nb, M, N, _p(Phi), M * N, _p(_c(P1)), _p(_c(V)), order)
+    return Phi
+
+
+def orthogonalize(Phi):
+    Phi = _c(Phi).copy()
+    nb, M, N = Phi.shape
+    ld = np.zeros(nb, np.complex128)
+    lib().orc_orthogonalize(nb, M, N, _p(Phi), M * N, _p(ld))
+    return Phi, ld
+
+
+def construct_X(vbias_, xi, vMF, vbias_bound=50.0, project=False, free_projection=False):
+    X = _c(vbias_).copy()
+    W, K = X.shape
+    HW = np.zeros(W, np.complex128)
+    lib().orc_construct_X(W, K, vbias_bound, int(project), int(free_projection), _p(_c(vMF)),
+                          _p(_c(xi, np.float64)), _p(X), _p(HW))
+    return X, HW
+
+
+def meanfield_factor(X, vMF):
+    X = _c(X)
+    W, K = X.shape
+    mf = np.zeros(W, np.complex128)
+    lib().orc_meanfield_factor(W, K, _p(X), _p(_c(vMF)), _p(mf))
+    return mf
+
+
+# ------------------------------------------------------------------ whole system
+class System:
+    """Closed-shell NOMSD (one determinant) + Real3Index Hamiltonian + base propagator."""
+
+    def __init__(self, h, Likn, PsiT, E0=0.0, dt=0.005):
+        self._h = _c(h, np.float64)
+        self._L = _c(Likn, np.float64)
+        self._P = _c(PsiT)
+        self.M = self._h.shape[0]
+        self.K = self._L.shape[2]
+        self.N = self._P.shape[0]
+        self.dt = dt
+        self.E0 = E0
+        self.ptr = lib().orc_system_create(self.M, self.N, self.K, _p(self._h), _p(self._L),
+                                           _p(self._P), E0, dt)
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            lib().orc_system_destroy(self.ptr)
+            self.ptr = None
+
+    @property
+    def ldw(self):
+        return self.M * self.N + P_COUNT
+
+    def set_params(self, taylor_n=6, vbias_bound=50.0, upper_cutoff_scale=10.0,
+                   lower_cutoff_scale=1.0, weight_bound_floor=100.0, weight_bound_fraction=0.1,
+                   free_projection=False, use_cp_constraint=False, project_force_bias=False):
+        lib().orc_system_set_params(self.ptr, taylor_n, vbias_bound, upper_cutoff_scale,
+                                    lower_cutoff_scale, weight_bound_floor,
+                                    weight_bound_fraction, int(free_projection),
+                                    int(use_cp_constraint), int(project_force_bias))
+
+    def tensors(self):
+        M, N, K = self.M, self.N, self.K
+        out = dict(haj=np.zeros((N, M), np.complex128), Lnak=np.zeros((K, N, M), np.complex128),
+                   v0=np.zeros((M, M)), vMF=np.zeros(K, np.complex128),
+                   P1=np.zeros((M, M), np.complex128), H1=np.zeros((M, M), np.complex128))
+        lib().orc_system_tensors(self.ptr, *[_p(out[k]) for k in
+                                             ("haj", "Lnak", "v0", "vMF", "P1", "H1")])
+        return out
+
+    def new_walkers(self, W):
+        """walker_buffer [W, M*N+11] with every walker at the trial, weight 1, phase 1."""
+        buf = np.zeros((W, self.ldw), np.complex128)
+        Phi0 = self._P.conj().T
+        buf[:, :self.M * self.N] = Phi0.reshape(-1)
+        buf[:, self.M * self.N + P_WEIGHT] = 1.0
+        buf[:, self.M * self.N + P_PHASE] = 1.0
+        buf[:, self.M * self.N + P_IS_UNITARY] = 1.0
+        return buf
+
+    def phi(self, buf):
+        return buf[:, :self.M * self.N].reshape(-1, self.M, self.N)
+
+    def props(self, buf, p):
+        return buf[:, self.M * self.N + p]
+
+    def mixed_density_matrix(self, buf):
+        W = buf.shape[0]
+        G = np.zeros((W, self.N, self.M), np.complex128)
+        ov = np.zeros(W, np.complex128)
+        lib().orc_system_mixed_density_matrix(self.ptr, W, _p(buf), buf.shape[1], _p(G), _p(ov))
+        return G, ov
+
+    def log_overlap(self, buf):
+        W = buf.shape[0]
+        ov = np.zeros(W, np.complex128)
+        lib().orc_system_log_overlap(self.ptr, W, _p(buf), buf.shape[1], _p(ov))
+        return ov
+
+    def energy(self, buf):
+        W = buf.shape[0]
+        E = np.zeros((W, 3), np.complex128)
+        ov = np.zeros(W, np.complex128)
+        lib().orc_system_energy(self.ptr, W, _p(buf), buf.shape[1], _p(E), _p(ov))
+        return E, ov
+
+    def seed_properties(self, buf):
+        lib().orc_system_seed_properties(self.ptr, buf.shape[0], _p(buf), buf.shape[1])
+
+    def step(self, buf, xi, Eshift, hybrid=True, capture=False):
+        W = buf.shape[0]
+        xi = _c(xi, np.float64)
+        out = None
+        if capture:
+            M, N, K = self.M, self.N, self.K
+            out = dict(vbias=np.zeros((W, K), np.complex128), X=np.zeros((W, K), np.complex128),
+                       V=np.zeros((W, M, M), np.complex128), ov=np.zeros(W, np.complex128),
+                       E=np.zeros((W, 3), np.complex128))
+            args = [_p(out[k]) for k in ("vbias", "X", "V", "ov", "E")]
+        else:
+            args = [None] * 5
+        lib().orc_system_step(self.ptr, W, _p(buf), buf.shape[1], _p(xi), Eshift, int(hybrid),
+                              *args)
+        return out
+
+    def orthogonalize(self, buf):
+        lib().orc_system_orthogonalize(self.ptr, buf.shape[0], _p(buf), buf.shape[1])
+
+    def stats(self):
+        e = np.zeros(3, np.int64)
+        w = np.zeros(2, np.int64)
+        lib().orc_system_stats(self.ptr, _p(e), _p(w))
+        return e, w
+
+
+def num_threads() -> int:
+    return lib().orc_num_threads()
+
+
+def set_num_threads(n: int) -> None:
+    lib().orc_set_num_threads(n)

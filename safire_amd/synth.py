@@ -1,0 +1,90 @@
+"""Synthetic closed-shell Cholesky Hamiltonians, RHF trials and auxiliary fields.
+
+The five workloads of the walker-step benchmark, built deterministically from fixed seeds
+(numpy PCG64). The construction, fixed once here:
+
+* H1 real symmetric: off-diagonal 0.1 * (A + A^T)/sqrt(2), A ~ N(0,1); diagonal linspace(-2, 0, M).
+* Cholesky vectors L^n real symmetric: s_n * (B_n + B_n^T)/sqrt(2), B_n ~ N(0,1),
+  s_n = 0.5 * exp(-n / decay).  Stored as Likn[i][j][n] (n fastest), the reference layout of
+  Real3IndexFactorization::Likn (Real3IndexFactorization.hpp:453-454).
+* Trial: the first N columns of Q from QR of an M x M N(0,1) matrix; PsiT = Q[:, :N]^H is
+  [N, M] complex (NOMSD stores orbitals as the conjugate transpose, NOMSD.cpp:133-147).
+* E0 = 0, dt = 0.005, every walker starts at the trial with weight 1.
+* Fields xi[step][w][n] ~ N(0,1), one seeded stream per workload (they replace probit(U) of
+  construct_X_kernel.hpp:71; U is uniform so probit(U) ~ N(0,1) too).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+WORKLOADS = {
+    # name: (M, N, K, W, steps, decay, ortho_interval)
+    "c1": (24, 6, 96, 64, 50, 24.0, 5),
+    "c2": (64, 16, 320, 512, 100, 24.0, 5),
+    "c3": (128, 32, 640, 1024, 200, 64.0, 5),
+    "c5": (64, 10, 320, 256, 50, 24.0, 5),
+}
+SEEDS = {"c1": 1001, "c2": 1002, "c3": 1003, "c4": 1004, "c5": 1005}
+DT = 0.005
+
+
+@dataclass
+class System:
+    M: int
+    N: int
+    K: int
+    h: np.ndarray            # [M, M] float64
+    Likn: np.ndarray         # [M, M, K] float64
+    PsiT: np.ndarray         # [N, M] complex128
+    E0: float = 0.0
+    dt: float = DT
+    seed: int = 0
+    name: str = ""
+    meta: dict = field(default_factory=dict)
+
+
+def make_system(M: int, N: int, K: int, seed: int, decay: float = 24.0, scale: float = 0.5,
+                name: str = "") -> System:
+    rng = np.random.default_rng(seed)
+    A = rng.standard_normal((M, M))
+    h = 0.1 * (A + A.T) / np.sqrt(2.0)
+    h[np.diag_indices(M)] = np.linspace(-2.0, 0.0, M)
+    B = rng.standard_normal((K, M, M))
+    Ln = (B + B.transpose(0, 2, 1)) / np.sqrt(2.0)
+    s = scale * np.exp(-np.arange(K) / decay)
+    Ln *= s[:, None, None]
+    Likn = np.ascontiguousarray(Ln.transpose(1, 2, 0))
+    Q, _ = np.linalg.qr(rng.standard_normal((M, M)))
+    PsiT = np.ascontiguousarray(Q[:, :N].conj().T).astype(np.complex128)
+    return System(M=M, N=N, K=K, h=np.ascontiguousarray(h), Likn=Likn, PsiT=PsiT, seed=seed,
+                  name=name)
+
+
+def workload(name: str) -> tuple[System, int, int, int]:
+    """Returns (system, walkers, steps, ortho_interval) of a named single-system workload."""
+    M, N, K, W, steps, decay, ortho = WORKLOADS[name]
+    return make_system(M, N, K, SEEDS[name], decay=decay, name=name), W, steps, ortho
+
+
+def fields(seed: int, steps: int, W: int, K: int, first_step: int = 0) -> np.ndarray:
+    """xi[step][w][n] ~ N(0,1); step s always draws the same values whatever the window."""
+    out = np.empty((steps, W, K))
+    for s in range(steps):
+        rng = np.random.default_rng([seed, 7919, first_step + s])
+        out[s] = rng.standard_normal((W, K))
+    return out
+
+
+def uneven_batch(seed: int = SEEDS["c4"], nsys: int = 32):
+    """32 independent systems: M uniform in {16,24,...,96}, N = M // 4, K = 5 M, W in [32,256]."""
+    rng = np.random.default_rng(seed)
+    Ms = rng.choice(np.arange(16, 97, 8), size=nsys)
+    Ws = rng.integers(32, 257, size=nsys)
+    systems = []
+    for i, (M, W) in enumerate(zip(Ms, Ws)):
+        M = int(M)
+        systems.append((make_system(M, max(1, M // 4), 5 * M, seed * 1000 + i, name=f"c4.{i}"),
+                        int(W)))
+    return systems
