@@ -1,0 +1,200 @@
+/* afq.h -- C ABI of the MI355X (gfx950) phaseless-AFQMC walker step.
+ *
+ * Drop-in for SAFIRE's walker step with a Cholesky-factorised Hamiltonian: the operations of
+ * HamiltonianOperations / Real3IndexFactorization (vbias, vHS, energy), of det_ops
+ * (MixedDensityMatrix, Log_Overlap, Propagate, orthogonalize) and of AFQMCBasePropagator
+ * (assemble_X, walker_update, Propagate, Orthogonalize), acting on a WalkerSet buffer.
+ *
+ * Conventions (identical to the reference so that results diff elementwise):
+ *   - C order, complex<double> interleaved (re, im): "c128".
+ *   - Walker buffer: c128 [W][ldw]; walker w's Slater matrix Phi_w[i][a] (i < M orbital,
+ *     a < N electron) at w*ldw + i*N + a, then the 11 property slots at w*ldw + M*N + p
+ *     (p = AFQ_P_WEIGHT ... AFQ_P_THETA) -- WalkerSetBase.cpp:43-124, WalkerSetBase.h:255-328.
+ *   - Compact mixed density matrix G: c128 [W][N][M] (= (Phi T^-1)^T, density_matrix.hpp:574).
+ *   - vbias / X: c128 [W][K];  vHS V: c128 [W][M][M];  E: c128 [W][3] = (E1 incl. E0, EXX, EJ).
+ *   - Pointers named d_* are device pointers; h_* are host pointers. `stream` is a hipStream_t
+ *     (NULL = the default stream); every call only enqueues work on it.
+ *   - Every entry point returns an afq_status; nothing throws across the ABI. afq_last_error()
+ *     returns the message of the last failure on the calling thread.
+ *   - Nothing allocates device memory after afq_ham_reserve(), so afq_step() and the other
+ *     per-step calls may be captured in a hipGraph.
+ */
+#ifndef SAFIRE_AMD_AFQ_H
+#define SAFIRE_AMD_AFQ_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct afq_ham_s* afq_ham;
+
+enum afq_status {
+  AFQ_OK = 0,
+  AFQ_EBADSHAPE = 1,   /* utils::check shape failures of the reference */
+  AFQ_EHIP = 2,        /* a HIP runtime call failed */
+  AFQ_ENOMEM = 3,
+  AFQ_EINVAL = 4,
+  AFQ_EUNSUPPORTED = 5,
+  AFQ_ECAPACITY = 6    /* more walkers than afq_ham_reserve() sized the workspace for */
+};
+
+/* WALKER_TYPES values of the reference (config.h:47-51). */
+enum afq_walker_type { AFQ_CLOSED = 1, AFQ_COLLINEAR = 2 };
+
+/* Arithmetic of the Cholesky contractions (vbias, vHS, exchange/Coulomb energy). Overlaps,
+ * log-determinants, inverses, QR, the propagator, weights and energy sums stay FP64. */
+enum afq_precision {
+  AFQ_FP64 = 0,        /* plain form: everything FP64 (FP64 MFMA)                          */
+  AFQ_FP32 = 1,        /* contractions in FP32 on MFMA with FP32 accumulation               */
+  AFQ_BF16X3 = 2       /* contractions as 3-product split-complex BF16 with hi/lo splits,
+                          FP32 accumulation                                                */
+};
+
+/* Walker property slots (WalkerConfig.hpp:28-48, offsets WalkerSetBase.cpp:87-117). */
+enum afq_property {
+  AFQ_P_WEIGHT = 0, AFQ_P_PHASE, AFQ_P_PSEUDO_ELOC, AFQ_P_E1, AFQ_P_EXX, AFQ_P_EJ, AFQ_P_OVLP,
+  AFQ_P_LOGSCL_UP, AFQ_P_LOGSCL_DN, AFQ_P_IS_UNITARY, AFQ_P_THETA, AFQ_P_COUNT
+};
+
+/* PropagatorParameters (parameters.hpp:103-126) with the defaults of resolve_defaults for a
+ * non-model Hamiltonian (parameter_defaults.cpp:274-282). */
+typedef struct {
+  int taylor_n;                 /* 6 */
+  double vbias_bound;           /* 50 */
+  double upper_cutoff_scale;    /* 10 */
+  double lower_cutoff_scale;    /* 1 */
+  double weight_bound_floor;    /* 100 */
+  double weight_bound_fraction; /* 0.1 */
+  int hybrid;                   /* 1 */
+  int free_projection;          /* 0 */
+  int use_cp_constraint;        /* 0 */
+  int project_force_bias;       /* 0 */
+} afq_step_params;
+
+void afq_default_params(afq_step_params* p);
+const char* afq_last_error(void);
+int afq_version(void);
+
+/* ---------------------------------------------------------------------------------------------
+ * Hamiltonian + trial + one-body propagator of one system.
+ * Replaces RealDenseHamiltonian::getHamiltonianOperations (RealDenseHamiltonian.cpp:52-185:
+ * haj, Lnak, v0), NOMSD construction (Wavefunction.cpp:256-287) and
+ * AFQMCBasePropagator::generateP1 (AFQMCBasePropagator.cpp:42-214: vMF, H1, P1 = exp(-H1/2)).
+ * Inputs are host arrays: h_h [M][M] f64 (hcore), h_Likn [M][M][K] f64 (DenseFactorized/L),
+ * h_PsiT [N][M] c128 (trial orbitals as the conjugate transpose), one determinant with
+ * coefficient 1, CLOSED walkers. All derived tensors are built on the GPU.
+ * ------------------------------------------------------------------------------------------- */
+int afq_ham_create(int M, int N, int K, const double* h_h, const double* h_Likn,
+                   const double* h_PsiT, double E0, double dt, int walker_type, afq_ham* out);
+int afq_ham_destroy(afq_ham ham);
+/* Sizes the per-step workspace for up to max_walkers walkers (the only allocation after
+ * create). Re-reserving with a larger count reallocates. */
+int afq_ham_reserve(afq_ham ham, int max_walkers);
+int afq_ham_set_precision(afq_ham ham, int precision);
+int afq_ham_set_params(afq_ham ham, const afq_step_params* p);
+int afq_ham_get_params(afq_ham ham, afq_step_params* p);
+/* Copies the derived tensors to host memory (any pointer may be NULL): haj [N][M] c128,
+ * Lnak [K][N][M] c128, v0 [M][M] f64, vMF [K] c128, P1 [M][M] c128, H1 [M][M] c128. */
+int afq_ham_tensors(afq_ham ham, double* h_haj, double* h_Lnak, double* h_v0, double* h_vMF,
+                    double* h_P1, double* h_H1);
+/* Local-energy bound statistics (total, upper, lower) and weight-bound statistics (total,
+ * upper) since creation (AFQMCBasePropagator.h:199-237). */
+int afq_ham_stats(afq_ham ham, long long* h_eloc3, long long* h_weight2);
+
+/* ---------------------------------------------------------------------------------------------
+ * det_ops level (density_matrix.hpp, propagate.hpp, orthogonalize.hpp): explicit operands.
+ * ------------------------------------------------------------------------------------------- */
+/* det_ops::MixedDensityMatrix (density_matrix.hpp:535-601), herm=true, compact:
+ * T = PsiT Phi_w, d_ov[w] += wrapped log det T, d_G[w] = T^{-T} Phi_w^T ([N][M]); a singular T
+ * gives G = 0 (getri_or_zero). d_G may be NULL (then this is det_ops::Log_Overlap,
+ * density_matrix.hpp:437-447). d_PsiT [N][M] c128, d_Phi [nb][ldw] c128. */
+int afq_det_mixed_density_matrix(int nb, int M, int N, const double* d_PsiT,
+                                 const double* d_Phi, long ldw, double* d_G, double* d_ov,
+                                 void* stream);
+/* det_ops::detail::propagate_impl<'N'> (propagate.hpp:241-298) with a dense P1 [M][M] and
+ * V [nb][M][M]: Phi <- P1 exp_taylor(iV, order) P1 Phi. */
+int afq_det_propagate(int nb, int M, int N, double* d_Phi, long ldw, const double* d_P1,
+                      const double* d_V, int order, void* stream);
+/* det_ops::orthogonalize (orthogonalize.hpp:131-169): Phi <- Q diag(sign Re R_ii);
+ * d_logdetR[w] += sum log(|R_ii|) (log_determinant_from_geqrf); d_logdetR may be NULL. */
+int afq_det_orthogonalize(int nb, int M, int N, double* d_Phi, long ldw, double* d_logdetR,
+                          void* stream);
+
+/* ---------------------------------------------------------------------------------------------
+ * Wavefunction / HamiltonianOperations level (NOMSD.cpp, Real3IndexFactorization.hpp).
+ * ------------------------------------------------------------------------------------------- */
+/* NOMSD::MixedDensityMatrix, CLOSED (NOMSD.cpp:219-237, NOMSD.hpp:292-331): G and
+ * ov = 2 log det(PsiT Phi_w). */
+int afq_mixed_density_matrix(afq_ham ham, int W, const double* d_wbuf, long ldw, double* d_G,
+                             double* d_ov, void* stream);
+/* NOMSD::Log_Overlap (NOMSD.cpp:280-301): ov = 2 log det(PsiT Phi_w) + log(conj c0). */
+int afq_log_overlap(afq_ham ham, int W, const double* d_wbuf, long ldw, double* d_ov,
+                    void* stream);
+/* Real3IndexFactorization::vbias (Real3IndexFactorization.hpp:328-361):
+ * v[w][n] = 2 sqrt(dt) sum_{a,k} G[w][a][k] Lnak[n][a][k]. */
+int afq_vbias(afq_ham ham, int W, const double* d_G, double* d_v, void* stream);
+/* Real3IndexFactorization::vHS (Real3IndexFactorization.hpp:282-326):
+ * V[w][i][j] = sqrt(dt) sum_n L[i][j][n] X[w][n]. */
+int afq_vhs(afq_ham ham, int W, const double* d_X, double* d_V, void* stream);
+/* Real3IndexFactorization::energy (Real3IndexFactorization.hpp:138-174, 476-590) from a compact
+ * G: E[w] = (E0 + 2 sum G haj, -sum_n tr(T_n^2), 2 sum_n tr(T_n)^2), T_n = G Lnak_n^T. */
+int afq_energy(afq_ham ham, int W, const double* d_G, double* d_E, void* stream);
+/* NOMSD::Energy (NOMSD.cpp:159-182): E and ov (= 2 log det + log conj c0) from the walkers. */
+int afq_wfn_energy(afq_ham ham, int W, const double* d_wbuf, long ldw, double* d_E,
+                   double* d_ov, void* stream);
+
+/* ---------------------------------------------------------------------------------------------
+ * Propagator level (AFQMCBasePropagator.cpp, construct_X_kernel.hpp, WalkerSetUpdate.hpp).
+ * ------------------------------------------------------------------------------------------- */
+/* assemble_X + construct_X_impl (AFQMCBasePropagator.cpp:686-733, construct_X_kernel.hpp:33-84)
+ * with N(0,1) fields d_xi [W][K] f64 in place of probit(uniform): on entry d_X holds vbias, on
+ * exit X = xi + i(clamp(vbias) - clamp(vMF)); d_HW[w] = -sum vdiff (X - vdiff/2); d_mf[w] =
+ * -Re(sum vMF^2)/2 + i sum X vMF (AFQMCBasePropagator.cpp:334-336). d_mf may be NULL. */
+int afq_construct_X(afq_ham ham, int W, double* d_X, const double* d_xi, double* d_HW,
+                    double* d_mf, void* stream);
+/* Propagate with the handle's P1 (apply_propagators, AFQMCBasePropagator.cpp:216-240). */
+int afq_propagate(afq_ham ham, int W, double* d_wbuf, long ldw, const double* d_V,
+                  void* stream);
+/* walker_update + bound_walker_weights (WalkerSetUpdate.hpp:46-218): new overlaps d_ov [W],
+ * energies d_E [W][3] (read when hybrid == 0), d_mf, d_HW [W]; target population = W. */
+int afq_walker_update(afq_ham ham, int W, double* d_wbuf, long ldw, const double* d_ov,
+                      const double* d_E, const double* d_mf, const double* d_HW, double Eshift,
+                      int hybrid, void* stream);
+/* AFQMCBasePropagator::Propagate (AFQMCBasePropagator.cpp:289-392): one fused step of all W
+ * walkers: G -> vbias -> X -> vHS -> P1 exp(iV) P1 -> overlap (hybrid) or energy -> weights.
+ * d_xi [W][K] f64 holds the step's N(0,1) fields. Allocation-free, graph-capturable. */
+int afq_step(afq_ham ham, int W, double* d_wbuf, long ldw, const double* d_xi, double Eshift,
+             void* stream);
+/* AFQMCBasePropagator::Orthogonalize (AFQMCBasePropagator.cpp:735-741): QR of every walker,
+ * then OVLP recomputed with NOMSD::Log_Overlap. */
+int afq_orthogonalize(afq_ham ham, int W, double* d_wbuf, long ldw, void* stream);
+/* NOMSD::Energy(wset) (NOMSD.cpp:69-80) + seedPseudoEnergy: OVLP, E1, EXX, EJ and
+ * PSEUDO_ELOC = E1+EXX+EJ of every walker (execute.cpp:172-184). */
+int afq_seed_properties(afq_ham ham, int W, double* d_wbuf, long ldw, void* stream);
+/* Weighted sums for the estimators (EnergyEstimator.h:51-100, average_energy.hpp:25-63):
+ * d_out[0] = sum w (E1+EXX+EJ), [1] = sum w, [2] = sum w exp(dt PSEUDO_ELOC), [3] = sum w
+ * PSEUDO_ELOC; d_out is c128 [4]. Deterministic order. */
+int afq_weighted_sums(afq_ham ham, int W, const double* d_wbuf, long ldw, double* d_out,
+                      void* stream);
+
+/* Region timers (the AFQMCTimers regions of AFQMCTimer.h:25-63, timed on the GPU): while
+ * enabled, afq_step and afq_orthogonalize record a pair of HIP events on their stream around
+ * each region, up to max_records regions (0 disables). afq_ham_read_timers waits for the
+ * recorded events, returns per region the summed milliseconds h_ms[AFQ_NUM_REGIONS] and the
+ * number of instances h_counts[AFQ_NUM_REGIONS], and resets them. Region names:
+ * afq_timer_name(i): G_for_vbias, vbias, assemble_X, vHS, propagate, overlap_G_E1, exchange,
+ * walker_update, ortho. */
+#define AFQ_NUM_REGIONS 9
+int afq_ham_enable_timers(afq_ham ham, int max_records);
+int afq_ham_read_timers(afq_ham ham, double* h_ms, long long* h_counts);
+const char* afq_timer_name(int region);
+
+/* Lane layout of the MFMA 16x16x4 builtins (kind 0: FP64, 1: FP32), for a self-check:
+ * D = A B with A[i][k] = i + 100 k and B[k][j] = (k == j % 4); d_out [64][4] f64 receives
+ * accumulator element r of every lane. */
+int afq_probe_mfma(int kind, double* d_out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SAFIRE_AMD_AFQ_H */

@@ -252,8 +252,11 @@ class System:
                                            _p(self._P), E0, dt)
 
     def __del__(self):
-        if getattr(self, "ptr", None):
-            lib().orc_system_destroy(self.ptr)
+        if getattr(self, "ptr", None) and _lib is not None:
+            try:
+                _lib.orc_system_destroy(self.ptr)
+            except Exception:
+                pass
             self.ptr = None
 
     @property

@@ -1,0 +1,307 @@
+"""Python mirror of the walker-step interface over the C ABI (include/afq.h).
+
+Names follow the reference's classes: HamiltonianOperations (Real3IndexFactorization.hpp), the
+det_ops free functions (SlaterDeterminantOperations/*.hpp), WalkerSet (WalkerSetBase.h) and
+Propagator (AFQMCBasePropagator.h). torch supplies device memory and the current HIP stream;
+every computation runs in libsafire_afq.so. There is no CPU fallback: without the library (or a
+GPU) the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsafire_afq.so")
+
+CLOSED, COLLINEAR = 1, 2
+FP64, FP32, BF16X3 = 0, 1, 2
+(P_WEIGHT, P_PHASE, P_PSEUDO_ELOC, P_E1, P_EXX, P_EJ, P_OVLP, P_LOGSCL_UP, P_LOGSCL_DN,
+ P_IS_UNITARY, P_THETA, P_COUNT) = range(12)
+_ERRORS = {1: "AFQ_EBADSHAPE", 2: "AFQ_EHIP", 3: "AFQ_ENOMEM", 4: "AFQ_EINVAL",
+           5: "AFQ_EUNSUPPORTED", 6: "AFQ_ECAPACITY"}
+
+
+class AfqError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{_ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class StepParams(C.Structure):
+    """afq_step_params: PropagatorParameters (parameters.hpp:103-126)."""
+    _fields_ = [("taylor_n", C.c_int), ("vbias_bound", C.c_double),
+                ("upper_cutoff_scale", C.c_double), ("lower_cutoff_scale", C.c_double),
+                ("weight_bound_floor", C.c_double), ("weight_bound_fraction", C.c_double),
+                ("hybrid", C.c_int), ("free_projection", C.c_int),
+                ("use_cp_constraint", C.c_int), ("project_force_bias", C.c_int)]
+
+
+# Exported symbols of include/afq.h and their argument types.
+_vp, _i, _l, _d = C.c_void_p, C.c_int, C.c_long, C.c_double
+SIGNATURES = {
+    "afq_default_params": (None, [_vp]),
+    "afq_last_error": (C.c_char_p, []),
+    "afq_version": (_i, []),
+    "afq_ham_create": (_i, [_i, _i, _i, _vp, _vp, _vp, _d, _d, _i, _vp]),
+    "afq_ham_destroy": (_i, [_vp]),
+    "afq_ham_reserve": (_i, [_vp, _i]),
+    "afq_ham_set_precision": (_i, [_vp, _i]),
+    "afq_ham_set_params": (_i, [_vp, _vp]),
+    "afq_ham_get_params": (_i, [_vp, _vp]),
+    "afq_ham_tensors": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "afq_ham_stats": (_i, [_vp, _vp, _vp]),
+    "afq_det_mixed_density_matrix": (_i, [_i, _i, _i, _vp, _vp, _l, _vp, _vp, _vp]),
+    "afq_det_propagate": (_i, [_i, _i, _i, _vp, _l, _vp, _vp, _i, _vp]),
+    "afq_det_orthogonalize": (_i, [_i, _i, _i, _vp, _l, _vp, _vp]),
+    "afq_mixed_density_matrix": (_i, [_vp, _i, _vp, _l, _vp, _vp, _vp]),
+    "afq_log_overlap": (_i, [_vp, _i, _vp, _l, _vp, _vp]),
+    "afq_vbias": (_i, [_vp, _i, _vp, _vp, _vp]),
+    "afq_vhs": (_i, [_vp, _i, _vp, _vp, _vp]),
+    "afq_energy": (_i, [_vp, _i, _vp, _vp, _vp]),
+    "afq_wfn_energy": (_i, [_vp, _i, _vp, _l, _vp, _vp, _vp]),
+    "afq_construct_X": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp]),
+    "afq_propagate": (_i, [_vp, _i, _vp, _l, _vp, _vp]),
+    "afq_walker_update": (_i, [_vp, _i, _vp, _l, _vp, _vp, _vp, _vp, _d, _i, _vp]),
+    "afq_step": (_i, [_vp, _i, _vp, _l, _vp, _d, _vp]),
+    "afq_orthogonalize": (_i, [_vp, _i, _vp, _l, _vp]),
+    "afq_seed_properties": (_i, [_vp, _i, _vp, _l, _vp]),
+    "afq_weighted_sums": (_i, [_vp, _i, _vp, _l, _vp, _vp]),
+    "afq_ham_enable_timers": (_i, [_vp, _i]),
+    "afq_ham_read_timers": (_i, [_vp, _vp, _vp]),
+    "afq_timer_name": (C.c_char_p, [_i]),
+    "afq_probe_mfma": (_i, [_i, _vp, _vp]),
+}
+NUM_REGIONS = 9
+
+_lib = None
+
+
+def lib():
+    """Loads libsafire_afq.so; raises if it was not built (no fallback path exists)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        raise AfqError(rc, lib().afq_last_error().decode())
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("safire_amd needs a GPU (torch.cuda.is_available() is False)")
+    return torch
+
+
+def stream_ptr(stream=None):
+    torch = _torch()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)
+
+
+def ptr(t):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def default_params() -> StepParams:
+    p = StepParams()
+    lib().afq_default_params(C.byref(p))
+    return p
+
+
+def to_device(a, dtype=None):
+    torch = _torch()
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.to("cuda")
+
+
+class HamiltonianOperations:
+    """Real3IndexFactorization + closed-shell NOMSD trial + one-body propagator of one system."""
+
+    def __init__(self, h, Likn, PsiT, E0=0.0, dt=0.005, walker_type=CLOSED):
+        _torch()
+        self.h = np.ascontiguousarray(h, np.float64)
+        self.Likn = np.ascontiguousarray(Likn, np.float64)
+        self.PsiT = np.ascontiguousarray(PsiT, np.complex128)
+        self.M = self.h.shape[0]
+        self.K = self.Likn.shape[2]
+        self.N = self.PsiT.shape[0]
+        self.dt, self.E0 = dt, E0
+        handle = C.c_void_p()
+        check(lib().afq_ham_create(self.M, self.N, self.K, self.h.ctypes.data,
+                                   self.Likn.ctypes.data, self.PsiT.ctypes.data, E0, dt,
+                                   walker_type, C.byref(handle)))
+        self.handle = handle
+        self.capacity = 0
+
+    def close(self):
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            lib().afq_ham_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def ldw(self):
+        return self.M * self.N + P_COUNT
+
+    def reserve(self, W):
+        check(lib().afq_ham_reserve(self.handle, W))
+        self.capacity = max(self.capacity, W)
+
+    def set_precision(self, precision):
+        check(lib().afq_ham_set_precision(self.handle, precision))
+
+    def set_params(self, **kw):
+        p = self.get_params()
+        for k, v in kw.items():
+            setattr(p, k, v)
+        check(lib().afq_ham_set_params(self.handle, C.byref(p)))
+
+    def get_params(self) -> StepParams:
+        p = StepParams()
+        check(lib().afq_ham_get_params(self.handle, C.byref(p)))
+        return p
+
+    def tensors(self):
+        M, N, K = self.M, self.N, self.K
+        out = dict(haj=np.zeros((N, M), np.complex128), Lnak=np.zeros((K, N, M), np.complex128),
+                   v0=np.zeros((M, M)), vMF=np.zeros(K, np.complex128),
+                   P1=np.zeros((M, M), np.complex128), H1=np.zeros((M, M), np.complex128))
+        check(lib().afq_ham_tensors(self.handle, *[out[k].ctypes.data for k in
+                                                   ("haj", "Lnak", "v0", "vMF", "P1", "H1")]))
+        return out
+
+    def enable_timers(self, max_records=4096):
+        check(lib().afq_ham_enable_timers(self.handle, max_records))
+
+    def read_timers(self):
+        """{region: (total ms, instances)} since the last read (HIP events on the step's stream)."""
+        ms = np.zeros(NUM_REGIONS)
+        cnt = np.zeros(NUM_REGIONS, np.int64)
+        check(lib().afq_ham_read_timers(self.handle, ms.ctypes.data, cnt.ctypes.data))
+        return {lib().afq_timer_name(i).decode(): (float(ms[i]), int(cnt[i]))
+                for i in range(NUM_REGIONS)}
+
+    def stats(self):
+        e = np.zeros(3, np.int64)
+        w = np.zeros(2, np.int64)
+        check(lib().afq_ham_stats(self.handle, e.ctypes.data, w.ctypes.data))
+        return e, w
+
+    # Real3IndexFactorization interface
+    def vbias(self, G, v, stream=None):
+        check(lib().afq_vbias(self.handle, G.shape[0], ptr(G), ptr(v), stream_ptr(stream)))
+
+    def vHS(self, X, V, stream=None):
+        check(lib().afq_vhs(self.handle, X.shape[0], ptr(X), ptr(V), stream_ptr(stream)))
+
+    def energy(self, G, E, stream=None):
+        check(lib().afq_energy(self.handle, G.shape[0], ptr(G), ptr(E), stream_ptr(stream)))
+
+
+class WalkerSet:
+    """walker_buffer c128 [W][M*N + 11] with the reference's layout (WalkerSetBase.cpp:43-124)."""
+
+    def __init__(self, ham: HamiltonianOperations, W: int, buf=None):
+        torch = _torch()
+        self.ham, self.W = ham, W
+        if buf is None:
+            Phi0 = ham.PsiT.conj().T.reshape(-1)
+            host = np.zeros((W, ham.ldw), np.complex128)
+            host[:, :ham.M * ham.N] = Phi0
+            host[:, ham.M * ham.N + P_WEIGHT] = 1.0
+            host[:, ham.M * ham.N + P_PHASE] = 1.0
+            host[:, ham.M * ham.N + P_IS_UNITARY] = 1.0
+            buf = host
+        self.buf = torch.from_numpy(np.ascontiguousarray(buf, np.complex128)).to("cuda")
+        if ham.capacity < W:
+            ham.reserve(W)
+
+    @property
+    def ldw(self):
+        return self.buf.shape[1]
+
+    def SlaterMatrices(self):
+        M, N = self.ham.M, self.ham.N
+        return self.buf[:, :M * N].reshape(self.W, M, N)
+
+    def getProperty(self, p):
+        return self.buf[:, self.ham.M * self.ham.N + p]
+
+    def host(self):
+        return self.buf.cpu().numpy()
+
+
+class det_ops:
+    """SlaterDeterminantOperations free functions on device tensors."""
+
+    @staticmethod
+    def MixedDensityMatrix(PsiT, Phi, G, ov, ldw=None, stream=None):
+        nb, M, N = Phi.shape
+        ld = ldw if ldw is not None else M * N
+        check(lib().afq_det_mixed_density_matrix(nb, M, N, ptr(PsiT), ptr(Phi), ld, ptr(G),
+                                                 ptr(ov), stream_ptr(stream)))
+
+    @staticmethod
+    def Log_Overlap(PsiT, Phi, ov, stream=None):
+        nb, M, N = Phi.shape
+        check(lib().afq_det_mixed_density_matrix(nb, M, N, ptr(PsiT), ptr(Phi), M * N, None,
+                                                 ptr(ov), stream_ptr(stream)))
+
+    @staticmethod
+    def Propagate(Phi, P1, V, order=6, stream=None):
+        nb, M, N = Phi.shape
+        check(lib().afq_det_propagate(nb, M, N, ptr(Phi), M * N, ptr(P1), ptr(V), order,
+                                      stream_ptr(stream)))
+
+    @staticmethod
+    def orthogonalize(Phi, logdetR=None, stream=None):
+        nb, M, N = Phi.shape
+        check(lib().afq_det_orthogonalize(nb, M, N, ptr(Phi), M * N, ptr(logdetR),
+                                          stream_ptr(stream)))
+
+
+class Propagator:
+    """AFQMCBasePropagator over a WalkerSet: Propagate (one fused step) and Orthogonalize."""
+
+    def __init__(self, ham: HamiltonianOperations):
+        self.ham = ham
+
+    def Propagate(self, wset: WalkerSet, xi, Eshift: float, stream=None):
+        check(lib().afq_step(self.ham.handle, wset.W, ptr(wset.buf), wset.ldw, ptr(xi),
+                             float(Eshift), stream_ptr(stream)))
+
+    def Orthogonalize(self, wset: WalkerSet, stream=None):
+        check(lib().afq_orthogonalize(self.ham.handle, wset.W, ptr(wset.buf), wset.ldw,
+                                      stream_ptr(stream)))
+
+    def seed(self, wset: WalkerSet, stream=None):
+        check(lib().afq_seed_properties(self.ham.handle, wset.W, ptr(wset.buf), wset.ldw,
+                                        stream_ptr(stream)))
+
+    def weighted_sums(self, wset: WalkerSet, stream=None):
+        torch = _torch()
+        out = torch.zeros(4, dtype=torch.complex128, device="cuda")
+        check(lib().afq_weighted_sums(self.ham.handle, wset.W, ptr(wset.buf), wset.ldw, ptr(out),
+                                      stream_ptr(stream)))
+        return out

@@ -1,0 +1,560 @@
+// C ABI of the walker-step library (declared in include/afq.h). Host C++: argument checks,
+// device memory of a system, and the launch sequence of one step. Every entry point converts
+// failures into an afq_status; the message is kept per thread for afq_last_error().
+#include "afq_internal.hpp"
+
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <vector>
+
+namespace afq {
+
+namespace {
+thread_local std::string g_err;
+
+template <typename F>
+int guard(F&& f) {
+  try {
+    f();
+    return AFQ_OK;
+  } catch (const Error& e) {
+    g_err = e.msg;
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_err = "out of host memory";
+    return AFQ_ENOMEM;
+  } catch (...) {
+    g_err = "unknown error";
+    return AFQ_EINVAL;
+  }
+}
+
+inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+template <typename T>
+void dalloc(T** p, size_t n) {
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(p), n * sizeof(T));
+  if (e != hipSuccess)
+    throw Error{AFQ_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e)};
+}
+
+template <typename T>
+void dfree(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+void free_workspace(Workspace& ws) {
+  dfree(ws.G);
+  dfree(ws.X);
+  dfree(ws.V);
+  dfree(ws.HW);
+  dfree(ws.mf);
+  dfree(ws.ov);
+  dfree(ws.E);
+  dfree(ws.Kl);
+  dfree(ws.exx);
+  ws.cap = 0;
+}
+
+void check_ham(afq_ham H) { AFQ_CHECK(H != nullptr, AFQ_EINVAL, "null afq_ham"); }
+
+void check_walkers(afq_ham H, int W, long ldw) {
+  AFQ_CHECK(W >= 0, AFQ_EBADSHAPE, "negative walker count");
+  AFQ_CHECK(W <= H->ws.cap, AFQ_ECAPACITY,
+            "more walkers than afq_ham_reserve() sized the workspace for");
+  AFQ_CHECK(ldw >= (long)H->M * H->N + AFQ_P_COUNT, AFQ_EBADSHAPE,
+            "walker stride smaller than M*N + 11");
+}
+
+cd ci_log(afq_ham H) {
+  const double a = std::hypot(H->ci0_re, H->ci0_im);
+  if (a < 1e-12) return cmk(std::log(1e-12), 0.0);
+  return clog_(cmk(H->ci0_re, -H->ci0_im));
+}
+
+const void* L_for(afq_ham H) { return H->precision == AFQ_FP64 ? (const void*)H->Lnak : (const void*)H->Lnak_f; }
+const void* Likn_for(afq_ham H) { return H->precision == AFQ_FP64 ? (const void*)H->Likn : (const void*)H->Likn_f; }
+double scl(afq_ham H) { return H->walker_type == AFQ_CLOSED ? 2.0 : 1.0; }
+
+// G, ov (= 2 logdet + log conj c0) and E (E1 by the Green's function kernel, EXX/EJ by the
+// exchange kernel) of all walkers into the workspace (or E/ov given)
+void energy_from_walkers(afq_ham H, hipStream_t s, int W, const double* wbuf, long ldw,
+                         double* E, double* ov) {
+  const int M = H->M, N = H->N, K = H->K;
+  launch_green(s, W, M, N, H->PsiT, wbuf, ldw, H->ws.G, ov, 1, scl(H), ci_log(H), H->haj,
+               H->E0, scl(H), E, 3);
+  launch_exchange(s, H->precision, W, M, N, K, H->ws.G, L_for(H), H->ws.Kl, H->ws.exx);
+  launch_energy_finalize(s, W, K, exx_nchunks(K), H->walker_type, H->ws.Kl, H->ws.exx, E);
+}
+
+const char* const kRegionNames[R_COUNT] = {"G_for_vbias", "vbias",          "assemble_X",
+                                           "vHS",         "propagate",      "overlap_G_E1",
+                                           "exchange",    "walker_update",  "ortho"};
+
+void destroy_timers(Timers& t) {
+  for (hipEvent_t e : t.ev) (void)hipEventDestroy(e);
+  t.ev.clear();
+  t.region.clear();
+  t.cap = t.used = 0;
+  t.on = false;
+}
+
+}  // namespace
+
+RegionScope::RegionScope(Timers& t_, hipStream_t s_, int r) : t(t_), s(s_) {
+  if (t.on && t.used < t.cap) {
+    rec = t.used++;
+    t.region[rec] = r;
+    (void)hipEventRecord(t.ev[2 * rec], s);
+  }
+}
+RegionScope::~RegionScope() {
+  if (rec >= 0) (void)hipEventRecord(t.ev[2 * rec + 1], s);
+}
+
+void set_error(const std::string& msg) { g_err = msg; }
+
+}  // namespace afq
+
+using namespace afq;
+
+extern "C" {
+
+void afq_default_params(afq_step_params* p) {
+  if (!p) return;
+  p->taylor_n = 6;
+  p->vbias_bound = 50.0;
+  p->upper_cutoff_scale = 10.0;
+  p->lower_cutoff_scale = 1.0;
+  p->weight_bound_floor = 100.0;
+  p->weight_bound_fraction = 0.1;
+  p->hybrid = 1;
+  p->free_projection = 0;
+  p->use_cp_constraint = 0;
+  p->project_force_bias = 0;
+}
+
+const char* afq_last_error(void) { return g_err.c_str(); }
+int afq_version(void) { return 1; }
+
+int afq_ham_create(int M, int N, int K, const double* h_h, const double* h_Likn,
+                   const double* h_PsiT, double E0, double dt, int walker_type, afq_ham* out) {
+  return guard([&] {
+    AFQ_CHECK(out != nullptr, AFQ_EINVAL, "null output handle");
+    *out = nullptr;
+    AFQ_CHECK(M > 0 && N > 0 && K > 0 && N <= M, AFQ_EBADSHAPE, "bad (M, N, K)");
+    AFQ_CHECK(h_h && h_Likn && h_PsiT, AFQ_EINVAL, "null input array");
+    AFQ_CHECK(walker_type == AFQ_CLOSED, AFQ_EUNSUPPORTED,
+              "only CLOSED walkers are supported by afq_ham_create");
+    AFQ_CHECK(N <= 32, AFQ_EUNSUPPORTED, "more than 32 electrons per spin");
+    AFQ_CHECK(dt > 0, AFQ_EINVAL, "dt must be positive");
+    afq_ham H = new afq_ham_s();
+    try {
+      H->M = M;
+      H->N = N;
+      H->K = K;
+      H->dt = dt;
+      H->E0 = E0;
+      H->walker_type = walker_type;
+      afq_default_params(&H->params);
+      AFQ_HIP(hipGetDevice(&H->device));
+      const size_t MM = (size_t)M * M, NM = (size_t)N * M;
+      dalloc(&H->PsiT, 2 * NM);
+      dalloc(&H->haj, 2 * NM);
+      dalloc(&H->Lnak, 2 * NM * K);
+      dalloc(&H->Likn, MM * K);
+      dalloc(&H->h, MM);
+      dalloc(&H->v0, MM);
+      dalloc(&H->vMF, 2 * (size_t)K);
+      dalloc(&H->P1, 2 * MM);
+      dalloc(&H->H1, 2 * MM);
+      dalloc(&H->Lnak_f, 2 * NM * K);
+      dalloc(&H->Likn_f, MM * K);
+      dalloc(&H->stats, 5);
+      AFQ_HIP(hipMemcpy(H->PsiT, h_PsiT, 2 * NM * sizeof(double), hipMemcpyHostToDevice));
+      AFQ_HIP(hipMemcpy(H->Likn, h_Likn, MM * K * sizeof(double), hipMemcpyHostToDevice));
+      AFQ_HIP(hipMemcpy(H->h, h_h, MM * sizeof(double), hipMemcpyHostToDevice));
+      AFQ_HIP(hipMemset(H->stats, 0, 5 * sizeof(unsigned long long)));
+      setup_hamiltonian(H, nullptr);
+      AFQ_HIP(hipDeviceSynchronize());
+    } catch (...) {
+      afq_ham_destroy(H);
+      throw;
+    }
+    *out = H;
+  });
+}
+
+int afq_ham_destroy(afq_ham H) {
+  return guard([&] {
+    if (!H) return;
+    destroy_timers(H->timers);
+    free_workspace(H->ws);
+    dfree(H->PsiT);
+    dfree(H->haj);
+    dfree(H->Lnak);
+    dfree(H->Likn);
+    dfree(H->h);
+    dfree(H->v0);
+    dfree(H->vMF);
+    dfree(H->P1);
+    dfree(H->H1);
+    dfree(H->Lnak_f);
+    dfree(H->Likn_f);
+    dfree(H->stats);
+    delete H;
+  });
+}
+
+int afq_ham_reserve(afq_ham H, int max_walkers) {
+  return guard([&] {
+    check_ham(H);
+    AFQ_CHECK(max_walkers >= 0, AFQ_EBADSHAPE, "negative walker count");
+    if (max_walkers <= H->ws.cap) return;
+    AFQ_HIP(hipDeviceSynchronize());
+    free_workspace(H->ws);
+    const size_t W = max_walkers;
+    const size_t M = H->M, N = H->N, K = H->K;
+    Workspace& ws = H->ws;
+    dalloc(&ws.G, 2 * W * N * M);
+    dalloc(&ws.X, 2 * W * K);
+    dalloc(&ws.V, 2 * W * M * M);
+    dalloc(&ws.HW, 2 * W);
+    dalloc(&ws.mf, 2 * W);
+    dalloc(&ws.ov, 2 * W);
+    dalloc(&ws.E, 6 * W);
+    dalloc(&ws.Kl, 2 * W * K);
+    dalloc(&ws.exx, 2 * W * (size_t)exx_nchunks(H->K));
+    ws.cap = max_walkers;
+  });
+}
+
+int afq_ham_set_precision(afq_ham H, int precision) {
+  return guard([&] {
+    check_ham(H);
+    AFQ_CHECK(precision == AFQ_FP64 || precision == AFQ_FP32, AFQ_EUNSUPPORTED,
+              "precision not available");
+    H->precision = precision;
+  });
+}
+
+int afq_ham_set_params(afq_ham H, const afq_step_params* p) {
+  return guard([&] {
+    check_ham(H);
+    AFQ_CHECK(p != nullptr, AFQ_EINVAL, "null params");
+    AFQ_CHECK(p->taylor_n >= 1, AFQ_EINVAL, "taylor_n must be >= 1");
+    H->params = *p;
+  });
+}
+
+int afq_ham_get_params(afq_ham H, afq_step_params* p) {
+  return guard([&] {
+    check_ham(H);
+    AFQ_CHECK(p != nullptr, AFQ_EINVAL, "null params");
+    *p = H->params;
+  });
+}
+
+int afq_ham_tensors(afq_ham H, double* haj, double* Lnak, double* v0, double* vMF, double* P1,
+                    double* H1) {
+  return guard([&] {
+    check_ham(H);
+    const size_t MM = (size_t)H->M * H->M, NM = (size_t)H->N * H->M;
+    AFQ_HIP(hipDeviceSynchronize());
+    if (haj) AFQ_HIP(hipMemcpy(haj, H->haj, 2 * NM * sizeof(double), hipMemcpyDeviceToHost));
+    if (Lnak)
+      AFQ_HIP(hipMemcpy(Lnak, H->Lnak, 2 * NM * H->K * sizeof(double), hipMemcpyDeviceToHost));
+    if (v0) AFQ_HIP(hipMemcpy(v0, H->v0, MM * sizeof(double), hipMemcpyDeviceToHost));
+    if (vMF) AFQ_HIP(hipMemcpy(vMF, H->vMF, 2 * (size_t)H->K * sizeof(double), hipMemcpyDeviceToHost));
+    if (P1) AFQ_HIP(hipMemcpy(P1, H->P1, 2 * MM * sizeof(double), hipMemcpyDeviceToHost));
+    if (H1) AFQ_HIP(hipMemcpy(H1, H->H1, 2 * MM * sizeof(double), hipMemcpyDeviceToHost));
+  });
+}
+
+int afq_ham_stats(afq_ham H, long long* eloc3, long long* weight2) {
+  return guard([&] {
+    check_ham(H);
+    unsigned long long st[5];
+    AFQ_HIP(hipDeviceSynchronize());
+    AFQ_HIP(hipMemcpy(st, H->stats, sizeof(st), hipMemcpyDeviceToHost));
+    if (eloc3)
+      for (int i = 0; i < 3; ++i) eloc3[i] = (long long)st[i];
+    if (weight2)
+      for (int i = 0; i < 2; ++i) weight2[i] = (long long)st[3 + i];
+  });
+}
+
+// ------------------------------------------------------------------ det_ops level
+int afq_det_mixed_density_matrix(int nb, int M, int N, const double* PsiT, const double* Phi,
+                                 long ldw, double* G, double* ov, void* stream) {
+  return guard([&] {
+    AFQ_CHECK(nb >= 0 && M > 0 && N >= 0 && N <= M, AFQ_EBADSHAPE, "bad shape");
+    AFQ_CHECK(ldw >= (long)M * N || nb <= 1, AFQ_EBADSHAPE, "walker stride smaller than M*N");
+    AFQ_CHECK(ov != nullptr, AFQ_EINVAL, "null overlap array");
+    launch_green(S(stream), nb, M, N, PsiT, Phi, ldw, G, ov, 2, 1.0, cmk(0, 0), nullptr, 0.0,
+                 0.0, nullptr, 0);
+  });
+}
+
+int afq_det_propagate(int nb, int M, int N, double* Phi, long ldw, const double* P1,
+                      const double* V, int order, void* stream) {
+  return guard([&] {
+    AFQ_CHECK(nb >= 0 && M > 0 && N >= 0 && N <= M, AFQ_EBADSHAPE, "bad shape");
+    AFQ_CHECK(order >= 0, AFQ_EINVAL, "negative Taylor order");
+    launch_propagate(S(stream), nb, M, N, Phi, ldw, P1, V, order);
+  });
+}
+
+int afq_det_orthogonalize(int nb, int M, int N, double* Phi, long ldw, double* logdetR,
+                          void* stream) {
+  return guard([&] {
+    AFQ_CHECK(nb >= 0 && M > 0 && N >= 0 && N <= M, AFQ_EBADSHAPE, "bad shape");
+    launch_orthogonalize(S(stream), nb, M, N, Phi, ldw, logdetR);
+  });
+}
+
+// ------------------------------------------------------------------ wavefunction level
+int afq_mixed_density_matrix(afq_ham H, int W, const double* wbuf, long ldw, double* G,
+                             double* ov, void* stream) {
+  return guard([&] {
+    check_ham(H);
+    check_walkers(H, W, ldw);
+    launch_green(S(stream), W, H->M, H->N, H->PsiT, wbuf, ldw, G, ov, ov ? 1 : 0, scl(H),
+                 cmk(0, 0), nullptr, 0.0, 0.0, nullptr, 0);
+  });
+}
+
+int afq_log_overlap(afq_ham H, int W, const double* wbuf, long ldw, double* ov, void* stream) {
+  return guard([&] {
+    check_ham(H);
+    check_walkers(H, W, ldw);
+    AFQ_CHECK(ov != nullptr, AFQ_EINVAL, "null overlap array");
+    launch_green(S(stream), W, H->M, H->N, H->PsiT, wbuf, ldw, nullptr, ov, 1, scl(H),
+                 ci_log(H), nullptr, 0.0, 0.0, nullptr, 0);
+  });
+}
+
+int afq_vbias(afq_ham H, int W, const double* G, double* v, void* stream) {
+  return guard([&] {
+    check_ham(H);
+    AFQ_CHECK(W >= 0, AFQ_EBADSHAPE, "negative walker count");
+    const long NM = (long)H->N * H->M;
+    launch_gemm_nt_z(S(stream), H->precision, W, H->K, (int)NM, std::sqrt(H->dt) * scl(H), G,
+                     NM, L_for(H), NM, true, v, H->K);
+  });
+}
+
+int afq_vhs(afq_ham H, int W, const double* X, double* V, void* stream) {
+  return guard([&] {
+    check_ham(H);
+    AFQ_CHECK(W >= 0, AFQ_EBADSHAPE, "negative walker count");
+    const int MM = H->M * H->M;
+    launch_gemm_nt_z(S(stream), H->precision, W, MM, H->K, std::sqrt(H->dt), X, H->K,
+                     Likn_for(H), H->K, false, V, MM);
+  });
+}
+
+int afq_energy(afq_ham H, int W, const double* G, double* E, void* stream) {
+  return guard([&] {
+    check_ham(H);
+    AFQ_CHECK(W <= H->ws.cap, AFQ_ECAPACITY, "more walkers than reserved");
+    launch_e1(S(stream), W, H->M, H->N, G, H->haj, H->E0, scl(H), E);
+    launch_exchange(S(stream), H->precision, W, H->M, H->N, H->K, G, L_for(H), H->ws.Kl,
+                    H->ws.exx);
+    launch_energy_finalize(S(stream), W, H->K, exx_nchunks(H->K), H->walker_type, H->ws.Kl,
+                           H->ws.exx, E);
+  });
+}
+
+int afq_wfn_energy(afq_ham H, int W, const double* wbuf, long ldw, double* E, double* ov,
+                   void* stream) {
+  return guard([&] {
+    check_ham(H);
+    check_walkers(H, W, ldw);
+    energy_from_walkers(H, S(stream), W, wbuf, ldw, E, ov);
+  });
+}
+
+// ------------------------------------------------------------------ propagator level
+int afq_construct_X(afq_ham H, int W, double* X, const double* xi, double* HW, double* mf,
+                    void* stream) {
+  return guard([&] {
+    check_ham(H);
+    AFQ_CHECK(W >= 0, AFQ_EBADSHAPE, "negative walker count");
+    const afq_step_params& p = H->params;
+    launch_construct_X(S(stream), W, H->K, p.vbias_bound,
+                       p.use_cp_constraint || p.project_force_bias, p.free_projection, H->vMF,
+                       xi, X, HW, mf);
+  });
+}
+
+int afq_propagate(afq_ham H, int W, double* wbuf, long ldw, const double* V, void* stream) {
+  return guard([&] {
+    check_ham(H);
+    check_walkers(H, W, ldw);
+    launch_propagate(S(stream), W, H->M, H->N, wbuf, ldw, H->P1, V, H->params.taylor_n);
+  });
+}
+
+int afq_walker_update(afq_ham H, int W, double* wbuf, long ldw, const double* ov,
+                      const double* E, const double* mf, const double* HW, double Eshift,
+                      int hybrid, void* stream) {
+  return guard([&] {
+    check_ham(H);
+    check_walkers(H, W, ldw);
+    launch_walker_update(S(stream), W, wbuf, ldw, (long)H->M * H->N, H->params, H->dt, Eshift,
+                         H->E0, ov, E, mf, HW, hybrid, H->stats);
+  });
+}
+
+int afq_step(afq_ham H, int W, double* wbuf, long ldw, const double* xi, double Eshift,
+             void* stream) {
+  return guard([&] {
+    check_ham(H);
+    check_walkers(H, W, ldw);
+    AFQ_CHECK(xi != nullptr, AFQ_EINVAL, "null fields");
+    hipStream_t s = S(stream);
+    const int M = H->M, N = H->N, K = H->K;
+    const long NM = (long)N * M;
+    const afq_step_params& p = H->params;
+    Workspace& ws = H->ws;
+    Timers& tm = H->timers;
+    // 1. vbias = 2 sqrt(dt) G Lnak^T   (NOMSD::vbias)
+    {
+      RegionScope r(tm, s, R_G_FOR_VBIAS);
+      launch_green(s, W, M, N, H->PsiT, wbuf, ldw, ws.G, nullptr, 0, 1.0, cmk(0, 0), nullptr,
+                   0.0, 0.0, nullptr, 0);
+    }
+    {
+      RegionScope r(tm, s, R_VBIAS);
+      launch_gemm_nt_z(s, H->precision, W, K, (int)NM, std::sqrt(H->dt) * scl(H), ws.G, NM,
+                       L_for(H), NM, true, ws.X, K);
+    }
+    // 2. X, HW, mean-field factor     (assemble_X)
+    {
+      RegionScope r(tm, s, R_ASSEMBLE_X);
+      launch_construct_X(s, W, K, p.vbias_bound, p.use_cp_constraint || p.project_force_bias,
+                         p.free_projection, H->vMF, xi, ws.X, ws.HW, ws.mf);
+    }
+    // 3. vHS = sqrt(dt) L X           4. Phi <- P1 exp(iV) P1 Phi
+    {
+      RegionScope r(tm, s, R_VHS);
+      launch_gemm_nt_z(s, H->precision, W, M * M, K, std::sqrt(H->dt), ws.X, K, Likn_for(H),
+                       K, false, ws.V, (long)M * M);
+    }
+    {
+      RegionScope r(tm, s, R_PROPAGATE);
+      launch_propagate(s, W, M, N, wbuf, ldw, H->P1, ws.V, p.taylor_n);
+    }
+    // 5. new overlap (hybrid) or local energy
+    if (p.hybrid) {
+      RegionScope r(tm, s, R_OVERLAP_G_E1);
+      launch_green(s, W, M, N, H->PsiT, wbuf, ldw, nullptr, ws.ov, 1, scl(H), ci_log(H),
+                   nullptr, 0.0, 0.0, nullptr, 0);
+    } else {
+      {
+        RegionScope r(tm, s, R_OVERLAP_G_E1);
+        launch_green(s, W, M, N, H->PsiT, wbuf, ldw, ws.G, ws.ov, 1, scl(H), ci_log(H),
+                     H->haj, H->E0, scl(H), ws.E, 3);
+      }
+      RegionScope r(tm, s, R_EXCHANGE);
+      launch_exchange(s, H->precision, W, M, N, K, ws.G, L_for(H), ws.Kl, ws.exx);
+    }
+    // 6.-7. energy assembly, weights, phases, energies, bounds
+    {
+      RegionScope r(tm, s, R_WALKER_UPDATE);
+      if (!p.hybrid)
+        launch_energy_finalize(s, W, K, exx_nchunks(K), H->walker_type, ws.Kl, ws.exx, ws.E);
+      launch_walker_update(s, W, wbuf, ldw, (long)M * N, p, H->dt, Eshift, H->E0, ws.ov, ws.E,
+                           ws.mf, ws.HW, p.hybrid, H->stats);
+    }
+  });
+}
+
+int afq_orthogonalize(afq_ham H, int W, double* wbuf, long ldw, void* stream) {
+  return guard([&] {
+    check_ham(H);
+    check_walkers(H, W, ldw);
+    hipStream_t s = S(stream);
+    RegionScope r(H->timers, s, R_ORTHO);
+    launch_orthogonalize(s, W, H->M, H->N, wbuf, ldw, nullptr);
+    launch_green(s, W, H->M, H->N, H->PsiT, wbuf, ldw, nullptr, H->ws.ov, 1, scl(H),
+                 ci_log(H), nullptr, 0.0, 0.0, nullptr, 0);
+    launch_set_properties(s, W, wbuf, ldw, (long)H->M * H->N, H->ws.ov, nullptr, 0);
+  });
+}
+
+int afq_seed_properties(afq_ham H, int W, double* wbuf, long ldw, void* stream) {
+  return guard([&] {
+    check_ham(H);
+    check_walkers(H, W, ldw);
+    hipStream_t s = S(stream);
+    energy_from_walkers(H, s, W, wbuf, ldw, H->ws.E, H->ws.ov);
+    launch_set_properties(s, W, wbuf, ldw, (long)H->M * H->N, H->ws.ov, H->ws.E, 1);
+  });
+}
+
+int afq_weighted_sums(afq_ham H, int W, const double* wbuf, long ldw, double* out,
+                      void* stream) {
+  return guard([&] {
+    check_ham(H);
+    check_walkers(H, W, ldw);
+    launch_weighted_sums(S(stream), W, wbuf, ldw, (long)H->M * H->N, H->dt, out);
+  });
+}
+
+int afq_ham_enable_timers(afq_ham H, int max_records) {
+  return guard([&] {
+    check_ham(H);
+    AFQ_CHECK(max_records >= 0, AFQ_EINVAL, "negative record count");
+    AFQ_HIP(hipDeviceSynchronize());
+    destroy_timers(H->timers);
+    for (int r = 0; r < R_COUNT; ++r) {
+      H->timers.ms[r] = 0.0;
+      H->timers.count[r] = 0;
+    }
+    if (max_records == 0) return;
+    H->timers.ev.resize(2 * (size_t)max_records);
+    for (auto& e : H->timers.ev) AFQ_HIP(hipEventCreate(&e));
+    H->timers.region.assign(max_records, 0);
+    H->timers.cap = max_records;
+    H->timers.on = true;
+  });
+}
+
+int afq_ham_read_timers(afq_ham H, double* h_ms, long long* h_counts) {
+  return guard([&] {
+    check_ham(H);
+    Timers& t = H->timers;
+    for (int i = 0; i < t.used; ++i) {
+      AFQ_HIP(hipEventSynchronize(t.ev[2 * i + 1]));
+      float ms = 0.f;
+      AFQ_HIP(hipEventElapsedTime(&ms, t.ev[2 * i], t.ev[2 * i + 1]));
+      t.ms[t.region[i]] += ms;
+      t.count[t.region[i]] += 1;
+    }
+    t.used = 0;
+    for (int r = 0; r < R_COUNT; ++r) {
+      if (h_ms) h_ms[r] = t.ms[r];
+      if (h_counts) h_counts[r] = t.count[r];
+      t.ms[r] = 0.0;
+      t.count[r] = 0;
+    }
+  });
+}
+
+const char* afq_timer_name(int region) {
+  return (region >= 0 && region < R_COUNT) ? kRegionNames[region] : "";
+}
+
+int afq_probe_mfma(int kind, double* out, void* stream) {
+  return guard([&] {
+    AFQ_CHECK(kind == 0 || kind == 1, AFQ_EINVAL, "probe kind must be 0 (FP64) or 1 (FP32)");
+    launch_probe_mfma(S(stream), kind, out);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,205 @@
+// Internal declarations of the MI355X walker-step library: device complex helpers, the MFMA
+// wrappers, the per-system engine state and the kernel launchers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/afq.h"
+
+namespace afq {
+
+// ------------------------------------------------------------------ error plumbing
+void set_error(const std::string& msg);
+struct Error {
+  int code;
+  std::string msg;
+};
+
+#define AFQ_HIP(call)                                                                       \
+  do {                                                                                      \
+    hipError_t e_ = (call);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      throw ::afq::Error{AFQ_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)};      \
+  } while (0)
+
+#define AFQ_CHECK(cond, code, msg)                                                          \
+  do {                                                                                      \
+    if (!(cond)) throw ::afq::Error{(code), (msg)};                                         \
+  } while (0)
+
+// ------------------------------------------------------------------ complex on the device
+struct cd {
+  double re, im;
+};
+
+__host__ __device__ inline cd cmk(double r, double i) { return cd{r, i}; }
+__host__ __device__ inline cd operator+(cd a, cd b) { return cd{a.re + b.re, a.im + b.im}; }
+__host__ __device__ inline cd operator-(cd a, cd b) { return cd{a.re - b.re, a.im - b.im}; }
+__host__ __device__ inline cd operator-(cd a) { return cd{-a.re, -a.im}; }
+__host__ __device__ inline cd operator*(cd a, cd b) {
+  return cd{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+__host__ __device__ inline cd operator*(double s, cd a) { return cd{s * a.re, s * a.im}; }
+__host__ __device__ inline cd& operator+=(cd& a, cd b) {
+  a.re += b.re;
+  a.im += b.im;
+  return a;
+}
+__host__ __device__ inline cd& operator-=(cd& a, cd b) {
+  a.re -= b.re;
+  a.im -= b.im;
+  return a;
+}
+__host__ __device__ inline cd cconj(cd a) { return cd{a.re, -a.im}; }
+__host__ __device__ inline double cnorm(cd a) { return a.re * a.re + a.im * a.im; }
+__host__ __device__ inline double cabs_(cd a) { return hypot(a.re, a.im); }
+__host__ __device__ inline double cabs1(cd a) { return fabs(a.re) + fabs(a.im); }
+__host__ __device__ inline bool cis0(cd a) { return a.re == 0.0 && a.im == 0.0; }
+// complex division with Smith's scaling (the -fcx-fortran-rules division of the host build)
+__host__ __device__ inline cd cdiv(cd a, cd b) {
+  if (fabs(b.re) >= fabs(b.im)) {
+    double r = b.im / b.re, d = b.re + b.im * r;
+    return cd{(a.re + a.im * r) / d, (a.im - a.re * r) / d};
+  } else {
+    double r = b.re / b.im, d = b.im + b.re * r;
+    return cd{(a.re * r + a.im) / d, (a.im * r - a.re) / d};
+  }
+}
+__host__ __device__ inline cd clog_(cd a) { return cd{log(hypot(a.re, a.im)), atan2(a.im, a.re)}; }
+__host__ __device__ inline cd cexp_(cd a) {
+  double e = exp(a.re);
+  return cd{e * cos(a.im), e * sin(a.im)};
+}
+
+// ------------------------------------------------------------------ MFMA wrappers (gfx950)
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+// D(16x16) += A(16x4) B(4x16). Lane l supplies A[l%16][l/16] and B[l/16][l%16]; accumulator
+// element r of lane l is D[acc_row<R>(l/16, r)][l%16]. The row maps were measured with the
+// probe kernel (afq_probe_mfma): FP64 holds rows g, g+4, g+8, g+12; FP32 holds rows 4g..4g+3.
+__device__ inline d4 mfma16x16x4(double a, double b, d4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ inline f4 mfma16x16x4(float a, float b, f4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+template <typename R> __host__ __device__ inline int acc_row(int g, int r);
+template <> __host__ __device__ inline int acc_row<double>(int g, int r) { return 4 * r + g; }
+template <> __host__ __device__ inline int acc_row<float>(int g, int r) { return 4 * g + r; }
+
+template <typename R> struct acc_of;
+template <> struct acc_of<double> { typedef d4 type; };
+template <> struct acc_of<float> { typedef f4 type; };
+
+// ------------------------------------------------------------------ engine state
+struct Workspace {
+  int cap = 0;
+  double* G = nullptr;       // c128 [W][N][M]
+  double* X = nullptr;       // c128 [W][K]
+  double* V = nullptr;       // c128 [W][M][M]
+  double* HW = nullptr;      // c128 [W]
+  double* mf = nullptr;      // c128 [W]
+  double* ov = nullptr;      // c128 [W]
+  double* E = nullptr;       // c128 [W][3]
+  double* Kl = nullptr;      // c128 [W][K]
+  double* exx = nullptr;     // c128 [W][nchunk]
+};
+
+// Region timers of a step (the AFQMCTimers regions of AFQMCTimer.h:25-63), as HIP events
+// recorded on the launch stream around each region; read back (and summed) on demand.
+enum Region {
+  R_G_FOR_VBIAS = 0, R_VBIAS, R_ASSEMBLE_X, R_VHS, R_PROPAGATE, R_OVERLAP_G_E1, R_EXCHANGE,
+  R_WALKER_UPDATE, R_ORTHO, R_COUNT
+};
+struct Timers {
+  bool on = false;
+  int cap = 0, used = 0;
+  std::vector<hipEvent_t> ev;      // 2 per record
+  std::vector<int> region;         // region of record i
+  double ms[R_COUNT] = {0};
+  long long count[R_COUNT] = {0};
+};
+// RAII region scope: records start/stop events when the timers are on.
+struct RegionScope {
+  Timers& t;
+  hipStream_t s;
+  int rec = -1;
+  RegionScope(Timers& t_, hipStream_t s_, int r);
+  ~RegionScope();
+};
+
+}  // namespace afq
+
+struct afq_ham_s {
+  int M = 0, N = 0, K = 0, walker_type = AFQ_CLOSED;
+  double dt = 0.005, E0 = 0.0;
+  double ci0_re = 1.0, ci0_im = 0.0;
+  int precision = AFQ_FP64;
+  afq_step_params params;
+  int device = 0;
+  // device tensors (built on the GPU at create)
+  double* PsiT = nullptr;    // c128 [N][M]
+  double* haj = nullptr;     // c128 [N][M]
+  double* Lnak = nullptr;    // c128 [K][N][M]
+  double* Likn = nullptr;    // f64 [M*M][K]
+  double* h = nullptr;       // f64 [M][M]
+  double* v0 = nullptr;      // f64 [M][M]
+  double* vMF = nullptr;     // c128 [K]
+  double* P1 = nullptr;      // c128 [M][M]
+  double* H1 = nullptr;      // c128 [M][M]
+  float* Lnak_f = nullptr;   // c64 [K][N][M]   (FP32 contraction copies)
+  float* Likn_f = nullptr;   // f32 [M*M][K]
+  unsigned long long* stats = nullptr;   // [5]: eloc total/upper/lower, weight total/upper
+  afq::Workspace ws;
+  afq::Timers timers;
+};
+
+namespace afq {
+
+// ------------------------------------------------------------------ launchers
+// k_green.hip
+void launch_green(hipStream_t s, int nb, int M, int N, const double* PsiT, const double* Phi,
+                  long ldw, double* G, double* ov, int ov_mode, double ov_scale, cd ov_add,
+                  const double* haj, double E0, double e1_scale, double* E, long ldE);
+// k_gemm.hip:  C[m][n] = alpha * sum_k A[m][k] B[n][k]; A complex, B complex or real
+void launch_gemm_nt_z(hipStream_t s, int precision, int m, int n, int k, double alpha,
+                      const double* A, long lda, const void* B, long ldb, bool b_complex,
+                      double* C, long ldc);
+// k_energy.hip
+int exx_nchunks(int K);
+void launch_exchange(hipStream_t s, int precision, int W, int M, int N, int K, const double* G,
+                     const void* Lnak, double* Kl, double* exx_part);
+void launch_energy_finalize(hipStream_t s, int W, int K, int nchunk, int walker_type,
+                            const double* Kl, const double* exx_part, double* E);
+void launch_e1(hipStream_t s, int W, int M, int N, const double* G, const double* haj,
+               double E0, double scl, double* E);
+// k_propagate.hip
+void launch_propagate(hipStream_t s, int nb, int M, int N, double* Phi, long ldw,
+                      const double* P1, const double* V, int order);
+// k_qr.hip
+void launch_orthogonalize(hipStream_t s, int nb, int M, int N, double* Phi, long ldw,
+                          double* logdetR);
+// k_walker.hip
+void launch_construct_X(hipStream_t s, int W, int K, double vbias_bound, int project,
+                        int free_projection, const double* vMF, const double* xi, double* X,
+                        double* HW, double* mf);
+void launch_walker_update(hipStream_t s, int W, double* wbuf, long ldw, long poff,
+                          const afq_step_params& p, double dt, double Eshift, double E0,
+                          const double* ov, const double* E, const double* mf,
+                          const double* HW, int hybrid, unsigned long long* stats);
+void launch_set_properties(hipStream_t s, int W, double* wbuf, long ldw, long poff,
+                           const double* ov, const double* E, int seed_pseudo);
+void launch_weighted_sums(hipStream_t s, int W, const double* wbuf, long ldw, long poff,
+                          double dt, double* out);
+void launch_probe_mfma(hipStream_t s, int kind, double* out);
+// k_setup.hip
+void setup_hamiltonian(afq_ham_s* H, hipStream_t s);
+
+}  // namespace afq

@@ -1,0 +1,248 @@
+// Per-walker scalar work of a step, kept on the GPU (the reference stages the walker properties
+// to the host every step, WalkerSetUpdate.hpp:89-100, 174-183):
+//   k_construct_X    construct_X_impl (construct_X_kernel.hpp:33-84) + meanfield factor
+//                    (AFQMCBasePropagator.cpp:334-336); one workgroup per walker, fixed-order sums
+//   k_walker_update  walker_update + bound_local_energy + bound_walker_weights
+//                    (WalkerSetUpdate.hpp:46-218); one thread per walker
+//   k_set_properties OVLP/E1/EXX/EJ/PSEUDO_ELOC seeding (NOMSD.cpp:69-80, execute.cpp:172-184)
+//   k_weighted_sums  the sums of EnergyEstimator::measure and averagePseudoEnergy
+#include "afq_internal.hpp"
+
+#include <cfloat>
+
+namespace afq {
+
+namespace {
+
+__device__ inline cd clamp_bound(cd z, double bound) {
+  const double a = cabs_(z);
+  if (a > bound) return (bound / a) * z;   // z/|z| * bound
+  return z;
+}
+
+__global__ __launch_bounds__(256) void k_construct_X(int W, int K, double bound, int project,
+                                                     int free_projection,
+                                                     const cd* __restrict__ vMF,
+                                                     const double* __restrict__ xi,
+                                                     cd* __restrict__ X, cd* __restrict__ HW,
+                                                     cd* __restrict__ mf) {
+  __shared__ double red[4][6];
+  const int w = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  cd hw = cmk(0, 0), xv = cmk(0, 0);
+  double v2 = 0.0;
+  for (int m = tid; m < K; m += 256) {
+    const cd vmf0 = vMF[m];
+    const cd vmf_t = clamp_bound(vmf0, bound);
+    cd vb_t = clamp_bound(X[(long)w * K + m], bound);
+    if (project) vb_t = cmk(vb_t.re, 0.0);
+    const cd d = vb_t - vmf_t;
+    const cd vdiff = free_projection ? cmk(0, 0) : cmk(-d.im, d.re);   // i (vb - vmf)
+    const cd x = cmk(xi[(long)w * K + m], 0.0) + vdiff;
+    X[(long)w * K + m] = x;
+    hw -= vdiff * (x - 0.5 * vdiff);
+    xv += x * vmf0;
+    v2 += (vmf0 * vmf0).re;
+  }
+  double v[6] = {hw.re, hw.im, xv.re, xv.im, v2, 0.0};
+#pragma unroll
+  for (int q = 0; q < 5; ++q)
+    for (int off = 32; off > 0; off >>= 1) v[q] += __shfl_xor(v[q], off, 64);
+  if (lane == 0)
+#pragma unroll
+    for (int q = 0; q < 5; ++q) red[wave][q] = v[q];
+  __syncthreads();
+  if (tid == 0) {
+    double s[5] = {0, 0, 0, 0, 0};
+    for (int qw = 0; qw < 4; ++qw)
+      for (int q = 0; q < 5; ++q) s[q] += red[qw][q];
+    HW[w] = cmk(s[0], s[1]);
+    if (mf) mf[w] = cmk(-0.5 * s[4] - s[3], s[2]);   // -Re(sum vMF^2)/2 + i sum X vMF
+  }
+}
+
+__global__ void k_walker_update(int W, cd* __restrict__ wbuf, long ldw, long poff, int hybrid,
+                                int free_projection, int use_cp, double dt, double Eshift,
+                                double E0, const cd* __restrict__ new_ov,
+                                const cd* __restrict__ E, const cd* __restrict__ mf,
+                                const cd* __restrict__ HW, double lower_scale,
+                                double upper_scale, double max_weight,
+                                unsigned long long* __restrict__ stats) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= W) return;
+  cd* p = wbuf + (long)i * ldw + poff;
+  const cd old_ov = p[AFQ_P_OVLP], old_eloc = p[AFQ_P_PSEUDO_ELOC];
+  double scale = 1.0, dtheta = 0.0;
+  if (!free_projection) {
+    dtheta = new_ov[i].im - old_ov.im - mf[i].im;
+    const double c = cos(dtheta);
+    scale = use_cp ? (c > 0.0 ? 1.0 : 0.0) : fmax(0.0, c);
+  }
+  p[AFQ_P_THETA] = cmk(dtheta, 0.0);
+  cd eloc;
+  if (hybrid) {
+    const cd num = mf[i] - HW[i] - (new_ov[i] - old_ov);
+    eloc = cmk(num.re / dt + E0, num.im / dt);
+  } else {
+    eloc = E[3 * i] + E[3 * i + 1] + E[3 * i + 2];
+  }
+  unsigned long long n_upper = 0, n_lower = 0;
+  if (!isfinite(eloc.re)) {
+    p[AFQ_P_WEIGHT] = cmk(0, 0);
+    p[AFQ_P_PHASE] = cmk(0, 0);
+    eloc = old_eloc;
+  } else {
+    if (!free_projection) {
+      const double hi = Eshift + upper_scale * sqrt(2.0 / dt);
+      const double lo = Eshift - lower_scale * sqrt(2.0 / dt);
+      if (eloc.re > hi) n_upper = 1;
+      else if (eloc.re < lo) n_lower = 1;
+      eloc.re = fmin(fmax(eloc.re, lo), hi);
+    }
+    const cd eeff = hybrid ? eloc : 0.5 * (eloc + old_eloc);
+    const cd wf = cexp_(cmk(-dt * (eeff.re - Eshift), -dt * eeff.im));
+    const double aw = cabs_(wf);
+    const cd wt = p[AFQ_P_WEIGHT];
+    p[AFQ_P_WEIGHT] = free_projection ? (wt * (scale * wf)) : (scale * aw) * wt;
+    p[AFQ_P_PHASE] = p[AFQ_P_PHASE] * cmk(wf.re / aw, wf.im / aw);
+  }
+  p[AFQ_P_PSEUDO_ELOC] = eloc;
+  p[AFQ_P_OVLP] = new_ov[i];
+  if (!hybrid) {
+    p[AFQ_P_E1] = E[3 * i];
+    p[AFQ_P_EXX] = E[3 * i + 1];
+    p[AFQ_P_EJ] = E[3 * i + 2];
+  }
+  // bound_walker_weights: |w| capped at max(floor, fraction * target population)
+  unsigned long long n_wcap = 0;
+  const cd wt = p[AFQ_P_WEIGHT];
+  const double a = cabs_(wt);
+  if (a > max_weight) {
+    n_wcap = 1;
+    p[AFQ_P_WEIGHT] = (max_weight / a) * wt;
+  }
+  if (stats) {
+    if (!free_projection && isfinite(eloc.re)) atomicAdd(&stats[0], 1ull);
+    if (n_upper) atomicAdd(&stats[1], 1ull);
+    if (n_lower) atomicAdd(&stats[2], 1ull);
+    atomicAdd(&stats[3], 1ull);
+    if (n_wcap) atomicAdd(&stats[4], 1ull);
+  }
+}
+
+__global__ void k_set_properties(int W, cd* __restrict__ wbuf, long ldw, long poff,
+                                 const cd* __restrict__ ov, const cd* __restrict__ E,
+                                 int seed_pseudo) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= W) return;
+  cd* p = wbuf + (long)i * ldw + poff;
+  p[AFQ_P_OVLP] = ov[i];
+  if (E) {
+    p[AFQ_P_E1] = E[3 * i];
+    p[AFQ_P_EXX] = E[3 * i + 1];
+    p[AFQ_P_EJ] = E[3 * i + 2];
+    if (seed_pseudo) p[AFQ_P_PSEUDO_ELOC] = E[3 * i] + E[3 * i + 1] + E[3 * i + 2];
+  }
+}
+
+// out[0] = sum w (E1+EXX+EJ), out[1] = sum w, out[2] = sum w exp(dt PSEUDO_ELOC),
+// out[3] = sum w PSEUDO_ELOC; one workgroup, fixed order
+__global__ __launch_bounds__(256) void k_weighted_sums(int W, const cd* __restrict__ wbuf,
+                                                       long ldw, long poff, double dt,
+                                                       cd* __restrict__ out) {
+  __shared__ cd red[4][4];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  cd s[4] = {cmk(0, 0), cmk(0, 0), cmk(0, 0), cmk(0, 0)};
+  for (int i = tid; i < W; i += 256) {
+    const cd* p = wbuf + (long)i * ldw + poff;
+    const cd w = p[AFQ_P_WEIGHT];
+    const cd e = p[AFQ_P_E1] + p[AFQ_P_EXX] + p[AFQ_P_EJ];
+    const cd pe = p[AFQ_P_PSEUDO_ELOC];
+    s[0] += w * e;
+    s[1] += w;
+    s[2] += w * cexp_(dt * pe);
+    s[3] += w * pe;
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    for (int off = 32; off > 0; off >>= 1) {
+      s[q].re += __shfl_xor(s[q].re, off, 64);
+      s[q].im += __shfl_xor(s[q].im, off, 64);
+    }
+  if (lane == 0)
+    for (int q = 0; q < 4; ++q) red[wave][q] = s[q];
+  __syncthreads();
+  if (tid < 4) {
+    cd t = cmk(0, 0);
+    for (int qw = 0; qw < 4; ++qw) t += red[qw][tid];
+    out[tid] = t;
+  }
+}
+
+// D = A B with A[i][k] = i + 100 k, B[k][j] = 1 if k == j%4 else 0  =>  D[i][j] = i + 100 (j%4);
+// out[lane][r] holds accumulator element r of each lane, for reading back the lane layout.
+template <typename R>
+__global__ void k_probe_mfma(double* out) {
+  typedef typename acc_of<R>::type acc_t;
+  const int lane = threadIdx.x;
+  const int i = lane & 15, k = lane >> 4, j = lane & 15;
+  const R a = (R)(i + 100 * k);
+  const R b = (k == (j & 3)) ? (R)1 : (R)0;
+  acc_t acc = {0, 0, 0, 0};
+  acc = mfma16x16x4(a, b, acc);
+  for (int r = 0; r < 4; ++r) out[lane * 4 + r] = (double)acc[r];
+}
+
+}  // namespace
+
+void launch_construct_X(hipStream_t s, int W, int K, double vbias_bound, int project,
+                        int free_projection, const double* vMF, const double* xi, double* X,
+                        double* HW, double* mf) {
+  if (W <= 0) return;
+  hipLaunchKernelGGL(k_construct_X, dim3(W), dim3(256), 0, s, W, K, vbias_bound, project,
+                     free_projection, reinterpret_cast<const cd*>(vMF), xi,
+                     reinterpret_cast<cd*>(X), reinterpret_cast<cd*>(HW),
+                     reinterpret_cast<cd*>(mf));
+  AFQ_HIP(hipGetLastError());
+}
+
+void launch_walker_update(hipStream_t s, int W, double* wbuf, long ldw, long poff,
+                          const afq_step_params& p, double dt, double Eshift, double E0,
+                          const double* ov, const double* E, const double* mf,
+                          const double* HW, int hybrid, unsigned long long* stats) {
+  if (W <= 0) return;
+  const double maxw = fmax(p.weight_bound_floor, p.weight_bound_fraction * (double)W);
+  hipLaunchKernelGGL(k_walker_update, dim3((W + 127) / 128), dim3(128), 0, s, W,
+                     reinterpret_cast<cd*>(wbuf), ldw, poff, hybrid, p.free_projection,
+                     p.use_cp_constraint, dt, Eshift, E0, reinterpret_cast<const cd*>(ov),
+                     reinterpret_cast<const cd*>(E), reinterpret_cast<const cd*>(mf),
+                     reinterpret_cast<const cd*>(HW), p.lower_cutoff_scale,
+                     p.upper_cutoff_scale, maxw, stats);
+  AFQ_HIP(hipGetLastError());
+}
+
+void launch_set_properties(hipStream_t s, int W, double* wbuf, long ldw, long poff,
+                           const double* ov, const double* E, int seed_pseudo) {
+  if (W <= 0) return;
+  hipLaunchKernelGGL(k_set_properties, dim3((W + 127) / 128), dim3(128), 0, s, W,
+                     reinterpret_cast<cd*>(wbuf), ldw, poff, reinterpret_cast<const cd*>(ov),
+                     reinterpret_cast<const cd*>(E), seed_pseudo);
+  AFQ_HIP(hipGetLastError());
+}
+
+void launch_weighted_sums(hipStream_t s, int W, const double* wbuf, long ldw, long poff,
+                          double dt, double* out) {
+  hipLaunchKernelGGL(k_weighted_sums, dim3(1), dim3(256), 0, s, W,
+                     reinterpret_cast<const cd*>(wbuf), ldw, poff, dt,
+                     reinterpret_cast<cd*>(out));
+  AFQ_HIP(hipGetLastError());
+}
+
+void launch_probe_mfma(hipStream_t s, int kind, double* out) {
+  if (kind == 0)
+    hipLaunchKernelGGL(k_probe_mfma<double>, dim3(1), dim3(64), 0, s, out);
+  else
+    hipLaunchKernelGGL(k_probe_mfma<float>, dim3(1), dim3(64), 0, s, out);
+  AFQ_HIP(hipGetLastError());
+}
+
+}  // namespace afq

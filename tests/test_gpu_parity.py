@@ -1,0 +1,247 @@
+"""GPU parity of the HIP walker step against the FP64 oracle, through the C ABI.
+
+Plain FP64 form: every quantity within 1e-10 relative of the oracle on the same seeded inputs
+(G, log-overlap, vbias, vHS, E1/EXX/EJ, propagated walkers, QR, whole steps).
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from safire_amd import synth
+from tests._helpers import cplx, load_json, rand_c, relerr
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+if not torch.cuda.is_available():  # pragma: no cover
+    pytest.skip("no GPU", allow_module_level=True)
+
+from safire_amd import afqmc as A  # noqa: E402
+
+TOL = 1e-10
+
+
+def dev(a, dtype=None):
+    return A.to_device(a, dtype)
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def make(name_or_shape, seed=0):
+    if isinstance(name_or_shape, str):
+        s, W, steps, ortho = synth.workload(name_or_shape)
+        return s, W
+    M, N, K, W = name_or_shape
+    return synth.make_system(M, N, K, seed=seed), W
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_mfma_lane_layout(kind):
+    """The accumulator row maps the kernels use (acc_row in afq_internal.hpp)."""
+    out = torch.zeros(64 * 4, dtype=torch.float64, device="cuda")
+    A.check(A.lib().afq_probe_mfma(kind, A.ptr(out), A.stream_ptr()))
+    d = host(out).reshape(64, 4)
+    for lane in range(64):
+        g, j = lane // 16, lane % 16
+        for r in range(4):
+            i = 4 * r + g if kind == 0 else 4 * g + r
+            assert d[lane, r] == i + 100 * (j % 4), (lane, r, d[lane])
+
+
+def test_det_ops_kat_on_gpu():
+    K = load_json("sdet_ops_kat.json")
+    Am = cplx(K["A"])
+    B = np.broadcast_to(cplx(K["B"]), (5, 4, 3)).copy()
+    G = torch.zeros((5, 3, 4), dtype=torch.complex128, device="cuda")
+    ov = torch.zeros(5, dtype=torch.complex128, device="cuda")
+    A.det_ops.MixedDensityMatrix(dev(Am), dev(B), G, ov)
+    np.testing.assert_allclose(host(G), np.broadcast_to(cplx(K["gc_ref"]), (5, 3, 4)), atol=1e-8)
+    np.testing.assert_allclose(host(ov), np.log(cplx(K["ov"])[0]), atol=1e-8)
+    # sub-range (test_sdet_ops.cpp:121-131): A[:2,:3], B[:, :3, :2]
+    ov2 = torch.zeros(5, dtype=torch.complex128, device="cuda")
+    A.det_ops.Log_Overlap(dev(Am[:2, :3].copy()), dev(B[:, :3, :2].copy()), ov2)
+    np.testing.assert_allclose(host(ov2), np.log(cplx(K["ov2"])[0]), atol=1e-8)
+
+
+@pytest.mark.parametrize("M,N,nb", [(4, 3, 5), (24, 6, 33), (64, 16, 70), (128, 32, 9), (5, 1, 3)])
+def test_mixed_density_matrix_parity(M, N, nb):
+    rng = np.random.default_rng(M * 100 + N)
+    PsiT = rand_c(rng, N, M)
+    Phi = rand_c(rng, nb, M, N)
+    Gref, ovref = O.mixed_density_matrix(PsiT, Phi)
+    G = torch.zeros((nb, N, M), dtype=torch.complex128, device="cuda")
+    ov = dev(np.full(nb, 0.25 + 0.5j))   # accumulates, like the reference
+    A.det_ops.MixedDensityMatrix(dev(PsiT), dev(Phi), G, ov)
+    assert relerr(host(G), Gref) < TOL
+    ovexp = ovref + 0.25 + 0.5j
+    ovexp.imag = (ovexp.imag + np.pi) % (2 * np.pi) - np.pi
+    np.testing.assert_allclose(host(ov), ovexp, rtol=TOL, atol=1e-12)
+
+
+def test_singular_overlap_gives_zero_G():
+    PsiT = np.array([[1, 0, 0], [1, 0, 0]], np.complex128)
+    Phi = np.array([[[1, 0], [0, 1], [0, 0]]], np.complex128)
+    G = torch.ones((1, 2, 3), dtype=torch.complex128, device="cuda")
+    ov = torch.zeros(1, dtype=torch.complex128, device="cuda")
+    A.det_ops.MixedDensityMatrix(dev(PsiT), dev(Phi), G, ov)
+    _, ovref = O.mixed_density_matrix(PsiT, Phi)
+    assert np.all(host(G) == 0)
+    np.testing.assert_allclose(host(ov), ovref, rtol=1e-14)
+
+
+@pytest.mark.parametrize("M,N,nb", [(8, 3, 2), (24, 6, 17), (64, 16, 5), (128, 32, 3)])
+def test_propagate_parity(M, N, nb):
+    rng = np.random.default_rng(7 + M)
+    Phi = rand_c(rng, nb, M, N)
+    P1 = rand_c(rng, M, M) * (0.5 / np.sqrt(M))
+    V = 0.05 * rand_c(rng, nb, M, M) / np.sqrt(M)
+    ref = O.propagate(Phi, P1, V, order=6)
+    t = dev(Phi)
+    A.det_ops.Propagate(t, dev(P1), dev(V), 6)
+    assert relerr(host(t), ref) < TOL
+
+
+@pytest.mark.parametrize("M,N,nb", [(4, 3, 5), (24, 6, 9), (64, 16, 5), (128, 32, 2)])
+def test_orthogonalize_parity(M, N, nb):
+    rng = np.random.default_rng(11 + M)
+    Phi = rand_c(rng, nb, M, N)
+    ref, ldref = O.orthogonalize(Phi)
+    t = dev(Phi)
+    ld = torch.zeros(nb, dtype=torch.complex128, device="cuda")
+    A.det_ops.orthogonalize(t, ld)
+    assert relerr(host(t), ref) < TOL
+    np.testing.assert_allclose(host(ld), ldref, rtol=TOL)
+
+
+@pytest.mark.parametrize("shape", ["c1", (16, 5, 40, 3), (40, 10, 200, 7), (64, 16, 320, 9),
+                                   (96, 24, 480, 5), (128, 32, 640, 2)])
+def test_setup_and_contractions_parity(shape):
+    s, W = make(shape, seed=3)
+    W = min(W, 40)
+    ham = A.HamiltonianOperations(s.h, s.Likn, s.PsiT, 0.3, s.dt)
+    ham.reserve(W)
+    ref = O.System(s.h, s.Likn, s.PsiT, 0.3, s.dt)
+    T, R = ham.tensors(), ref.tensors()
+    for k in ("haj", "Lnak", "v0", "vMF", "H1", "P1"):
+        assert relerr(T[k], R[k]) < 1e-12, k
+    rng = np.random.default_rng(5)
+    Phi = (s.PsiT.conj().T[None] + 0.3 * rand_c(rng, W, s.M, s.N))
+    G, _ = O.mixed_density_matrix(s.PsiT, Phi)
+    Gd = dev(G)
+    v = torch.zeros((W, s.K), dtype=torch.complex128, device="cuda")
+    ham.vbias(Gd, v)
+    vref = O.vbias(G, R["Lnak"], s.dt)
+    assert relerr(host(v), vref) < TOL
+    X = rand_c(rng, W, s.K)
+    V = torch.zeros((W, s.M, s.M), dtype=torch.complex128, device="cuda")
+    ham.vHS(dev(X), V)
+    assert relerr(host(V), O.vhs(X, s.Likn, s.dt)) < TOL
+    E = torch.zeros((W, 3), dtype=torch.complex128, device="cuda")
+    ham.energy(Gd, E)
+    Eref = O.energy(G, R["haj"], R["Lnak"], E0=0.3)
+    Eg = host(E)
+    for c in range(3):
+        assert relerr(Eg[:, c], Eref[:, c]) < TOL, c
+    ham.close()
+
+
+def run_resynced(shape, steps, hybrid, ortho=5, seed=9):
+    """Runs the oracle trajectory; before every step the GPU gets the oracle's walker buffer, so
+    each step is compared on identical inputs. Returns per step and walker the relative error of
+    the Slater matrix, the error of the properties relative to (1 + |value|), and kappa(PsiT Phi)
+    of the walker before the step (the conditioning that amplifies rounding through G)."""
+    s, W = make(shape, seed=seed)
+    ham = A.HamiltonianOperations(s.h, s.Likn, s.PsiT, 0.0, s.dt)
+    ham.set_params(hybrid=int(hybrid))
+    prop = A.Propagator(ham)
+    ref = O.System(s.h, s.Likn, s.PsiT, 0.0, s.dt)
+    buf = ref.new_walkers(W)
+    ref.seed_properties(buf)
+    seeded = A.WalkerSet(ham, W, ref.new_walkers(W))
+    prop.seed(seeded)
+    P = s.M * s.N
+    assert relerr(seeded.host()[:, P:], buf[:, P:]) < TOL
+    Eshift = float(ref.energy(buf)[0].sum(1).real.mean())
+    xi = synth.fields(s.seed, steps, W, s.K)
+    out = []
+    for t in range(steps):
+        Phi = buf[:, :P].reshape(W, s.M, s.N)
+        kappa = np.array([np.linalg.cond(s.PsiT @ Phi[w]) for w in range(W)])
+        wset = A.WalkerSet(ham, W, buf.copy())
+        prop.Propagate(wset, dev(xi[t]), Eshift)
+        ref.step(buf, xi[t], Eshift, hybrid=hybrid)
+        if t % ortho == 0:
+            prop.Orthogonalize(wset)
+            ref.orthogonalize(buf)
+        g = wset.host()
+        e_sm = np.abs(g[:, :P] - buf[:, :P]).max(1) / np.abs(buf[:, :P]).max(1)
+        e_pr = (np.abs(g[:, P:] - buf[:, P:]) / (1 + np.abs(buf[:, P:]))).max(1)
+        # a walker whose weight is exactly 0 stays 0 for good (the phaseless projection killed
+        # it); it is still propagated, its energies grow without bound, and only its weight
+        # carries meaning: it must be 0 on both sides
+        dead = buf[:, P + A.P_WEIGHT] == 0
+        assert np.array_equal(dead, g[:, P + A.P_WEIGHT] == 0), t
+        e_sm[dead] = 0.0
+        e_pr[dead] = 0.0
+        out.append((e_sm, e_pr, kappa))
+    ham.close()
+    return out
+
+
+def worst(e, kappa):
+    w = int(np.argmax(e))
+    return f"walker {w}: err {e[w]:.3e}, kappa {kappa[w]:.3e}"
+
+
+@pytest.mark.parametrize("hybrid", [True, False])
+def test_walker_steps_parity_c1(hybrid):
+    """c1 (M=24, N=6, K=96): 64 walkers, 50 steps, QR every 5 steps, each step within 1e-10."""
+    for t, (e_sm, e_pr, kappa) in enumerate(run_resynced("c1", 50, hybrid)):
+        assert e_sm.max() < TOL, (t, worst(e_sm, kappa))
+        assert e_pr.max() < TOL, (t, worst(e_pr, kappa))
+
+
+@pytest.mark.parametrize("hybrid", [True, False])
+def test_walker_steps_parity_c2(hybrid):
+    """c2 (M=64, N=16, K=320), 512 walkers, 12 steps. Rounding is amplified by the conditioning
+    of each walker's overlap matrix; the bound is 1e-10 + 1e-13 kappa_w per walker."""
+    for t, (e_sm, e_pr, kappa) in enumerate(run_resynced("c2", 12, hybrid)):
+        bound = TOL + 1e-13 * kappa
+        assert np.all(e_sm < bound), (t, worst(e_sm / bound, kappa))
+        assert np.all(e_pr < bound), (t, worst(e_pr / bound, kappa))
+
+
+@pytest.mark.parametrize("W", [1, 7, 65])
+def test_ragged_walker_counts(W):
+    for t, (e_sm, e_pr, kappa) in enumerate(run_resynced((24, 6, 96, W), 3, False)):
+        assert e_sm.max() < TOL, (t, worst(e_sm, kappa))
+        assert e_pr.max() < TOL, (t, worst(e_pr, kappa))
+
+
+def test_bound_statistics_match():
+    """Local-energy and weight bound counters agree with the oracle over a free-running c1 run."""
+    s, W = make("c1")
+    ham = A.HamiltonianOperations(s.h, s.Likn, s.PsiT, 0.0, s.dt)
+    ham.set_params(hybrid=0)
+    wset = A.WalkerSet(ham, W)
+    prop = A.Propagator(ham)
+    prop.seed(wset)
+    ref = O.System(s.h, s.Likn, s.PsiT, 0.0, s.dt)
+    buf = ref.new_walkers(W)
+    ref.seed_properties(buf)
+    Eshift = float(ref.energy(buf)[0].sum(1).real.mean())
+    xi = synth.fields(s.seed, 8, W, s.K)
+    for t in range(8):
+        prop.Propagate(wset, dev(xi[t]), Eshift)
+        ref.step(buf, xi[t], Eshift, hybrid=False)
+        if t % 5 == 0:
+            prop.Orthogonalize(wset)
+            ref.orthogonalize(buf)
+    e_gpu, w_gpu = ham.stats()
+    e_ref, w_ref = ref.stats()
+    assert np.array_equal(e_gpu, e_ref) and np.array_equal(w_gpu, w_ref)
+    assert relerr(wset.host(), buf) < TOL
+    ham.close()
