@@ -49,7 +49,20 @@ def step_flops(M, N, K):
             + (16 * M * N * N - 16 * N ** 3 / 3) / 5)        # QR every 5 steps
 
 
-def run_form(A, torch, s, W, xi_dev, warmup, steps, ortho, precision, hybrid):
+POPCTL_INTERVAL = 10   # population_control_interval default (config.h:44)
+
+
+def run_loop_step(prop, wset, xi_t, Eshift, t, ortho, uniforms):
+    """One iteration of run_afqmc's loop (run_afqmc.cpp:69-84): Propagate, Orthogonalize every
+    `ortho` steps, population control every POPCTL_INTERVAL steps."""
+    prop.Propagate(wset, xi_t, Eshift)
+    if t % ortho == 0:
+        prop.Orthogonalize(wset)
+    if t % POPCTL_INTERVAL == 0:
+        prop.population_control(wset, uniforms[t])
+
+
+def run_form(A, torch, s, W, xi_dev, uniforms, warmup, steps, ortho, precision, hybrid):
     ham = A.HamiltonianOperations(s.h, s.Likn, s.PsiT, 0.0, s.dt)
     ham.set_params(hybrid=int(hybrid))
     ham.set_precision(precision)
@@ -59,9 +72,7 @@ def run_form(A, torch, s, W, xi_dev, warmup, steps, ortho, precision, hybrid):
     sums = prop.weighted_sums(wset).cpu().numpy()
     Eshift = float((sums[0] / sums[1]).real)
     for t in range(warmup):
-        prop.Propagate(wset, xi_dev[t], Eshift)
-        if t % ortho == 0:
-            prop.Orthogonalize(wset)
+        run_loop_step(prop, wset, xi_dev[t], Eshift, t, ortho, uniforms)
     torch.cuda.synchronize()
     ham.enable_timers(16 * steps + 16)
     start = torch.cuda.Event(enable_timing=True)
@@ -70,9 +81,7 @@ def run_form(A, torch, s, W, xi_dev, warmup, steps, ortho, precision, hybrid):
     t0 = time.perf_counter()
     start.record()
     for t in range(warmup, warmup + steps):
-        prop.Propagate(wset, xi_dev[t], Eshift)
-        if t % ortho == 0:
-            prop.Orthogonalize(wset)
+        run_loop_step(prop, wset, xi_dev[t], Eshift, t, ortho, uniforms)
     stop.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -94,6 +103,8 @@ def eloc(A, torch, ham, wset):
 
 def cpu_baseline(s, W, ortho, seconds):
     """The FP64 oracle (C++, OpenMP over walkers) on a bounded sample of the same workload."""
+    import numpy as np
+
     from oracle import oracle as O
     from safire_amd import synth
     Ws = W
@@ -102,6 +113,7 @@ def cpu_baseline(s, W, ortho, seconds):
     ref.seed_properties(buf)
     Eshift = float(ref.energy(buf)[0].sum(1).real.mean())
     max_steps = 200
+    uniforms = np.random.default_rng([s.seed, 31]).random(max_steps)
     done, el = 0, 0.0
     while done < max_steps and el < seconds:
         xi = synth.fields(s.seed, 1, W, s.K, first_step=done)[0]   # input, not timed
@@ -109,6 +121,8 @@ def cpu_baseline(s, W, ortho, seconds):
         ref.step(buf, xi[:Ws], Eshift, hybrid=False)
         if done % ortho == 0:
             ref.orthogonalize(buf)
+        if done % POPCTL_INTERVAL == 0:
+            O.population_control(buf, s.M * s.N, uniforms[done])
         el += time.perf_counter() - t0
         done += 1
     return {"value": round(Ws * done / el, 2), "unit": "walker-steps/s",
@@ -148,11 +162,12 @@ def main(argv=None):
     print(f"[bench] {s.name}: M={M} N={N} K={K} W={W}, fields ready in "
           f"{time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
 
+    uniforms = np.random.default_rng([s.seed, 31]).random(nst)
     forms = {}
     for f in args.forms.split(","):
         prec = {"fp64": A.FP64, "fp32": A.FP32}[f]
-        forms[f] = run_form(A, torch, s, W, xi_dev, args.warmup, args.steps, ortho, prec,
-                            hybrid=False)
+        forms[f] = run_form(A, torch, s, W, xi_dev, uniforms, args.warmup, args.steps, ortho,
+                            prec, hybrid=False)
         r = forms[f]
         print(f"[bench] {f}: {r['ms'] / args.steps:.3f} ms/step, "
               f"{W * args.steps / (r['ms'] / 1e3):.0f} walker-steps/s; regions (ms total): "
@@ -160,26 +175,37 @@ def main(argv=None):
               file=sys.stderr, flush=True)
 
     # accuracy of each reduced-precision form on identical walker states: the seeded trial state
-    # and the final state of the FP64 run, over the walkers whose weight is not zero (a walker
-    # killed by the phaseless projection keeps weight 0 and enters no estimator)
+    # and the states right after population control at steps 10, 20 and 30 of an (untimed) FP64
+    # run, over the walkers whose weight is not zero (a walker killed by the phaseless
+    # projection keeps weight 0 and enters no estimator)
     ref_form = forms.get("fp64")
     gate = {}
     if ref_form is not None:
         P = M * N
-        init = A.WalkerSet(ref_form["ham"], W)
-        states = [init, ref_form["wset"]]
+        ham64 = ref_form["ham"]
+        states = [A.WalkerSet(ham64, W)]
+        run = A.WalkerSet(ham64, W)
+        prop = A.Propagator(ham64)
+        prop.seed(run)
+        for t in range(31):
+            run_loop_step(prop, run, xi_dev[t], ref_form["Eshift"], t, ortho, uniforms)
+            if t in (10, 20, 30):
+                states.append(A.WalkerSet(ham64, W, run.host()))
         for f in forms:
             if f == "fp64":
                 continue
-            worst = 0.0
+            worst, nlive = 0.0, 0
             for st in states:
                 live = np.abs(st.host()[:, P + A.P_WEIGHT]) > 0
-                E64 = eloc(A, torch, ref_form["ham"], st).sum(1)
+                if not np.any(live):
+                    continue
+                E64 = eloc(A, torch, ham64, st).sum(1)
                 Ef = eloc(A, torch, forms[f]["ham"], st).sum(1)
                 worst = max(worst, float(np.max(np.abs(E64 - Ef)[live])))
+                nlive += int(live.sum())
             gate[f] = worst
-            print(f"[bench] {f}: max |E_loc({f}) - E_loc(fp64)| = {worst:.3e} Ha over live "
-                  f"walkers of the trial and final states", file=sys.stderr, flush=True)
+            print(f"[bench] {f}: max |E_loc({f}) - E_loc(fp64)| = {worst:.3e} Ha over {nlive} "
+                  f"live walker states", file=sys.stderr, flush=True)
     passing = [f for f in forms if f == "fp64" or gate.get(f, 1.0) < GATE_HA]
     # the headline: the fastest form that meets the accuracy gate
     mixed = min(passing, key=lambda f: forms[f]["ms"])

@@ -98,8 +98,9 @@ int afq_ham_get_params(afq_ham ham, afq_step_params* p);
 int afq_ham_tensors(afq_ham ham, double* h_haj, double* h_Lnak, double* h_v0, double* h_vMF,
                     double* h_P1, double* h_H1);
 /* Local-energy bound statistics (total, upper, lower) and weight-bound statistics (total,
- * upper) since creation (AFQMCBasePropagator.h:199-237). */
-int afq_ham_stats(afq_ham ham, long long* h_eloc3, long long* h_weight2);
+ * upper) since creation (AFQMCBasePropagator.h:199-237), and population-control (calls, calls
+ * refused because the total weight collapsed below 1e-6). Any pointer may be NULL. */
+int afq_ham_stats(afq_ham ham, long long* h_eloc3, long long* h_weight2, long long* h_popctl2);
 
 /* ---------------------------------------------------------------------------------------------
  * det_ops level (density_matrix.hpp, propagate.hpp, orthogonalize.hpp): explicit operands.
@@ -171,6 +172,14 @@ int afq_orthogonalize(afq_ham ham, int W, double* d_wbuf, long ldw, void* stream
 /* NOMSD::Energy(wset) (NOMSD.cpp:69-80) + seedPseudoEnergy: OVLP, E1, EXX, EJ and
  * PSEUDO_ELOC = E1+EXX+EJ of every walker (execute.cpp:172-184). */
 int afq_seed_properties(afq_ham ham, int W, double* d_wbuf, long ldw, void* stream);
+/* population_control (population_control.hpp:219-257) on one GPU: the comb
+ * (population_control.hpp:34-65) over |weight| with the given uniform in [0,1), W teeth; a
+ * walker with k offspring keeps its slot and its k-1 copies fill the slots of zero-offspring
+ * walkers in increasing order (swap_walkers_async, population_control.hpp:143-209); then every
+ * weight becomes w/|w|. A total weight <= 1e-6 (where the reference aborts) leaves the walkers
+ * untouched and is counted in afq_ham_stats. */
+int afq_population_control(afq_ham ham, int W, double* d_wbuf, long ldw, double uniform,
+                           void* stream);
 /* Weighted sums for the estimators (EnergyEstimator.h:51-100, average_energy.hpp:25-63):
  * d_out[0] = sum w (E1+EXX+EJ), [1] = sum w, [2] = sum w exp(dt PSEUDO_ELOC), [3] = sum w
  * PSEUDO_ELOC; d_out is c128 [4]. Deterministic order. */

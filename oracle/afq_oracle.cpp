@@ -656,6 +656,55 @@ void orc_bound_weights(int W, cd* wbuf, long ldw, long poff, double floor_, doub
   }
 }
 
+// comb (population_control.hpp:34-65): offspring counts of W walkers from |w| and one uniform r.
+// Returns 0, or 1 when the total weight collapsed (<= 1e-6; the reference aborts there).
+int orc_comb(int W, const double* weights, double r, int num_target, int* offspring) {
+  std::vector<double> cum(W);
+  std::partial_sum(weights, weights + W, cum.begin());
+  const double total = cum[W - 1];
+  if (!(total > 1e-6)) return 1;
+  int last = W - 1;
+  while (weights[last] <= 0.0) --last;
+  std::fill(offspring, offspring + W, 0);
+  int iw = 0, ic = 0;
+  while (ic < num_target) {
+    const double comb = (ic + r) * total / num_target;
+    if (comb < cum[iw] || iw == last) {
+      offspring[iw] += 1;
+      ic++;
+    } else {
+      iw++;
+    }
+  }
+  return 0;
+}
+
+// population_control (population_control.hpp:219-257) on one rank: comb by |w|, then
+// swap_walkers_async's single-rank placement (population_control.hpp:143-209): a walker with k
+// copies keeps its slot and its k-1 extra copies fill the slots of zero-offspring walkers in
+// increasing slot order; finally every weight becomes w/|w|.
+int orc_population_control(int W, cd* wbuf, long ldw, long poff, double r, int* offspring_out) {
+  std::vector<double> wt(W);
+  for (int i = 0; i < W; ++i) wt[i] = std::abs(wbuf[(long)i * ldw + poff + P_WEIGHT]);
+  std::vector<int> off(W);
+  if (orc_comb(W, wt.data(), r, W, off.data())) return 1;
+  std::vector<int> free_slots;
+  for (int i = 0; i < W; ++i)
+    if (off[i] == 0) free_slots.push_back(i);
+  size_t next = 0;
+  for (int i = 0; i < W; ++i)
+    for (int c = 1; c < off[i]; ++c) {
+      const int dst = free_slots[next++];
+      std::copy(wbuf + (long)i * ldw, wbuf + (long)(i + 1) * ldw, wbuf + (long)dst * ldw);
+    }
+  for (int i = 0; i < W; ++i) {
+    cd& w = wbuf[(long)i * ldw + poff + P_WEIGHT];
+    w = w / std::abs(w);
+  }
+  if (offspring_out) std::copy(off.begin(), off.end(), offspring_out);
+  return 0;
+}
+
 // ============================= whole system (closed-shell NOMSD, one determinant) =============
 // A Real3IndexFactorization Hamiltonian + NOMSD trial + AFQMCBasePropagator state: the inputs
 // of one walker step, built as RealDenseHamiltonian::getHamiltonianOperations and generateP1 do.

@@ -67,6 +67,8 @@ def _declare(L):
         "orc_walker_update": (None, [_i, _vp, _l, _l, _i, _i, _i, _d, _d, _d, _vp, _vp, _vp,
                                      _vp, _d, _d, _vp]),
         "orc_bound_weights": (None, [_i, _vp, _l, _l, _d, _d, _d, _vp]),
+        "orc_comb": (_i, [_i, _vp, _d, _i, _vp]),
+        "orc_population_control": (_i, [_i, _vp, _l, _l, _d, _vp]),
         "orc_system_create": (_vp, [_i, _i, _i, _vp, _vp, _vp, _d, _d]),
         "orc_system_destroy": (None, [_vp]),
         "orc_system_set_params": (None, [_vp, _i, _d, _d, _d, _d, _d, _i, _i, _i]),
@@ -233,6 +235,22 @@ def meanfield_factor(X, vMF):
     mf = np.zeros(W, np.complex128)
     lib().orc_meanfield_factor(W, K, _p(X), _p(_c(vMF)), _p(mf))
     return mf
+
+
+def comb(weights, r, num_target=None):
+    w = np.ascontiguousarray(weights, np.float64)
+    n = len(w) if num_target is None else num_target
+    off = np.zeros(len(w), np.int32)
+    rc = lib().orc_comb(len(w), _p(w), float(r), n, _p(off))
+    return off, rc
+
+
+def population_control(buf, poff, r):
+    """In place on a walker buffer [W, ldw]; returns (offspring, status)."""
+    W = buf.shape[0]
+    off = np.zeros(W, np.int32)
+    rc = lib().orc_population_control(W, _p(buf), buf.shape[1], poff, float(r), _p(off))
+    return off, rc
 
 
 # ------------------------------------------------------------------ whole system

@@ -52,7 +52,8 @@ SIGNATURES = {
     "afq_ham_set_params": (_i, [_vp, _vp]),
     "afq_ham_get_params": (_i, [_vp, _vp]),
     "afq_ham_tensors": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
-    "afq_ham_stats": (_i, [_vp, _vp, _vp]),
+    "afq_ham_stats": (_i, [_vp, _vp, _vp, _vp]),
+    "afq_population_control": (_i, [_vp, _i, _vp, _l, _d, _vp]),
     "afq_det_mixed_density_matrix": (_i, [_i, _i, _i, _vp, _vp, _l, _vp, _vp, _vp]),
     "afq_det_propagate": (_i, [_i, _i, _i, _vp, _l, _vp, _vp, _i, _vp]),
     "afq_det_orthogonalize": (_i, [_i, _i, _i, _vp, _l, _vp, _vp]),
@@ -203,10 +204,12 @@ class HamiltonianOperations:
                 for i in range(NUM_REGIONS)}
 
     def stats(self):
+        """(eloc bound total/upper/lower, weight bound total/upper, popctl calls/collapses)."""
         e = np.zeros(3, np.int64)
         w = np.zeros(2, np.int64)
-        check(lib().afq_ham_stats(self.handle, e.ctypes.data, w.ctypes.data))
-        return e, w
+        p = np.zeros(2, np.int64)
+        check(lib().afq_ham_stats(self.handle, e.ctypes.data, w.ctypes.data, p.ctypes.data))
+        return e, w, p
 
     # Real3IndexFactorization interface
     def vbias(self, G, v, stream=None):
@@ -294,6 +297,10 @@ class Propagator:
     def Orthogonalize(self, wset: WalkerSet, stream=None):
         check(lib().afq_orthogonalize(self.ham.handle, wset.W, ptr(wset.buf), wset.ldw,
                                       stream_ptr(stream)))
+
+    def population_control(self, wset: WalkerSet, uniform: float, stream=None):
+        check(lib().afq_population_control(self.ham.handle, wset.W, ptr(wset.buf), wset.ldw,
+                                           float(uniform), stream_ptr(stream)))
 
     def seed(self, wset: WalkerSet, stream=None):
         check(lib().afq_seed_properties(self.ham.handle, wset.W, ptr(wset.buf), wset.ldw,

@@ -56,6 +56,12 @@ void free_workspace(Workspace& ws) {
   dfree(ws.E);
   dfree(ws.Kl);
   dfree(ws.exx);
+  dfree(ws.split);
+  ws.split_elems = 0;
+  dfree(ws.cum);
+  dfree(ws.off);
+  dfree(ws.src);
+  dfree(ws.ok);
   ws.cap = 0;
 }
 
@@ -173,11 +179,11 @@ int afq_ham_create(int M, int N, int K, const double* h_h, const double* h_Likn,
       dalloc(&H->H1, 2 * MM);
       dalloc(&H->Lnak_f, 2 * NM * K);
       dalloc(&H->Likn_f, MM * K);
-      dalloc(&H->stats, 5);
+      dalloc(&H->stats, 7);
       AFQ_HIP(hipMemcpy(H->PsiT, h_PsiT, 2 * NM * sizeof(double), hipMemcpyHostToDevice));
       AFQ_HIP(hipMemcpy(H->Likn, h_Likn, MM * K * sizeof(double), hipMemcpyHostToDevice));
       AFQ_HIP(hipMemcpy(H->h, h_h, MM * sizeof(double), hipMemcpyHostToDevice));
-      AFQ_HIP(hipMemset(H->stats, 0, 5 * sizeof(unsigned long long)));
+      AFQ_HIP(hipMemset(H->stats, 0, 7 * sizeof(unsigned long long)));
       setup_hamiltonian(H, nullptr);
       AFQ_HIP(hipDeviceSynchronize());
     } catch (...) {
@@ -228,6 +234,12 @@ int afq_ham_reserve(afq_ham H, int max_walkers) {
     dalloc(&ws.E, 6 * W);
     dalloc(&ws.Kl, 2 * W * K);
     dalloc(&ws.exx, 2 * W * (size_t)exx_nchunks(H->K));
+    ws.split_elems = (long)kMaxSplits * W * K;
+    dalloc(&ws.split, 2 * (size_t)ws.split_elems);
+    dalloc(&ws.cum, W);
+    dalloc(&ws.off, W);
+    dalloc(&ws.src, W);
+    dalloc(&ws.ok, 1);
     ws.cap = max_walkers;
   });
 }
@@ -274,16 +286,30 @@ int afq_ham_tensors(afq_ham H, double* haj, double* Lnak, double* v0, double* vM
   });
 }
 
-int afq_ham_stats(afq_ham H, long long* eloc3, long long* weight2) {
+int afq_ham_stats(afq_ham H, long long* eloc3, long long* weight2, long long* popctl2) {
   return guard([&] {
     check_ham(H);
-    unsigned long long st[5];
+    unsigned long long st[7];
     AFQ_HIP(hipDeviceSynchronize());
     AFQ_HIP(hipMemcpy(st, H->stats, sizeof(st), hipMemcpyDeviceToHost));
     if (eloc3)
       for (int i = 0; i < 3; ++i) eloc3[i] = (long long)st[i];
     if (weight2)
       for (int i = 0; i < 2; ++i) weight2[i] = (long long)st[3 + i];
+    if (popctl2)
+      for (int i = 0; i < 2; ++i) popctl2[i] = (long long)st[5 + i];
+  });
+}
+
+int afq_population_control(afq_ham H, int W, double* wbuf, long ldw, double uniform,
+                           void* stream) {
+  return guard([&] {
+    check_ham(H);
+    check_walkers(H, W, ldw);
+    AFQ_CHECK(uniform >= 0.0 && uniform < 1.0, AFQ_EINVAL, "uniform must be in [0, 1)");
+    Workspace& ws = H->ws;
+    launch_population_control(S(stream), W, wbuf, ldw, (long)H->M * H->N, uniform, ws.cum,
+                              ws.off, ws.src, ws.ok, H->stats);
   });
 }
 
@@ -342,8 +368,9 @@ int afq_vbias(afq_ham H, int W, const double* G, double* v, void* stream) {
     check_ham(H);
     AFQ_CHECK(W >= 0, AFQ_EBADSHAPE, "negative walker count");
     const long NM = (long)H->N * H->M;
+    AFQ_CHECK(W <= H->ws.cap, AFQ_ECAPACITY, "more walkers than reserved");
     launch_gemm_nt_z(S(stream), H->precision, W, H->K, (int)NM, std::sqrt(H->dt) * scl(H), G,
-                     NM, L_for(H), NM, true, v, H->K);
+                     NM, L_for(H), NM, true, v, H->K, H->ws.split, H->ws.split_elems);
   });
 }
 
@@ -431,7 +458,7 @@ int afq_step(afq_ham H, int W, double* wbuf, long ldw, const double* xi, double 
     {
       RegionScope r(tm, s, R_VBIAS);
       launch_gemm_nt_z(s, H->precision, W, K, (int)NM, std::sqrt(H->dt) * scl(H), ws.G, NM,
-                       L_for(H), NM, true, ws.X, K);
+                       L_for(H), NM, true, ws.X, K, ws.split, ws.split_elems);
     }
     // 2. X, HW, mean-field factor     (assemble_X)
     {

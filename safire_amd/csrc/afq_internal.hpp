@@ -110,6 +110,12 @@ struct Workspace {
   double* E = nullptr;       // c128 [W][3]
   double* Kl = nullptr;      // c128 [W][K]
   double* exx = nullptr;     // c128 [W][nchunk]
+  double* cum = nullptr;     // f64 [W]   population control: cumulative |w|
+  int* off = nullptr;        // i32 [W]   offspring counts
+  int* src = nullptr;        // i32 [W]   source walker of each slot (-1 = keep)
+  int* ok = nullptr;         // i32 [1]   comb accepted the population
+  double* split = nullptr;   // c128 [kMaxSplits][W][K]   split-k partials of vbias
+  long split_elems = 0;
 };
 
 // Region timers of a step (the AFQMCTimers regions of AFQMCTimer.h:25-63), as HIP events
@@ -156,7 +162,8 @@ struct afq_ham_s {
   double* H1 = nullptr;      // c128 [M][M]
   float* Lnak_f = nullptr;   // c64 [K][N][M]   (FP32 contraction copies)
   float* Likn_f = nullptr;   // f32 [M*M][K]
-  unsigned long long* stats = nullptr;   // [5]: eloc total/upper/lower, weight total/upper
+  // [7]: eloc total/upper/lower, weight total/upper, population control calls/collapses
+  unsigned long long* stats = nullptr;
   afq::Workspace ws;
   afq::Timers timers;
 };
@@ -168,10 +175,13 @@ namespace afq {
 void launch_green(hipStream_t s, int nb, int M, int N, const double* PsiT, const double* Phi,
                   long ldw, double* G, double* ov, int ov_mode, double ov_scale, cd ov_add,
                   const double* haj, double E0, double e1_scale, double* E, long ldE);
-// k_gemm.hip:  C[m][n] = alpha * sum_k A[m][k] B[n][k]; A complex, B complex or real
+// k_gemm.hip:  C[m][n] = alpha * sum_k A[m][k] B[n][k]; A complex, B complex or real.
+// split_ws (c128, split_ws_elems complex numbers) enables a deterministic split of k; NULL = none.
+constexpr int kMaxSplits = 8;
+int gemm_splits(int m, int n, int k);
 void launch_gemm_nt_z(hipStream_t s, int precision, int m, int n, int k, double alpha,
                       const double* A, long lda, const void* B, long ldb, bool b_complex,
-                      double* C, long ldc);
+                      double* C, long ldc, double* split_ws = nullptr, long split_ws_elems = 0);
 // k_energy.hip
 int exx_nchunks(int K);
 void launch_exchange(hipStream_t s, int precision, int W, int M, int N, int K, const double* G,
@@ -199,6 +209,9 @@ void launch_set_properties(hipStream_t s, int W, double* wbuf, long ldw, long po
 void launch_weighted_sums(hipStream_t s, int W, const double* wbuf, long ldw, long poff,
                           double dt, double* out);
 void launch_probe_mfma(hipStream_t s, int kind, double* out);
+void launch_population_control(hipStream_t s, int W, double* wbuf, long ldw, long poff,
+                               double r, double* cum, int* off, int* src, int* ok,
+                               unsigned long long* stats);
 // k_setup.hip
 void setup_hamiltonian(afq_ham_s* H, hipStream_t s);
 

@@ -77,40 +77,60 @@ __global__ __launch_bounds__(64 * WPB) void k_exchange(int W, int M, int N, int 
         sB[2 * ((v * NB + a) * LDB + kk) + 1] = im;
       }
       __syncthreads();
-      // A fragments: G_w[t*16 + r16][k0 + 4g + s]
+      // A fragments: G_w[t*16 + r16][k0 + 4g + s]   (16-byte loads when the slice is full)
       R ar[NT][4], ai[NT][4];
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
         const int row = t * 16 + r16;
+        const int kb = k0 + 4 * g;
+        if (row < N && kb + 4 <= M) {
+          const double2* p = reinterpret_cast<const double2*>(Gw + 2 * ((long)row * M + kb));
 #pragma unroll
-        for (int s = 0; s < 4; ++s) {
-          const int kx = k0 + 4 * g + s;
-          const bool ok = row < N && kx < M;
-          ar[t][s] = ok ? (R)Gw[2 * ((long)row * M + kx)] : (R)0;
-          ai[t][s] = ok ? (R)Gw[2 * ((long)row * M + kx) + 1] : (R)0;
+          for (int s = 0; s < 4; ++s) {
+            const double2 x = p[s];
+            ar[t][s] = (R)x.x;
+            ai[t][s] = (R)x.y;
+          }
+        } else {
+#pragma unroll
+          for (int s = 0; s < 4; ++s) {
+            const bool ok = row < N && kb + s < M;
+            ar[t][s] = ok ? (R)Gw[2 * ((long)row * M + kb + s)] : (R)0;
+            ai[t][s] = ok ? (R)Gw[2 * ((long)row * M + kb + s) + 1] : (R)0;
+          }
         }
       }
+      // k-step outer, vectors inner: consecutive MFMAs go to independent accumulators
 #pragma unroll
-      for (int v = 0; v < NV; ++v)
+      for (int s = 0; s < 4; ++s) {
+        R br[NV][NT], bi[NV][NT];
 #pragma unroll
-        for (int s = 0; s < 4; ++s) {
-          R br[NT], bi[NT];
+        for (int v = 0; v < NV; ++v)
 #pragma unroll
           for (int u = 0; u < NT; ++u) {
             const int idx = 2 * ((v * NB + u * 16 + r16) * LDB + 4 * g + s);
-            br[u] = sB[idx];
-            bi[u] = sB[idx + 1];
+            br[v][u] = sB[idx];
+            bi[v][u] = sB[idx + 1];
           }
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
 #pragma unroll
           for (int a = 0; a < NT; ++a)
 #pragma unroll
             for (int b = 0; b < NT; ++b) {
-              tr_[v][a][b] = mfma16x16x4(ar[a][s], br[b], tr_[v][a][b]);
-              tr_[v][a][b] = mfma16x16x4(-ai[a][s], bi[b], tr_[v][a][b]);
-              ti_[v][a][b] = mfma16x16x4(ar[a][s], bi[b], ti_[v][a][b]);
-              ti_[v][a][b] = mfma16x16x4(ai[a][s], br[b], ti_[v][a][b]);
+              tr_[v][a][b] = mfma16x16x4(ar[a][s], br[v][b], tr_[v][a][b]);
+              ti_[v][a][b] = mfma16x16x4(ar[a][s], bi[v][b], ti_[v][a][b]);
             }
-        }
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+#pragma unroll
+          for (int a = 0; a < NT; ++a)
+#pragma unroll
+            for (int b = 0; b < NT; ++b) {
+              tr_[v][a][b] = mfma16x16x4(-ai[a][s], bi[v][b], tr_[v][a][b]);
+              ti_[v][a][b] = mfma16x16x4(ai[a][s], br[v][b], ti_[v][a][b]);
+            }
+      }
     }
 
     // epilogue: per vector, T block -> LDS, tr(T^2) lane partials, tr(T) -> Kl

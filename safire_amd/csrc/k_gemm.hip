@@ -10,6 +10,10 @@
 // (64 bytes) per 16-wide k chunk; A and B use the same permutation, which leaves the sum
 // unchanged. Both FP64 (16x16x4f64) and FP32 (16x16x4f32) MFMA are instantiated; the FP32 form
 // converts A on load and reads B from the FP32 copy of the Cholesky tensor.
+//
+// vbias has few output tiles (W x K = 512 x 320 is 40 tiles) and a long k (N*M), so k is split
+// over grid.z: each split writes its partial tile to a workspace and a second kernel sums the
+// splits in a fixed order (deterministic, no atomics) and applies alpha.
 #include "afq_internal.hpp"
 
 namespace afq {
@@ -18,19 +22,21 @@ namespace {
 
 constexpr int kTile = 64;
 
-template <typename R> struct cplx2 { R re, im; };
-
 template <typename R, bool BC>
-__global__ __launch_bounds__(256) void k_gemm_nt(int m, int n, int k, double alpha,
+__global__ __launch_bounds__(256) void k_gemm_nt(int m, int n, int k, int kchunk, double alpha,
                                                  const double* __restrict__ A, long lda,
                                                  const R* __restrict__ B, long ldb,
-                                                 double* __restrict__ C, long ldc) {
+                                                 double* __restrict__ C, long ldc,
+                                                 long split_stride) {
   typedef typename acc_of<R>::type acc_t;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int g = lane >> 4, r16 = lane & 15;
   const int m0 = blockIdx.x * kTile + (wave >> 1) * 32;
   const int n0 = blockIdx.y * kTile + (wave & 1) * 32;
+  const int kbeg = blockIdx.z * kchunk;
+  const int kend = min(k, kbeg + kchunk);
+  C += blockIdx.z * split_stride;
 
   acc_t cr[2][2], ci[2][2];
 #pragma unroll
@@ -41,23 +47,34 @@ __global__ __launch_bounds__(256) void k_gemm_nt(int m, int n, int k, double alp
       ci[a][b] = acc_t{0, 0, 0, 0};
     }
 
-  for (int k0 = 0; k0 < k; k0 += 16) {
+  for (int k0 = kbeg; k0 < kend; k0 += 16) {
     const int kb = k0 + 4 * g;
+    const bool kfull = kb + 4 <= kend;
     R ar[2][4], ai[2][4], br[2][4], bi[2][4];
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
       const int row = m0 + t * 16 + r16;
-      const double* pa = A + 2 * ((long)row * lda + kb);
+      if (row < m && kfull) {
+        const double2* pa = reinterpret_cast<const double2*>(A + 2 * ((long)row * lda + kb));
 #pragma unroll
-      for (int s = 0; s < 4; ++s) {
-        const bool ok = row < m && kb + s < k;
-        ar[t][s] = ok ? (R)pa[2 * s] : (R)0;
-        ai[t][s] = ok ? (R)pa[2 * s + 1] : (R)0;
+        for (int s = 0; s < 4; ++s) {
+          const double2 v = pa[s];
+          ar[t][s] = (R)v.x;
+          ai[t][s] = (R)v.y;
+        }
+      } else {
+        const double* pa = A + 2 * ((long)row * lda + kb);
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          const bool ok = row < m && kb + s < kend;
+          ar[t][s] = ok ? (R)pa[2 * s] : (R)0;
+          ai[t][s] = ok ? (R)pa[2 * s + 1] : (R)0;
+        }
       }
       const int col = n0 + t * 16 + r16;
 #pragma unroll
       for (int s = 0; s < 4; ++s) {
-        const bool ok = col < n && kb + s < k;
+        const bool ok = col < n && kb + s < kend;
         if (BC) {
           const R* pb = B + 2 * ((long)col * ldb + kb);
           br[t][s] = ok ? pb[2 * s] : (R)0;
@@ -69,19 +86,26 @@ __global__ __launch_bounds__(256) void k_gemm_nt(int m, int n, int k, double alp
         }
       }
     }
+    // independent accumulators interleaved: 8 (or 4) chains per k-step
 #pragma unroll
-    for (int s = 0; s < 4; ++s)
+    for (int s = 0; s < 4; ++s) {
 #pragma unroll
       for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b) {
           cr[a][b] = mfma16x16x4(ar[a][s], br[b][s], cr[a][b]);
           ci[a][b] = mfma16x16x4(ai[a][s], br[b][s], ci[a][b]);
-          if (BC) {
+        }
+      if (BC) {
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int b = 0; b < 2; ++b) {
             cr[a][b] = mfma16x16x4(-ai[a][s], bi[b][s], cr[a][b]);
             ci[a][b] = mfma16x16x4(ar[a][s], bi[b][s], ci[a][b]);
           }
-        }
+      }
+    }
   }
 
 #pragma unroll
@@ -93,37 +117,78 @@ __global__ __launch_bounds__(256) void k_gemm_nt(int m, int n, int k, double alp
       for (int r = 0; r < 4; ++r) {
         const int row = m0 + a * 16 + acc_row<R>(g, r);
         if (row < m && col < n) {
-          double* pc = C + 2 * ((long)row * ldc + col);
-          pc[0] = alpha * (double)cr[a][b][r];
-          pc[1] = alpha * (double)ci[a][b][r];
+          double2 v;
+          v.x = alpha * (double)cr[a][b][r];
+          v.y = alpha * (double)ci[a][b][r];
+          *reinterpret_cast<double2*>(C + 2 * ((long)row * ldc + col)) = v;
         }
       }
     }
 }
 
+// C[row][col] = alpha * sum_z P[z][row][col]   (fixed order over the splits)
+__global__ void k_split_reduce(int m, int n, int splits, double alpha,
+                               const double2* __restrict__ P, double2* __restrict__ C,
+                               long ldc) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)m * n) return;
+  double re = 0.0, im = 0.0;
+  for (int z = 0; z < splits; ++z) {
+    const double2 v = P[(long)z * m * n + e];
+    re += v.x;
+    im += v.y;
+  }
+  const long row = e / n, col = e % n;
+  C[row * ldc + col] = double2{alpha * re, alpha * im};
+}
+
 }  // namespace
+
+int gemm_splits(int m, int n, int k) {
+  const int tiles = ((m + kTile - 1) / kTile) * ((n + kTile - 1) / kTile);
+  int s = (512 + tiles - 1) / tiles;
+  s = std::min(s, std::max(1, k / 128));
+  return std::max(1, std::min(s, kMaxSplits));
+}
 
 void launch_gemm_nt_z(hipStream_t s, int precision, int m, int n, int k, double alpha,
                       const double* A, long lda, const void* B, long ldb, bool b_complex,
-                      double* C, long ldc) {
+                      double* C, long ldc, double* split_ws, long split_ws_elems) {
   if (m <= 0 || n <= 0) return;
-  dim3 grid((m + kTile - 1) / kTile, (n + kTile - 1) / kTile);
+  int splits = split_ws ? gemm_splits(m, n, k) : 1;
+  if ((long)splits * m * n > split_ws_elems) splits = 1;
+  int kchunk = (k + splits - 1) / splits;
+  kchunk = (kchunk + 15) & ~15;   // k chunks on the 16-wide MFMA slices
+  splits = (k + kchunk - 1) / kchunk;
+  if (splits < 1) splits = 1;
+  dim3 grid((m + kTile - 1) / kTile, (n + kTile - 1) / kTile, splits);
+  double* out = splits > 1 ? split_ws : C;
+  const long ldo = splits > 1 ? n : ldc;
+  const long sstride = splits > 1 ? 2L * m * n : 0;
+  const double a = splits > 1 ? 1.0 : alpha;
   if (precision == AFQ_FP64) {
     if (b_complex)
-      hipLaunchKernelGGL((k_gemm_nt<double, true>), grid, dim3(256), 0, s, m, n, k, alpha, A,
-                         lda, (const double*)B, ldb, C, ldc);
+      hipLaunchKernelGGL((k_gemm_nt<double, true>), grid, dim3(256), 0, s, m, n, k, kchunk, a, A,
+                         lda, (const double*)B, ldb, out, ldo, sstride);
     else
-      hipLaunchKernelGGL((k_gemm_nt<double, false>), grid, dim3(256), 0, s, m, n, k, alpha, A,
-                         lda, (const double*)B, ldb, C, ldc);
+      hipLaunchKernelGGL((k_gemm_nt<double, false>), grid, dim3(256), 0, s, m, n, k, kchunk, a,
+                         A, lda, (const double*)B, ldb, out, ldo, sstride);
   } else {
     if (b_complex)
-      hipLaunchKernelGGL((k_gemm_nt<float, true>), grid, dim3(256), 0, s, m, n, k, alpha, A,
-                         lda, (const float*)B, ldb, C, ldc);
+      hipLaunchKernelGGL((k_gemm_nt<float, true>), grid, dim3(256), 0, s, m, n, k, kchunk, a, A,
+                         lda, (const float*)B, ldb, out, ldo, sstride);
     else
-      hipLaunchKernelGGL((k_gemm_nt<float, false>), grid, dim3(256), 0, s, m, n, k, alpha, A,
-                         lda, (const float*)B, ldb, C, ldc);
+      hipLaunchKernelGGL((k_gemm_nt<float, false>), grid, dim3(256), 0, s, m, n, k, kchunk, a,
+                         A, lda, (const float*)B, ldb, out, ldo, sstride);
   }
   AFQ_HIP(hipGetLastError());
+  if (splits > 1) {
+    const long tot = (long)m * n;
+    hipLaunchKernelGGL(k_split_reduce, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, m,
+                       n, splits, alpha, reinterpret_cast<const double2*>(split_ws),
+                       reinterpret_cast<double2*>(C), ldc);
+    AFQ_HIP(hipGetLastError());
+  }
 }
 
 }  // namespace afq

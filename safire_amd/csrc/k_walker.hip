@@ -178,6 +178,70 @@ __global__ __launch_bounds__(256) void k_weighted_sums(int W, const cd* __restri
   }
 }
 
+// Population control by the comb (population_control.hpp:34-65) and single-rank placement
+// (swap_walkers_async, population_control.hpp:143-209). The comb is a sequential walk over the
+// cumulative weights; one thread runs it in the reference's order so that the offspring are
+// bit-identical. src[slot] = walker copied into slot, or -1 when the slot keeps its walker.
+__global__ void k_comb_plan(int W, const cd* __restrict__ wbuf, long ldw, long poff, double r,
+                            double* __restrict__ cum, int* __restrict__ off,
+                            int* __restrict__ src, int* __restrict__ ok,
+                            unsigned long long* __restrict__ stats) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  ok[0] = 0;
+  double total = 0.0;
+  int last = -1;
+  for (int i = 0; i < W; ++i) {
+    const double a = cabs_(wbuf[(long)i * ldw + poff + AFQ_P_WEIGHT]);
+    total += a;
+    cum[i] = total;
+    off[i] = 0;
+    src[i] = -1;
+    if (a > 0.0) last = i;
+  }
+  if (stats) atomicAdd(&stats[5], 1ull);
+  if (!(total > 1e-6) || last < 0) {
+    if (stats) atomicAdd(&stats[6], 1ull);   // collapsed: walkers left unchanged
+    return;
+  }
+  int iw = 0, ic = 0;
+  while (ic < W) {
+    const double comb = (ic + r) * total / W;
+    if (comb < cum[iw] || iw == last) {
+      off[iw] += 1;
+      ic++;
+    } else {
+      iw++;
+    }
+  }
+  int next = 0;   // next zero-offspring slot
+  for (int i = 0; i < W; ++i)
+    for (int c = 1; c < off[i]; ++c) {
+      while (off[next] != 0) ++next;
+      src[next] = i;
+      ++next;
+    }
+  ok[0] = 1;
+}
+
+__global__ void k_copy_walkers(int W, cd* __restrict__ wbuf, long ldw,
+                               const int* __restrict__ src) {
+  const int slot = blockIdx.x;
+  const int from = src[slot];
+  if (from < 0) return;
+  for (long e = threadIdx.x; e < ldw; e += blockDim.x)
+    wbuf[(long)slot * ldw + e] = wbuf[(long)from * ldw + e];
+}
+
+__global__ void k_normalize_weights(int W, cd* __restrict__ wbuf, long ldw, long poff,
+                                    const int* __restrict__ ok) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= W) return;
+  if (ok[0] == 0) return;   // the comb refused a collapsed population: weights unchanged
+  cd& w = wbuf[(long)i * ldw + poff + AFQ_P_WEIGHT];
+  const double a = cabs_(w);
+  w = cmk(w.re / a, w.im / a);
+}
+
 // D = A B with A[i][k] = i + 100 k, B[k][j] = 1 if k == j%4 else 0  =>  D[i][j] = i + 100 (j%4);
 // out[lane][r] holds accumulator element r of each lane, for reading back the lane layout.
 template <typename R>
@@ -234,6 +298,19 @@ void launch_weighted_sums(hipStream_t s, int W, const double* wbuf, long ldw, lo
   hipLaunchKernelGGL(k_weighted_sums, dim3(1), dim3(256), 0, s, W,
                      reinterpret_cast<const cd*>(wbuf), ldw, poff, dt,
                      reinterpret_cast<cd*>(out));
+  AFQ_HIP(hipGetLastError());
+}
+
+void launch_population_control(hipStream_t s, int W, double* wbuf, long ldw, long poff,
+                               double r, double* cum, int* off, int* src, int* ok,
+                               unsigned long long* stats) {
+  if (W <= 0) return;
+  hipLaunchKernelGGL(k_comb_plan, dim3(1), dim3(64), 0, s, W, reinterpret_cast<const cd*>(wbuf),
+                     ldw, poff, r, cum, off, src, ok, stats);
+  hipLaunchKernelGGL(k_copy_walkers, dim3(W), dim3(256), 0, s, W, reinterpret_cast<cd*>(wbuf),
+                     ldw, (const int*)src);
+  hipLaunchKernelGGL(k_normalize_weights, dim3((W + 127) / 128), dim3(128), 0, s, W,
+                     reinterpret_cast<cd*>(wbuf), ldw, poff, (const int*)ok);
   AFQ_HIP(hipGetLastError());
 }
 

@@ -166,7 +166,6 @@ def run_resynced(shape, steps, hybrid, ortho=5, seed=9):
     assert relerr(seeded.host()[:, P:], buf[:, P:]) < TOL
     Eshift = float(ref.energy(buf)[0].sum(1).real.mean())
     xi = synth.fields(s.seed, steps, W, s.K)
-    out = []
     for t in range(steps):
         Phi = buf[:, :P].reshape(W, s.M, s.N)
         kappa = np.array([np.linalg.cond(s.PsiT @ Phi[w]) for w in range(W)])
@@ -178,7 +177,18 @@ def run_resynced(shape, steps, hybrid, ortho=5, seed=9):
             ref.orthogonalize(buf)
         g = wset.host()
         e_sm = np.abs(g[:, :P] - buf[:, :P]).max(1) / np.abs(buf[:, :P]).max(1)
-        e_pr = (np.abs(g[:, P:] - buf[:, P:]) / (1 + np.abs(buf[:, P:]))).max(1)
+        # Properties are compared on the scale they are computed from: the energies against
+        # S = 1 + |E1| + |EXX| + |EJ| (E_loc is their sum), the phase and weight, which rotate
+        # and scale by exp(-dt E), against 1 + dt S; the other slots against 1 + |value|.
+        S = 1 + np.abs(buf[:, P + A.P_E1]) + np.abs(buf[:, P + A.P_EXX]) + np.abs(buf[:, P + A.P_EJ])
+        scale = 1 + np.abs(buf[:, P:])
+        for slot in (A.P_PSEUDO_ELOC, A.P_E1, A.P_EXX, A.P_EJ):
+            scale[:, slot] = np.maximum(scale[:, slot], S)
+        for slot in (A.P_PHASE, A.P_WEIGHT):
+            scale[:, slot] = np.maximum(scale[:, slot], 1 + s.dt * S)
+        e_slot = np.abs(g[:, P:] - buf[:, P:]) / scale
+        e_pr = e_slot.max(1)
+        run_resynced.last = (g, buf, e_slot, P)
         # a walker whose weight is exactly 0 stays 0 for good (the phaseless projection killed
         # it); it is still propagated, its energies grow without bound, and only its weight
         # carries meaning: it must be 0 on both sides
@@ -186,14 +196,25 @@ def run_resynced(shape, steps, hybrid, ortho=5, seed=9):
         assert np.array_equal(dead, g[:, P + A.P_WEIGHT] == 0), t
         e_sm[dead] = 0.0
         e_pr[dead] = 0.0
-        out.append((e_sm, e_pr, kappa))
+        e_slot[dead] = 0.0
+        yield e_sm, e_pr, kappa
     ham.close()
-    return out
+
+
+SLOTS = ["WEIGHT", "PHASE", "PSEUDO_ELOC", "E1", "EXX", "EJ", "OVLP", "LOGSCL_UP", "LOGSCL_DN",
+         "IS_UNITARY", "THETA"]
 
 
 def worst(e, kappa):
     w = int(np.argmax(e))
-    return f"walker {w}: err {e[w]:.3e}, kappa {kappa[w]:.3e}"
+    msg = f"walker {w}: err {e[w]:.3e}, kappa {kappa[w]:.3e}"
+    last = getattr(run_resynced, "last", None)
+    if last is not None:
+        g, buf, e_slot, P = last
+        s = int(np.argmax(e_slot[w]))
+        msg += (f"; worst slot {SLOTS[s]} gpu {g[w, P + s]} ref {buf[w, P + s]}; "
+                f"E ref {buf[w, P + 3:P + 6]}")
+    return msg
 
 
 @pytest.mark.parametrize("hybrid", [True, False])
@@ -240,8 +261,48 @@ def test_bound_statistics_match():
         if t % 5 == 0:
             prop.Orthogonalize(wset)
             ref.orthogonalize(buf)
-    e_gpu, w_gpu = ham.stats()
+    e_gpu, w_gpu, _ = ham.stats()
     e_ref, w_ref = ref.stats()
     assert np.array_equal(e_gpu, e_ref) and np.array_equal(w_gpu, w_ref)
     assert relerr(wset.host(), buf) < TOL
+    ham.close()
+
+
+@pytest.mark.parametrize("W", [1, 5, 64, 512])
+def test_population_control_parity(W):
+    """Comb resampling and single-rank placement are bit-identical to the oracle's restatement
+    of population_control.hpp (integer offspring, copied rows, w/|w|)."""
+    s = synth.make_system(6, 2, 4, seed=1)
+    rng = np.random.default_rng(W)
+    ldw = s.M * s.N + A.P_COUNT
+    P = s.M * s.N
+    ham = A.HamiltonianOperations(s.h, s.Likn, s.PsiT, 0.0, s.dt)
+    prop = A.Propagator(ham)
+    for trial in range(4):
+        buf = rand_c(rng, W, ldw)
+        w = np.abs(rng.standard_normal(W)) * np.exp(1j * rng.uniform(0, 6, W))
+        w[rng.random(W) < 0.3] = 0.0                      # dead walkers are never copied
+        if not np.any(w):
+            w[0] = 1.0
+        buf[:, P + A.P_WEIGHT] = w
+        r = rng.random()
+        wset = A.WalkerSet(ham, W, buf.copy())
+        prop.population_control(wset, r)
+        ref = buf.copy()
+        off, rc = O.population_control(ref, P, r)
+        assert rc == 0 and off.sum() == W
+        got = wset.host()
+        # rows are copied bit for bit; w/|w| may differ in the last ulp (host vs device hypot)
+        keep = np.ones(ldw, bool)
+        keep[P + A.P_WEIGHT] = False
+        assert np.array_equal(got[:, keep], ref[:, keep]), trial
+        np.testing.assert_allclose(got[:, P + A.P_WEIGHT], ref[:, P + A.P_WEIGHT], rtol=1e-15,
+                                   atol=0)
+    # a collapsed population is left alone and counted
+    buf[:, P + A.P_WEIGHT] = 0.0
+    wset = A.WalkerSet(ham, W, buf.copy())
+    prop.population_control(wset, 0.5)
+    assert np.array_equal(wset.host(), buf)
+    _, _, pc = ham.stats()
+    assert pc[0] == 5 and pc[1] == 1
     ham.close()

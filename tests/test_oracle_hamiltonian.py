@@ -231,3 +231,43 @@ def test_bound_weights():
                               st.ctypes.data)
     np.testing.assert_allclose(buf[:, 2 + O.P_WEIGHT], [1.0, 100.0 * np.exp(0.3j), 99.0])
     assert list(st) == [3, 1]
+
+
+def test_comb_closed_form():
+    """population_control.hpp:34-65: teeth (ic + r) * total / n over the cumulative weights."""
+    off, rc = O.comb([1.0, 0.0, 3.0], 0.5)
+    # total 4, teeth 0.667, 2.0, 3.333 over cum (1, 1, 4)
+    assert rc == 0 and list(off) == [1, 0, 2]
+    off, _ = O.comb([0.25, 0.25, 0.25, 0.25], 0.999)
+    assert list(off) == [1, 1, 1, 1]
+    # the last tooth lands on the last walker with weight, not on a zero-weight one behind it
+    off, _ = O.comb([1.0, 1.0, 0.0], 0.9999999999999999)
+    assert list(off) == [1, 2, 0]
+    assert O.comb([0.0, 1e-7], 0.3)[1] == 1          # collapsed total weight
+
+
+def test_comb_is_unbiased():
+    rng = np.random.default_rng(3)
+    w = rng.random(50) ** 3
+    acc = np.zeros(50)
+    n = 4000
+    for _ in range(n):
+        off, _ = O.comb(w, rng.random())
+        assert off.sum() == 50
+        acc += off
+    np.testing.assert_allclose(acc / n, 50 * w / w.sum(), atol=0.05)
+
+
+def test_population_control_placement():
+    # walkers 0..3 with offspring (0, 3, 1, 0): walker 1 keeps slot 1, its two extra copies go
+    # to the zero-offspring slots 0 and 3 in order; weights become w/|w|
+    W, P = 4, 2
+    buf = np.zeros((W, P + O.P_COUNT), complex)
+    buf[:, 0] = [10, 11, 12, 13]
+    buf[:, P + O.P_WEIGHT] = [0.0, 2.0 * 1j, 0.9, 0.0]
+    off, rc = O.population_control(buf, P, 0.1)
+    # total 2.9, teeth (0.1, 1.1, 2.1, 3.1) * 2.9 / 4 over cum (0, 2, 2.9, 2.9)
+    assert rc == 0 and list(off) == [0, 3, 1, 0]
+    for slot, s in {0: 1, 1: 1, 2: 2, 3: 1}.items():
+        assert buf[slot, 0] == 10 + s
+    np.testing.assert_allclose(np.abs(buf[:, P + O.P_WEIGHT]), 1.0)
