@@ -93,7 +93,7 @@ void energy_from_walkers(afq_ham H, hipStream_t s, int W, const double* wbuf, lo
   launch_green(s, W, M, N, H->PsiT, wbuf, ldw, H->ws.G, ov, 1, scl(H), ci_log(H), H->haj,
                H->E0, scl(H), E, 3);
   launch_exchange(s, H->precision, W, M, N, K, H->ws.G, L_for(H), H->ws.Kl, H->ws.exx);
-  launch_energy_finalize(s, W, K, exx_nchunks(K), H->walker_type, H->ws.Kl, H->ws.exx, E);
+  launch_energy_finalize(s, W, K, exx_nchunks(W, N, K), H->walker_type, H->ws.Kl, H->ws.exx, E);
 }
 
 const char* const kRegionNames[R_COUNT] = {"G_for_vbias", "vbias",          "assemble_X",
@@ -233,7 +233,7 @@ int afq_ham_reserve(afq_ham H, int max_walkers) {
     dalloc(&ws.ov, 2 * W);
     dalloc(&ws.E, 6 * W);
     dalloc(&ws.Kl, 2 * W * K);
-    dalloc(&ws.exx, 2 * W * (size_t)exx_nchunks(H->K));
+    dalloc(&ws.exx, 2 * W * (size_t)K);   // at most one chunk per Cholesky vector
     ws.split_elems = (long)kMaxSplits * W * K;
     dalloc(&ws.split, 2 * (size_t)ws.split_elems);
     dalloc(&ws.cum, W);
@@ -391,7 +391,7 @@ int afq_energy(afq_ham H, int W, const double* G, double* E, void* stream) {
     launch_e1(S(stream), W, H->M, H->N, G, H->haj, H->E0, scl(H), E);
     launch_exchange(S(stream), H->precision, W, H->M, H->N, H->K, G, L_for(H), H->ws.Kl,
                     H->ws.exx);
-    launch_energy_finalize(S(stream), W, H->K, exx_nchunks(H->K), H->walker_type, H->ws.Kl,
+    launch_energy_finalize(S(stream), W, H->K, exx_nchunks(W, H->N, H->K), H->walker_type, H->ws.Kl,
                            H->ws.exx, E);
   });
 }
@@ -494,7 +494,7 @@ int afq_step(afq_ham H, int W, double* wbuf, long ldw, const double* xi, double 
     {
       RegionScope r(tm, s, R_WALKER_UPDATE);
       if (!p.hybrid)
-        launch_energy_finalize(s, W, K, exx_nchunks(K), H->walker_type, ws.Kl, ws.exx, ws.E);
+        launch_energy_finalize(s, W, K, exx_nchunks(W, N, K), H->walker_type, ws.Kl, ws.exx, ws.E);
       launch_walker_update(s, W, wbuf, ldw, (long)M * N, p, H->dt, Eshift, H->E0, ws.ov, ws.E,
                            ws.mf, ws.HW, p.hybrid, H->stats);
     }

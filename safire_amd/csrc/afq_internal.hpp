@@ -183,7 +183,7 @@ void launch_gemm_nt_z(hipStream_t s, int precision, int m, int n, int k, double 
                       const double* A, long lda, const void* B, long ldb, bool b_complex,
                       double* C, long ldc, double* split_ws = nullptr, long split_ws_elems = 0);
 // k_energy.hip
-int exx_nchunks(int K);
+int exx_nchunks(int W, int N, int K);   // chunks of Cholesky vectors the exchange kernel uses
 void launch_exchange(hipStream_t s, int precision, int W, int M, int N, int K, const double* G,
                      const void* Lnak, double* Kl, double* exx_part);
 void launch_energy_finalize(hipStream_t s, int W, int K, int nchunk, int walker_type,

@@ -20,10 +20,8 @@ namespace afq {
 
 namespace {
 
-constexpr int kChunk = 32;   // Cholesky vectors per workgroup
-
 template <typename R, int NT, int NV, int WPB>
-__global__ __launch_bounds__(64 * WPB) void k_exchange(int W, int M, int N, int K,
+__global__ __launch_bounds__(64 * WPB) void k_exchange(int W, int M, int N, int K, int kChunk,
                                                       const double* __restrict__ G,
                                                       const R* __restrict__ L,
                                                       double* __restrict__ Kl,
@@ -252,18 +250,60 @@ __global__ __launch_bounds__(256) void k_e1(int NM, const double* __restrict__ G
   }
 }
 
+int device_cus() {
+  static int cus = 0;
+  if (cus == 0) {
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+      cus = prop.multiProcessorCount;
+    if (cus <= 0) cus = 256;
+  }
+  return cus;
+}
+
+// Cholesky vectors per workgroup: a multiple of NV chosen so that the grid fills whole rounds
+// of one workgroup per CU (a partial last round leaves CUs idle while it runs); among equally
+// balanced choices the larger chunk (fewer workgroups, less per-workgroup setup) wins.
+int choose_chunk(int W, int K, int NV, int WPB) {
+  const int cus = device_cus();
+  const int wgroups = (W + WPB - 1) / WPB;
+  int best = NV;
+  double best_eff = -1.0;
+  for (int j = 1; j <= 8; ++j) {
+    const int chunk = NV * j;
+    const int nch = (K + chunk - 1) / chunk;
+    const long nwg = (long)wgroups * nch;
+    const long rounds = (nwg + cus - 1) / cus;
+    // useful vector-work over the work of the rounds the grid occupies
+    const double eff = (double)wgroups * K / ((double)rounds * cus * chunk);
+    if (eff > best_eff + 1e-9 || (eff > best_eff - 1e-9 && chunk > best)) {
+      best_eff = eff;
+      best = chunk;
+    }
+  }
+  return best;
+}
+
+constexpr int exx_nv(int N) { return N <= 16 ? 8 : 2; }
+constexpr int exx_wpb(int N) { return N <= 16 ? 8 : 4; }
+
 template <typename R, int NT, int NV, int WPB>
 void launch_exchange_t(hipStream_t s, int W, int M, int N, int K, const double* G, const R* L,
                        double* Kl, double* exx_part) {
-  const int nchunk = exx_nchunks(K);
+  const int chunk = choose_chunk(W, K, NV, WPB);
+  const int nchunk = (K + chunk - 1) / chunk;
   dim3 grid((W + WPB - 1) / WPB, nchunk);
-  hipLaunchKernelGGL((k_exchange<R, NT, NV, WPB>), grid, dim3(64 * WPB), 0, s, W, M, N, K, G,
-                     L, Kl, exx_part, nchunk);
+  hipLaunchKernelGGL((k_exchange<R, NT, NV, WPB>), grid, dim3(64 * WPB), 0, s, W, M, N, K, chunk,
+                     G, L, Kl, exx_part, nchunk);
 }
 
 }  // namespace
 
-int exx_nchunks(int K) { return (K + kChunk - 1) / kChunk; }
+int exx_nchunks(int W, int N, int K) {
+  const int chunk = choose_chunk(W, K, exx_nv(N), exx_wpb(N));
+  return (K + chunk - 1) / chunk;
+}
 
 void launch_exchange(hipStream_t s, int precision, int W, int M, int N, int K, const double* G,
                      const void* Lnak, double* Kl, double* exx_part) {
@@ -271,14 +311,18 @@ void launch_exchange(hipStream_t s, int precision, int W, int M, int N, int K, c
   AFQ_CHECK(N <= 32, AFQ_EUNSUPPORTED, "exchange energy: more than 32 electrons per spin");
   if (precision == AFQ_FP64) {
     if (N <= 16)
-      launch_exchange_t<double, 1, 8, 8>(s, W, M, N, K, G, (const double*)Lnak, Kl, exx_part);
+      launch_exchange_t<double, 1, exx_nv(16), exx_wpb(16)>(s, W, M, N, K, G,
+                                                            (const double*)Lnak, Kl, exx_part);
     else
-      launch_exchange_t<double, 2, 2, 4>(s, W, M, N, K, G, (const double*)Lnak, Kl, exx_part);
+      launch_exchange_t<double, 2, exx_nv(32), exx_wpb(32)>(s, W, M, N, K, G,
+                                                            (const double*)Lnak, Kl, exx_part);
   } else {
     if (N <= 16)
-      launch_exchange_t<float, 1, 8, 8>(s, W, M, N, K, G, (const float*)Lnak, Kl, exx_part);
+      launch_exchange_t<float, 1, exx_nv(16), exx_wpb(16)>(s, W, M, N, K, G,
+                                                           (const float*)Lnak, Kl, exx_part);
     else
-      launch_exchange_t<float, 2, 2, 4>(s, W, M, N, K, G, (const float*)Lnak, Kl, exx_part);
+      launch_exchange_t<float, 2, exx_nv(32), exx_wpb(32)>(s, W, M, N, K, G,
+                                                           (const float*)Lnak, Kl, exx_part);
   }
   AFQ_HIP(hipGetLastError());
 }

@@ -1,17 +1,20 @@
-// Batched mixed density matrix and log-overlap of closed-shell walkers (one workgroup per walker).
+// Batched mixed density matrix and log-overlap of closed-shell walkers, one wavefront per walker.
 //
 // det_ops::MixedDensityMatrix / detail::log_overlap_impl (density_matrix.hpp:113-156, 535-601):
-//   T = PsiT Phi_w               [N,N]
-//   LU with partial pivoting     (LAPACK getrf: pivot = first max of |re|+|im|)
-//   ov += sum_i log(+-U_ii + DBL_MIN), imaginary part wrapped to [-pi,pi]
-//                                (log_determinant_from_getrf, determinants_impl.hpp:32-75)
-//   T^-1, all zero when some U_ii == 0   (getri_or_zero, nda_functions.hpp:422-438)
-//   G = T^-T Phi_w^T             [N,M] compact Green's function
+//   T = PsiT Phi_w                  [N,N]
+//   partial pivoting (LAPACK getrf: pivot = first max of |re|+|im| in the column)
+//   ov += sum_i log(+-U_ii + DBL_MIN), the sign flipped where rows were swapped, imaginary part
+//        wrapped to [-pi,pi]        (log_determinant_from_getrf, determinants_impl.hpp:32-75)
+//   T^-1, all zero when a pivot is exactly 0 (getri_or_zero, nda_functions.hpp:422-438)
+//   G = T^-T Phi_w^T                [N,M] compact Green's function
 // Optional epilogue: E1 = E0 + scl * sum_{a,j} G[a][j] haj[a][j] (Real3IndexFactorization.hpp:505-508).
 //
-// The walker's Slater matrix, T and T^-1 live in LDS; the LU runs with one barrier per
-// elimination step. Small N (6..32) makes this kernel latency-bound; it carries ~3% of a step's
-// flops.
+// Both products run on FP64 MFMA 16x16x4 tiles with operands read straight from memory (64 bytes
+// per lane and k-chunk). The inverse comes from Gauss-Jordan elimination of [T | I] with partial
+// pivoting: its pivots are getrf's (the rows at and below step j see the same eliminations), so
+// the determinant convention is unchanged. Lane c holds column c of [T | I] (2N <= 64 columns)
+// in registers; each step broadcasts the pivot column through LDS. One 64-thread workgroup per
+// walker: no workgroup barrier, and W workgroups spread over all CUs.
 #include "afq_internal.hpp"
 
 #include <cfloat>
@@ -20,95 +23,170 @@ namespace afq {
 
 namespace {
 
-constexpr int kThreads = 256;
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
-__device__ inline double shfl_xor_d(double v, int m) { return __shfl_xor(v, m, 64); }
-
-__global__ __launch_bounds__(kThreads) void k_green(int M, int N, const cd* __restrict__ PsiT,
-                                                    const cd* __restrict__ Phi, long ldw,
-                                                    cd* __restrict__ G, cd* __restrict__ ov,
-                                                    int ov_mode, double ov_scale, cd ov_add,
-                                                    const cd* __restrict__ haj, double E0,
-                                                    double e1_scale, cd* __restrict__ E,
-                                                    long ldE) {
-  extern __shared__ cd smem[];
-  cd* sPhi = smem;           // [M][N]
-  cd* sT = sPhi + M * N;     // [N][N]  LU factors
-  cd* sX = sT + N * N;       // [N][N]  inverse
-  __shared__ int ipiv[64];
-  __shared__ int singular;
-  __shared__ double red[2 * (kThreads / 64)];
-
+template <int NMAX, bool STAGE>
+__global__ __launch_bounds__(64) void k_green(int M, int N, const cd* __restrict__ PsiT,
+                                              const cd* __restrict__ Phi, long ldw,
+                                              cd* __restrict__ G, cd* __restrict__ ov,
+                                              int ov_mode, double ov_scale, cd ov_add,
+                                              const cd* __restrict__ haj, double E0,
+                                              double e1_scale, cd* __restrict__ E, long ldE) {
+  constexpr int NT = (NMAX + 15) / 16;     // 16x16 tiles along N
+  constexpr int NB = 16 * NT;
+  __shared__ cd sT[NB * NB];               // T, then T^-1, row-major
+  __shared__ cd sCol[NMAX];                // pivot column broadcast
+  extern __shared__ cd sPhi[];             // Phi_w [M][N] when STAGE
+  const int lane = threadIdx.x;
+  const int g = lane >> 4, r16 = lane & 15;
   const int w = blockIdx.x;
-  const int tid = threadIdx.x;
-  const int lane = tid & 63;
-  const cd* P = Phi + (long)w * ldw;
-
-  for (int e = tid; e < M * N; e += kThreads) sPhi[e] = P[e];
-  __syncthreads();
-
-  // T[a][b] = sum_i PsiT[a][i] Phi[i][b]
-  for (int e = tid; e < N * N; e += kThreads) {
-    const int a = e / N, b = e - a * N;
-    const cd* pa = PsiT + (long)a * M;
-    cd s = cmk(0.0, 0.0);
-    for (int i = 0; i < M; ++i) s += pa[i] * sPhi[i * N + b];
-    sT[e] = s;
+  const cd* Pg = Phi + (long)w * ldw;
+  if (STAGE) {
+    for (int e = lane; e < M * N; e += 64) sPhi[e] = Pg[e];
+    wave_sync();
   }
-  __syncthreads();
+  const cd* P = STAGE ? sPhi : Pg;
 
-  // right-looking LU with partial pivoting (zgetf2 order of operations)
-  for (int j = 0; j < N; ++j) {
-    if (tid < 64) {
-      const int i = j + lane;
-      double v = (i < N) ? cabs1(sT[i * N + j]) : -1.0;
-      int idx = (i < N) ? i : 0x7fffffff;
-      for (int off = 32; off > 0; off >>= 1) {
-        double ov_ = shfl_xor_d(v, off);
-        int oi = __shfl_xor(idx, off, 64);
-        if (ov_ > v || (ov_ == v && oi < idx)) {
-          v = ov_;
-          idx = oi;
+  // ---- T[a][c] = sum_i PsiT[a][i] Phi[i][c]  (MFMA; k = orbital i) ----
+  {
+    d4 tr[NT][NT], ti[NT][NT];
+#pragma unroll
+    for (int a = 0; a < NT; ++a)
+#pragma unroll
+      for (int b = 0; b < NT; ++b) {
+        tr[a][b] = d4{0, 0, 0, 0};
+        ti[a][b] = d4{0, 0, 0, 0};
+      }
+    for (int k0 = 0; k0 < M; k0 += 16) {
+      double ar[NT][4], ai[NT][4], br[NT][4], bi[NT][4];
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const int a = t * 16 + r16;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          const int k = k0 + 4 * g + s;
+          const cd x = (a < N && k < M) ? PsiT[(long)a * M + k] : cmk(0, 0);
+          ar[t][s] = x.re;
+          ai[t][s] = x.im;
+          const int c = t * 16 + r16;
+          const cd y = (c < N && k < M) ? P[(long)k * N + c] : cmk(0, 0);
+          br[t][s] = y.re;
+          bi[t][s] = y.im;
         }
       }
-      if (lane == 0) ipiv[j] = idx;
-    }
-    __syncthreads();
-    const int p = ipiv[j];
-    if (p != j)
-      for (int c = tid; c < N; c += kThreads) {
-        cd t = sT[j * N + c];
-        sT[j * N + c] = sT[p * N + c];
-        sT[p * N + c] = t;
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+#pragma unroll
+        for (int a = 0; a < NT; ++a)
+#pragma unroll
+          for (int b = 0; b < NT; ++b) {
+            tr[a][b] = mfma16x16x4(ar[a][s], br[b][s], tr[a][b]);
+            ti[a][b] = mfma16x16x4(ar[a][s], bi[b][s], ti[a][b]);
+          }
+#pragma unroll
+        for (int a = 0; a < NT; ++a)
+#pragma unroll
+          for (int b = 0; b < NT; ++b) {
+            tr[a][b] = mfma16x16x4(-ai[a][s], bi[b][s], tr[a][b]);
+            ti[a][b] = mfma16x16x4(ai[a][s], br[b][s], ti[a][b]);
+          }
       }
-    __syncthreads();
-    const cd piv = sT[j * N + j];
-    if (!cis0(piv)) {
-      const cd r = cdiv(cmk(1.0, 0.0), piv);
-      for (int i = j + 1 + tid; i < N; i += kThreads) sT[i * N + j] = sT[i * N + j] * r;
     }
-    __syncthreads();
-    const int nt = N - 1 - j;
-    for (int e = tid; e < nt * nt; e += kThreads) {
-      const int i = j + 1 + e / nt, c = j + 1 + e % nt;
-      const cd l = sT[i * N + j];
-      if (!cis0(l)) sT[i * N + c] -= l * sT[j * N + c];
+#pragma unroll
+    for (int a = 0; a < NT; ++a)
+#pragma unroll
+      for (int b = 0; b < NT; ++b)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          sT[(a * 16 + acc_row<double>(g, r)) * NB + b * 16 + r16] = cmk(tr[a][b][r], ti[a][b][r]);
+  }
+  wave_sync();
+
+  // ---- Gauss-Jordan on [T | I]: lane c < 2N holds column c ----
+  cd x[NMAX];
+#pragma unroll
+  for (int r = 0; r < NMAX; ++r) {
+    cd v = cmk(0.0, 0.0);
+    if (r < N) {
+      if (lane < N) v = sT[r * NB + lane];
+      else if (lane - N == r) v = cmk(1.0, 0.0);
     }
-    __syncthreads();
+    x[r] = v;
+  }
+  cd upiv = cmk(1.0, 0.0);     // lane j keeps +-U_jj; the logs are taken in parallel afterwards
+  bool singular = false;
+  for (int j = 0; j < N; ++j) {
+    wave_sync();
+    if (lane == j)
+#pragma unroll
+      for (int r = 0; r < NMAX; ++r)
+        if (r < N) sCol[r] = x[r];
+    wave_sync();
+    double v = -1.0;
+    int idx = 0x7fffffff;
+    if (lane >= j && lane < N) {
+      v = cabs1(sCol[lane]);
+      idx = lane;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      const double ov_ = __shfl_xor(v, off, 64);
+      const int oi = __shfl_xor(idx, off, 64);
+      if (ov_ > v || (ov_ == v && oi < idx)) {
+        v = ov_;
+        idx = oi;
+      }
+    }
+    const int p = idx;
+    const cd piv = sCol[p];
+    if (lane == j) upiv = (p == j) ? piv : -piv;   // the sign flips where the pivot row moved
+    if (cis0(piv)) {
+      singular = true;   // getrf records the zero pivot and goes on; getri_or_zero zeroes T^-1
+      continue;
+    }
+    if (p != j) {
+      cd xj = cmk(0, 0), xp = cmk(0, 0);
+#pragma unroll
+      for (int r = 0; r < NMAX; ++r) {
+        if (r == j) xj = x[r];
+        if (r == p) xp = x[r];
+      }
+#pragma unroll
+      for (int r = 0; r < NMAX; ++r) {
+        if (r == j) x[r] = xp;
+        if (r == p) x[r] = xj;
+      }
+    }
+    const cd rinv = cdiv(cmk(1.0, 0.0), piv);
+    cd xjn = cmk(0, 0);
+#pragma unroll
+    for (int r = 0; r < NMAX; ++r)
+      if (r == j) xjn = x[r] * rinv;
+#pragma unroll
+    for (int r = 0; r < NMAX; ++r) {
+      if (r < N) {
+        const int rr = (r == j) ? p : ((r == p) ? j : r);   // row r's column-j entry, pre-swap
+        const cd f = sCol[rr];
+        if (r == j) x[r] = xjn;
+        else if (!cis0(f)) x[r] -= f * xjn;
+      }
+    }
   }
 
-  if (tid == 0) {
-    cd acc = (ov_mode == 2) ? ov[w] : cmk(0.0, 0.0);
-    int sing = 0;
-    for (int i = 0; i < N; ++i) {
-      const cd u = sT[i * N + i];
-      if (cis0(u)) sing = 1;
-      const cd x = (ipiv[i] == i) ? u : -u;
-      acc += clog_(cmk(x.re + DBL_MIN, x.im));
-    }
+  // sum_j log(+-U_jj + DBL_MIN): one log per lane, then a wavefront sum
+  cd acc = (lane < N) ? clog_(cmk(upiv.re + DBL_MIN, upiv.im)) : cmk(0.0, 0.0);
+  for (int off = 32; off > 0; off >>= 1) {
+    acc.re += __shfl_xor(acc.re, off, 64);
+    acc.im += __shfl_xor(acc.im, off, 64);
+  }
+  if (lane == 0) {
+    if (ov_mode == 2) acc += ov[w];
     const double pi = 3.14159265358979323846;
-    // the reference loops until the value is in range; a non-finite accumulated input would never
-    // get there, so only finite values are wrapped
+    // the reference loops until the value is in range; a non-finite value would never get
+    // there, so only finite values are wrapped
     if (!isfinite(acc.im)) {
     } else if (acc.im > pi) {
       while (acc.im > pi) acc.im -= 2 * pi;
@@ -116,62 +194,76 @@ __global__ __launch_bounds__(kThreads) void k_green(int M, int N, const cd* __re
       while (acc.im < -pi) acc.im += 2 * pi;
     }
     if (ov_mode != 0) ov[w] = ov_scale * acc + ov_add;
-    singular = sing;
   }
-  __syncthreads();
   if (G == nullptr && E == nullptr) return;
 
-  // T^-1 by column: X = U^-1 L^-1 P e_c  (the getri_or_zero result)
-  if (!singular && tid < N) {
-    const int c = tid;
-    for (int i = 0; i < N; ++i) sX[i * N + c] = cmk(i == c ? 1.0 : 0.0, 0.0);
-    for (int i = 0; i < N; ++i) {
-      const int p = ipiv[i];
-      if (p != i) {
-        cd t = sX[i * N + c];
-        sX[i * N + c] = sX[p * N + c];
-        sX[p * N + c] = t;
-      }
-    }
-    for (int i = 0; i < N; ++i) {
-      cd s = sX[i * N + c];
-      for (int k = 0; k < i; ++k) s -= sT[i * N + k] * sX[k * N + c];
-      sX[i * N + c] = s;
-    }
-    for (int i = N - 1; i >= 0; --i) {
-      cd s = sX[i * N + c];
-      for (int k = i + 1; k < N; ++k) s -= sT[i * N + k] * sX[k * N + c];
-      sX[i * N + c] = cdiv(s, sT[i * N + i]);
-    }
+  // ---- T^-1 to LDS as [b][a] (zero padded to NB x NB) ----
+  wave_sync();
+  for (int e = lane; e < NB * NB; e += 64) sT[e] = cmk(0, 0);
+  wave_sync();
+  if (lane >= N && lane < 2 * N && !singular) {
+    const int a = lane - N;
+#pragma unroll
+    for (int r = 0; r < NMAX; ++r)
+      if (r < N) sT[r * NB + a] = x[r];
   }
-  __syncthreads();
+  wave_sync();
 
-  // G[a][i] = sum_b X[b][a] Phi[i][b];  E1 partial sum_{a,i} G[a][i] haj[a][i]
+  // ---- G[a][i] = sum_b Tinv[b][a] Phi[i][b]  (MFMA; rows a, cols i, k = b) ----
   cd e1 = cmk(0.0, 0.0);
   cd* Gw = G ? G + (long)w * N * M : nullptr;
-  for (int e = tid; e < N * M; e += kThreads) {
-    const int a = e / M, i = e - a * M;
-    cd s = cmk(0.0, 0.0);
-    if (!singular)
-      for (int b = 0; b < N; ++b) s += sX[b * N + a] * sPhi[i * N + b];
-    if (Gw) Gw[e] = s;
-    if (haj) e1 += s * haj[e];
+  for (int i0 = 0; i0 < M; i0 += 16) {
+    d4 gr[NT], gi[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      gr[t] = d4{0, 0, 0, 0};
+      gi[t] = d4{0, 0, 0, 0};
+    }
+    const int i = i0 + r16;
+#pragma unroll
+    for (int b0 = 0; b0 < NB; b0 += 16) {
+      double ar[NT][4], ai[NT][4], br[4], bi[4];
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const int b = b0 + 4 * g + s;
+        const cd y = (i < M && b < N) ? P[(long)i * N + b] : cmk(0, 0);
+        br[s] = y.re;
+        bi[s] = y.im;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+          const cd z = sT[b * NB + t * 16 + r16];   // Tinv^T[a][b] = Tinv[b][a]
+          ar[t][s] = z.re;
+          ai[t][s] = z.im;
+        }
+      }
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+          gr[t] = mfma16x16x4(ar[t][s], br[s], gr[t]);
+          gi[t] = mfma16x16x4(ar[t][s], bi[s], gi[t]);
+          gr[t] = mfma16x16x4(-ai[t][s], bi[s], gr[t]);
+          gi[t] = mfma16x16x4(ai[t][s], br[s], gi[t]);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int a = t * 16 + acc_row<double>(g, r);
+        if (a < N && i < M) {
+          const cd v = cmk(gr[t][r], gi[t][r]);
+          if (Gw) Gw[(long)a * M + i] = v;
+          if (haj) e1 += v * haj[(long)a * M + i];
+        }
+      }
   }
   if (E != nullptr && haj != nullptr) {
     for (int off = 32; off > 0; off >>= 1) {
-      e1.re += shfl_xor_d(e1.re, off);
-      e1.im += shfl_xor_d(e1.im, off);
+      e1.re += __shfl_xor(e1.re, off, 64);
+      e1.im += __shfl_xor(e1.im, off, 64);
     }
-    if (lane == 0) {
-      red[2 * (tid >> 6)] = e1.re;
-      red[2 * (tid >> 6) + 1] = e1.im;
-    }
-    __syncthreads();
-    if (tid == 0) {
-      cd s = cmk(0.0, 0.0);
-      for (int q = 0; q < kThreads / 64; ++q) s += cmk(red[2 * q], red[2 * q + 1]);
-      E[(long)w * ldE] = cmk(E0, 0.0) + e1_scale * s;
-    }
+    if (lane == 0) E[(long)w * ldE] = cmk(E0, 0.0) + e1_scale * e1;
   }
 }
 
@@ -181,18 +273,26 @@ void launch_green(hipStream_t s, int nb, int M, int N, const double* PsiT, const
                   long ldw, double* G, double* ov, int ov_mode, double ov_scale, cd ov_add,
                   const double* haj, double E0, double e1_scale, double* E, long ldE) {
   if (nb <= 0 || N <= 0) return;
-  AFQ_CHECK(N <= 64, AFQ_EUNSUPPORTED, "mixed density matrix: N > 64 electrons per spin");
-  const size_t lds = sizeof(cd) * ((size_t)M * N + 2 * (size_t)N * N);
-  AFQ_CHECK(lds <= 160 * 1024 - 1024, AFQ_EUNSUPPORTED,
-            "mixed density matrix: walker does not fit in LDS");
-  if (lds > 64 * 1024)
-    AFQ_HIP(hipFuncSetAttribute((const void*)k_green,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  hipLaunchKernelGGL(k_green, dim3(nb), dim3(kThreads), lds, s, M, N,
-                     reinterpret_cast<const cd*>(PsiT), reinterpret_cast<const cd*>(Phi), ldw,
-                     reinterpret_cast<cd*>(G), reinterpret_cast<cd*>(ov), ov_mode, ov_scale,
-                     ov_add, reinterpret_cast<const cd*>(haj), E0, e1_scale,
-                     reinterpret_cast<cd*>(E), ldE);
+  AFQ_CHECK(N <= 32, AFQ_EUNSUPPORTED, "mixed density matrix: more than 32 electrons per spin");
+  // the walker's matrix goes through LDS (coalesced loads, two reads) when it is small
+  const size_t phi_bytes = sizeof(cd) * (size_t)M * N;
+  const bool stage = phi_bytes <= 32 * 1024;
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3(nb), dim3(64), stage ? phi_bytes : 0, s, M, N,
+                       reinterpret_cast<const cd*>(PsiT), reinterpret_cast<const cd*>(Phi), ldw,
+                       reinterpret_cast<cd*>(G), reinterpret_cast<cd*>(ov), ov_mode, ov_scale,
+                       ov_add, reinterpret_cast<const cd*>(haj), E0, e1_scale,
+                       reinterpret_cast<cd*>(E), ldE);
+  };
+  if (stage) {
+    if (N <= 8) go(k_green<8, true>);
+    else if (N <= 16) go(k_green<16, true>);
+    else go(k_green<32, true>);
+  } else {
+    if (N <= 8) go(k_green<8, false>);
+    else if (N <= 16) go(k_green<16, false>);
+    else go(k_green<32, false>);
+  }
   AFQ_HIP(hipGetLastError());
 }
 

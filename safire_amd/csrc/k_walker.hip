@@ -223,6 +223,121 @@ __global__ void k_comb_plan(int W, const cd* __restrict__ wbuf, long ldw, long p
   ok[0] = 1;
 }
 
+// The same comb in parallel, one workgroup. The cumulative weights are summed in the
+// reference's sequential order (one thread, from LDS), because their rounding decides ties.
+// Tooth ic sits at comb_ic = (ic + r) * total / W (the reference's expression, monotone in ic);
+// it lands on the first walker i with comb_ic < cum[i], or on the last walker with weight.
+// So offspring[i] = count(cum[i]) - count(cum[i-1]) with count(x) = #{ic : comb_ic < x}, found
+// by binary search; the copy destinations follow from exclusive scans of the extra copies and of
+// the zero-offspring slots (integers, exact).
+__device__ inline double comb_tooth(int ic, double r, double total, int W) {
+  return ((double)ic + r) * total / W;
+}
+
+__device__ inline int comb_count(double x, double r, double total, int W) {
+  int lo = 0, hi = W;   // first ic with comb_ic >= x
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (comb_tooth(mid, r, total, W) < x) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// exclusive scan of v[0..n) in place (integers), one workgroup of 256 threads
+__device__ void block_exclusive_scan(int* v, int n, int* tmp256) {
+  const int t = threadIdx.x;
+  const int per = (n + 255) / 256;
+  const int b = t * per, e = min(n, b + per);
+  int s = 0;
+  for (int i = b; i < e; ++i) s += v[i];
+  tmp256[t] = s;
+  __syncthreads();
+  if (t == 0) {
+    int acc = 0;
+    for (int q = 0; q < 256; ++q) {
+      const int x = tmp256[q];
+      tmp256[q] = acc;
+      acc += x;
+    }
+  }
+  __syncthreads();
+  int acc = tmp256[t];
+  for (int i = b; i < e; ++i) {
+    const int x = v[i];
+    v[i] = acc;
+    acc += x;
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void k_comb_parallel(int W, const cd* __restrict__ wbuf,
+                                                       long ldw, long poff, double r,
+                                                       int* __restrict__ src,
+                                                       int* __restrict__ ok,
+                                                       unsigned long long* __restrict__ stats) {
+  extern __shared__ double smem[];
+  double* cum = smem;                               // [W]
+  int* off = reinterpret_cast<int*>(cum + W);       // [W]
+  int* ext = off + W;                               // [W] extra copies -> base
+  int* zer = ext + W;                               // [W] zero flags -> free index
+  __shared__ int tmp[256];
+  __shared__ int s_last;
+  __shared__ double s_total;
+  const int t = threadIdx.x;
+  for (int i = t; i < W; i += 256) {
+    cum[i] = cabs_(wbuf[(long)i * ldw + poff + AFQ_P_WEIGHT]);
+    src[i] = -1;
+  }
+  __syncthreads();
+  if (t == 0) {
+    double s = 0.0;
+    int last = -1;
+    for (int i = 0; i < W; ++i) {
+      const double a = cum[i];
+      if (a > 0.0) last = i;
+      s += a;
+      cum[i] = s;
+    }
+    s_total = s;
+    s_last = last;
+    ok[0] = 0;
+    if (stats) atomicAdd(&stats[5], 1ull);
+    if (!(s > 1e-6) || last < 0) {
+      if (stats) atomicAdd(&stats[6], 1ull);   // collapsed: walkers left unchanged
+    }
+  }
+  __syncthreads();
+  const double total = s_total;
+  const int last = s_last;
+  if (!(total > 1e-6) || last < 0) return;
+  for (int i = t; i < W; i += 256) {
+    int n = 0;
+    if (i < last) {
+      const int hi = comb_count(cum[i], r, total, W);
+      const int lo = i > 0 ? comb_count(cum[i - 1], r, total, W) : 0;
+      n = hi - lo;
+    } else if (i == last) {
+      n = W - (i > 0 ? comb_count(cum[i - 1], r, total, W) : 0);
+    }
+    off[i] = n;
+    ext[i] = n > 1 ? n - 1 : 0;
+    zer[i] = n == 0 ? 1 : 0;
+  }
+  __syncthreads();
+  block_exclusive_scan(ext, W, tmp);   // ext[i]: first free-list entry of walker i's copies
+  block_exclusive_scan(zer, W, tmp);   // zer[i]: free-list entry of zero-offspring slot i
+  int* freelist = reinterpret_cast<int*>(cum);      // cum is no longer needed
+  for (int i = t; i < W; i += 256)
+    if (off[i] == 0) freelist[zer[i]] = i;
+  __syncthreads();
+  for (int i = t; i < W; i += 256) {
+    const int n = off[i];
+    for (int c = 1; c < n; ++c) src[freelist[ext[i] + c - 1]] = i;
+  }
+  if (t == 0) ok[0] = 1;
+}
+
 __global__ void k_copy_walkers(int W, cd* __restrict__ wbuf, long ldw,
                                const int* __restrict__ src) {
   const int slot = blockIdx.x;
@@ -305,8 +420,17 @@ void launch_population_control(hipStream_t s, int W, double* wbuf, long ldw, lon
                                double r, double* cum, int* off, int* src, int* ok,
                                unsigned long long* stats) {
   if (W <= 0) return;
-  hipLaunchKernelGGL(k_comb_plan, dim3(1), dim3(64), 0, s, W, reinterpret_cast<const cd*>(wbuf),
-                     ldw, poff, r, cum, off, src, ok, stats);
+  const size_t lds = (size_t)W * (sizeof(double) + 3 * sizeof(int));
+  if (lds <= 120 * 1024) {
+    if (lds > 64 * 1024)
+      AFQ_HIP(hipFuncSetAttribute((const void*)k_comb_parallel,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k_comb_parallel, dim3(1), dim3(256), lds, s, W,
+                       reinterpret_cast<const cd*>(wbuf), ldw, poff, r, src, ok, stats);
+  } else {
+    hipLaunchKernelGGL(k_comb_plan, dim3(1), dim3(64), 0, s, W,
+                       reinterpret_cast<const cd*>(wbuf), ldw, poff, r, cum, off, src, ok, stats);
+  }
   hipLaunchKernelGGL(k_copy_walkers, dim3(W), dim3(256), 0, s, W, reinterpret_cast<cd*>(wbuf),
                      ldw, (const int*)src);
   hipLaunchKernelGGL(k_normalize_weights, dim3((W + 127) / 128), dim3(128), 0, s, W,

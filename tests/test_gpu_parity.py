@@ -268,7 +268,7 @@ def test_bound_statistics_match():
     ham.close()
 
 
-@pytest.mark.parametrize("W", [1, 5, 64, 512])
+@pytest.mark.parametrize("W", [1, 5, 64, 512, 1000, 3000])
 def test_population_control_parity(W):
     """Comb resampling and single-rank placement are bit-identical to the oracle's restatement
     of population_control.hpp (integer offspring, copied rows, w/|w|)."""
@@ -282,6 +282,8 @@ def test_population_control_parity(W):
         buf = rand_c(rng, W, ldw)
         w = np.abs(rng.standard_normal(W)) * np.exp(1j * rng.uniform(0, 6, W))
         w[rng.random(W) < 0.3] = 0.0                      # dead walkers are never copied
+        if trial % 2:
+            w = np.round(np.abs(w) * 4) / 4               # ties: teeth on exact boundaries
         if not np.any(w):
             w[0] = 1.0
         buf[:, P + A.P_WEIGHT] = w
