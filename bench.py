@@ -30,7 +30,10 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 METRIC = "walker-steps/s mixed vs FP64 at |ΔE_loc|<1e-5 Ha; % MFMA/HBM peak"
-PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
+# dense MFMA peak of the arithmetic each form's exchange kernel runs on
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0, "int8": 5000.0}
+# executed MFMA flops per algorithmic flop of the exchange contraction
+EXECUTED_PER_ALGO = {"fp64": 1.0, "fp32": 1.0, "bf16x3": 6.0, "int8": 15.0}
 GATE_HA = 1e-5
 
 
@@ -139,7 +142,7 @@ def main(argv=None):
     ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--forms", default="fp64,fp32")
+    ap.add_argument("--forms", default="fp64,fp32,bf16x3,int8")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the walker step runs on a single MI355X here and "
@@ -165,7 +168,7 @@ def main(argv=None):
     uniforms = np.random.default_rng([s.seed, 31]).random(nst)
     forms = {}
     for f in args.forms.split(","):
-        prec = {"fp64": A.FP64, "fp32": A.FP32}[f]
+        prec = {"fp64": A.FP64, "fp32": A.FP32, "bf16x3": A.BF16X3, "int8": A.INT8_SPLIT}[f]
         forms[f] = run_form(A, torch, s, W, xi_dev, uniforms, args.warmup, args.steps, ortho,
                             prec, hybrid=False)
         r = forms[f]
@@ -240,7 +243,8 @@ def main(argv=None):
                      "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": None,
                      "kernel": f"k_exchange ({mixed} MFMA)",
                      "flop_per_launch": exchange_flops(W, M, N, K),
-                     "mean_launch_ms": round(ex_avg, 5)},
+                     "mean_launch_ms": round(ex_avg, 5),
+                     "executed_frac": round(achieved * EXECUTED_PER_ALGO[mixed] / peak, 4)},
     }
     if ref_form is not None:
         v64 = W * args.steps / (ref_form["ms"] / 1e3)

@@ -46,8 +46,17 @@ enum afq_walker_type { AFQ_CLOSED = 1, AFQ_COLLINEAR = 2 };
 enum afq_precision {
   AFQ_FP64 = 0,        /* plain form: everything FP64 (FP64 MFMA)                          */
   AFQ_FP32 = 1,        /* contractions in FP32 on MFMA with FP32 accumulation               */
-  AFQ_BF16X3 = 2       /* contractions as 3-product split-complex BF16 with hi/lo splits,
-                          FP32 accumulation                                                */
+  AFQ_BF16X3 = 2,      /* mixed form: the exchange energy on BF16 MFMA with both operands
+                          split into three BF16 slices (hi/mid/lo, ~24 bits) and the six
+                          slice products of order <= 2 accumulated in FP32, per real/imag
+                          part; vbias and vHS as in AFQ_FP32. Needs N <= 16, M <= 96 and
+                          M % 8 == 0 (AFQ_EUNSUPPORTED otherwise)                           */
+  AFQ_INT8_SPLIT = 3   /* mixed form: the exchange energy on int8 MFMA with every row of G
+                          and Lnak scaled by a power of two and split into 5 int8 slices
+                          (x = u (q0 + q1/254 + ... + q4/254^4)); the 15 slice products of
+                          order <= 4 accumulate exactly in int32 and are combined in FP64
+                          (energy error ~1e-3 of AFQ_FP32's). vbias and vHS as in AFQ_FP32.
+                          Needs N <= 16 and M <= 128 (AFQ_EUNSUPPORTED otherwise)           */
 };
 
 /* Walker property slots (WalkerConfig.hpp:28-48, offsets WalkerSetBase.cpp:87-117). */
@@ -198,9 +207,12 @@ int afq_ham_enable_timers(afq_ham ham, int max_records);
 int afq_ham_read_timers(afq_ham ham, double* h_ms, long long* h_counts);
 const char* afq_timer_name(int region);
 
-/* Lane layout of the MFMA 16x16x4 builtins (kind 0: FP64, 1: FP32), for a self-check:
- * D = A B with A[i][k] = i + 100 k and B[k][j] = (k == j % 4); d_out [64][4] f64 receives
- * accumulator element r of every lane. */
+/* Lane layout of the MFMA builtins the kernels use, for a self-check; d_out [64][4] f64
+ * receives accumulator element r of every lane.
+ * kind 0 / 1: FP64 / FP32 16x16x4, D = A B with A[i][k] = i + 100 k, B[k][j] = (k == j % 4).
+ * kind 2 / 3: BF16 16x16x32, D = A B with A[i][k] = i + 1 and B[k][j] = j + 1 only for k = 0
+ *             (kind 2) or k = 31 (kind 3), so D[i][j] = (i + 1)(j + 1).
+ * kind 4 / 5: int8 16x16x64 (i32 accumulator), as kinds 2 / 3 with k = 0 or k = 63. */
 int afq_probe_mfma(int kind, double* d_out, void* stream);
 
 #ifdef __cplusplus

@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libsafire_afq.so")
 
 CLOSED, COLLINEAR = 1, 2
-FP64, FP32, BF16X3 = 0, 1, 2
+FP64, FP32, BF16X3, INT8_SPLIT = 0, 1, 2, 3
 (P_WEIGHT, P_PHASE, P_PSEUDO_ELOC, P_E1, P_EXX, P_EJ, P_OVLP, P_LOGSCL_UP, P_LOGSCL_DN,
  P_IS_UNITARY, P_THETA, P_COUNT) = range(12)
 _ERRORS = {1: "AFQ_EBADSHAPE", 2: "AFQ_EHIP", 3: "AFQ_ENOMEM", 4: "AFQ_EINVAL",

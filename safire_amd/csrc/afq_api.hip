@@ -81,19 +81,41 @@ cd ci_log(afq_ham H) {
   return clog_(cmk(H->ci0_re, -H->ci0_im));
 }
 
+// vbias / vHS operands: FP64 in the plain form, the FP32 copies in both reduced forms
 const void* L_for(afq_ham H) { return H->precision == AFQ_FP64 ? (const void*)H->Lnak : (const void*)H->Lnak_f; }
 const void* Likn_for(afq_ham H) { return H->precision == AFQ_FP64 ? (const void*)H->Likn : (const void*)H->Likn_f; }
 double scl(afq_ham H) { return H->walker_type == AFQ_CLOSED ? 2.0 : 1.0; }
+
+// the exchange contraction in the handle's precision: Kl and per-chunk tr(T^2) partials
+void run_exchange(afq_ham H, hipStream_t s, int W, const double* G) {
+  if (H->precision == AFQ_BF16X3)
+    launch_exchange_bf16(s, W, H->M, H->N, H->K, G, H->Lnak_b, H->ws.Kl, H->ws.exx);
+  else if (H->precision == AFQ_INT8_SPLIT)
+    launch_exchange_i8(s, W, H->M, H->N, H->K, G, H->Lnak_q, H->Lnak_u, H->ws.Kl, H->ws.exx);
+  else
+    launch_exchange(s, H->precision, W, H->M, H->N, H->K, G, L_for(H), H->ws.Kl, H->ws.exx);
+}
+
+int exchange_chunks(afq_ham H, int W) {
+  if (H->precision == AFQ_BF16X3) return exchange_bf16_chunks(W, H->K);
+  if (H->precision == AFQ_INT8_SPLIT) return exchange_i8_chunks(W, H->K);
+  return exx_nchunks(W, H->N, H->K);
+}
+
+void finalize_energy(afq_ham H, hipStream_t s, int W, double* E) {
+  launch_energy_finalize(s, W, H->K, exchange_chunks(H, W), H->walker_type, H->ws.Kl,
+                         H->ws.exx, E);
+}
 
 // G, ov (= 2 logdet + log conj c0) and E (E1 by the Green's function kernel, EXX/EJ by the
 // exchange kernel) of all walkers into the workspace (or E/ov given)
 void energy_from_walkers(afq_ham H, hipStream_t s, int W, const double* wbuf, long ldw,
                          double* E, double* ov) {
-  const int M = H->M, N = H->N, K = H->K;
+  const int M = H->M, N = H->N;
   launch_green(s, W, M, N, H->PsiT, wbuf, ldw, H->ws.G, ov, 1, scl(H), ci_log(H), H->haj,
                H->E0, scl(H), E, 3);
-  launch_exchange(s, H->precision, W, M, N, K, H->ws.G, L_for(H), H->ws.Kl, H->ws.exx);
-  launch_energy_finalize(s, W, K, exx_nchunks(W, N, K), H->walker_type, H->ws.Kl, H->ws.exx, E);
+  run_exchange(H, s, W, H->ws.G);
+  finalize_energy(H, s, W, E);
 }
 
 const char* const kRegionNames[R_COUNT] = {"G_for_vbias", "vbias",          "assemble_X",
@@ -185,6 +207,15 @@ int afq_ham_create(int M, int N, int K, const double* h_h, const double* h_Likn,
       AFQ_HIP(hipMemcpy(H->h, h_h, MM * sizeof(double), hipMemcpyHostToDevice));
       AFQ_HIP(hipMemset(H->stats, 0, 7 * sizeof(unsigned long long)));
       setup_hamiltonian(H, nullptr);
+      if (exchange_bf16_supported(M, N)) {
+        dalloc(&H->Lnak_b, 6 * NM * K);
+        split_lnak_bf16(nullptr, M, N, K, H->Lnak, H->Lnak_b);
+      }
+      if (exchange_i8_supported(M, N)) {
+        dalloc(&H->Lnak_q, exchange_i8_bytes(M, N, K));
+        dalloc(&H->Lnak_u, (size_t)K * N);
+        split_lnak_i8(nullptr, M, N, K, H->Lnak, H->Lnak_q, H->Lnak_u);
+      }
       AFQ_HIP(hipDeviceSynchronize());
     } catch (...) {
       afq_ham_destroy(H);
@@ -210,6 +241,9 @@ int afq_ham_destroy(afq_ham H) {
     dfree(H->H1);
     dfree(H->Lnak_f);
     dfree(H->Likn_f);
+    dfree(H->Lnak_b);
+    dfree(H->Lnak_q);
+    dfree(H->Lnak_u);
     dfree(H->stats);
     delete H;
   });
@@ -247,8 +281,12 @@ int afq_ham_reserve(afq_ham H, int max_walkers) {
 int afq_ham_set_precision(afq_ham H, int precision) {
   return guard([&] {
     check_ham(H);
-    AFQ_CHECK(precision == AFQ_FP64 || precision == AFQ_FP32, AFQ_EUNSUPPORTED,
-              "precision not available");
+    AFQ_CHECK(precision >= AFQ_FP64 && precision <= AFQ_INT8_SPLIT, AFQ_EINVAL,
+              "unknown precision");
+    AFQ_CHECK(precision != AFQ_BF16X3 || H->Lnak_b != nullptr, AFQ_EUNSUPPORTED,
+              "BF16X3 exchange needs N <= 16 and M <= 96 with M % 8 == 0");
+    AFQ_CHECK(precision != AFQ_INT8_SPLIT || H->Lnak_q != nullptr, AFQ_EUNSUPPORTED,
+              "INT8_SPLIT exchange needs N <= 16 and M <= 128");
     H->precision = precision;
   });
 }
@@ -389,10 +427,8 @@ int afq_energy(afq_ham H, int W, const double* G, double* E, void* stream) {
     check_ham(H);
     AFQ_CHECK(W <= H->ws.cap, AFQ_ECAPACITY, "more walkers than reserved");
     launch_e1(S(stream), W, H->M, H->N, G, H->haj, H->E0, scl(H), E);
-    launch_exchange(S(stream), H->precision, W, H->M, H->N, H->K, G, L_for(H), H->ws.Kl,
-                    H->ws.exx);
-    launch_energy_finalize(S(stream), W, H->K, exx_nchunks(W, H->N, H->K), H->walker_type, H->ws.Kl,
-                           H->ws.exx, E);
+    run_exchange(H, S(stream), W, G);
+    finalize_energy(H, S(stream), W, E);
   });
 }
 
@@ -488,13 +524,12 @@ int afq_step(afq_ham H, int W, double* wbuf, long ldw, const double* xi, double 
                      H->haj, H->E0, scl(H), ws.E, 3);
       }
       RegionScope r(tm, s, R_EXCHANGE);
-      launch_exchange(s, H->precision, W, M, N, K, ws.G, L_for(H), ws.Kl, ws.exx);
+      run_exchange(H, s, W, ws.G);
     }
     // 6.-7. energy assembly, weights, phases, energies, bounds
     {
       RegionScope r(tm, s, R_WALKER_UPDATE);
-      if (!p.hybrid)
-        launch_energy_finalize(s, W, K, exx_nchunks(W, N, K), H->walker_type, ws.Kl, ws.exx, ws.E);
+      if (!p.hybrid) finalize_energy(H, s, W, ws.E);
       launch_walker_update(s, W, wbuf, ldw, (long)M * N, p, H->dt, Eshift, H->E0, ws.ov, ws.E,
                            ws.mf, ws.HW, p.hybrid, H->stats);
     }
@@ -579,7 +614,8 @@ const char* afq_timer_name(int region) {
 
 int afq_probe_mfma(int kind, double* out, void* stream) {
   return guard([&] {
-    AFQ_CHECK(kind == 0 || kind == 1, AFQ_EINVAL, "probe kind must be 0 (FP64) or 1 (FP32)");
+    AFQ_CHECK(kind >= 0 && kind <= 5, AFQ_EINVAL,
+              "probe kind must be 0 (FP64), 1 (FP32), 2 or 3 (BF16), 4 or 5 (int8)");
     launch_probe_mfma(S(stream), kind, out);
   });
 }

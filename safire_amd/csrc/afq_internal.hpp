@@ -90,6 +90,13 @@ __device__ inline f4 mfma16x16x4(float a, float b, f4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
+// D(16x16, f32) += A(16x32, bf16) B(32x16, bf16): lane l supplies A[l%16][8(l/16)+q] and
+// B[8(l/16)+q][l%16], q = 0..7; the accumulator follows the FP32 layout (acc_row<float>).
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+__device__ inline f4 mfma_bf16(bf16x8 a, bf16x8 b, f4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
 template <typename R> __host__ __device__ inline int acc_row(int g, int r);
 template <> __host__ __device__ inline int acc_row<double>(int g, int r) { return 4 * r + g; }
 template <> __host__ __device__ inline int acc_row<float>(int g, int r) { return 4 * g + r; }
@@ -162,6 +169,9 @@ struct afq_ham_s {
   double* H1 = nullptr;      // c128 [M][M]
   float* Lnak_f = nullptr;   // c64 [K][N][M]   (FP32 contraction copies)
   float* Likn_f = nullptr;   // f32 [M*M][K]
+  __bf16* Lnak_b = nullptr;  // bf16 [3 slices][re,im][K][N][M]   (BF16X3 exchange)
+  int8_t* Lnak_q = nullptr;  // i8 [4 slices][re,im][K][N][Mp]     (INT8_SPLIT exchange)
+  double* Lnak_u = nullptr;  // f64 [K][N]   power-of-two unit of each Lnak row
   // [7]: eloc total/upper/lower, weight total/upper, population control calls/collapses
   unsigned long long* stats = nullptr;
   afq::Workspace ws;
@@ -183,7 +193,24 @@ void launch_gemm_nt_z(hipStream_t s, int precision, int m, int n, int k, double 
                       const double* A, long lda, const void* B, long ldb, bool b_complex,
                       double* C, long ldc, double* split_ws = nullptr, long split_ws_elems = 0);
 // k_energy.hip
+int device_cus();                        // compute units of the current device (cached)
 int exx_nchunks(int W, int N, int K);   // chunks of Cholesky vectors the exchange kernel uses
+// k_exchange_bf16.hip: the exchange on 3-slice BF16 MFMA (N <= 16, M <= 96, M % 8 == 0)
+bool exchange_bf16_supported(int M, int N);
+int exchange_bf16_chunk(int W, int K);
+int exchange_bf16_chunks(int W, int K);
+void split_lnak_bf16(hipStream_t s, int M, int N, int K, const double* Lnak, __bf16* Lb);
+void launch_exchange_bf16(hipStream_t s, int W, int M, int N, int K, const double* G,
+                          const __bf16* Lb, double* Kl, double* exx_part);
+// k_exchange_i8.hip: the exchange on int8 MFMA, Ozaki-split rows (N <= 16, M <= 128)
+bool exchange_i8_supported(int M, int N);
+int exchange_i8_row(int M);                        // Mp: padded row length of the slices
+size_t exchange_i8_bytes(int M, int N, int K);     // size of the Lnak slices
+int exchange_i8_chunks(int W, int K);
+void split_lnak_i8(hipStream_t s, int M, int N, int K, const double* Lnak, int8_t* Lq,
+                   double* Lu);
+void launch_exchange_i8(hipStream_t s, int W, int M, int N, int K, const double* G,
+                        const int8_t* Lq, const double* Lu, double* Kl, double* exx_part);
 void launch_exchange(hipStream_t s, int precision, int W, int M, int N, int K, const double* G,
                      const void* Lnak, double* Kl, double* exx_part);
 void launch_energy_finalize(hipStream_t s, int W, int K, int nchunk, int walker_type,

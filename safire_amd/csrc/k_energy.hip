@@ -250,18 +250,6 @@ __global__ __launch_bounds__(256) void k_e1(int NM, const double* __restrict__ G
   }
 }
 
-int device_cus() {
-  static int cus = 0;
-  if (cus == 0) {
-    int dev = 0;
-    hipDeviceProp_t prop;
-    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
-      cus = prop.multiProcessorCount;
-    if (cus <= 0) cus = 256;
-  }
-  return cus;
-}
-
 // Cholesky vectors per workgroup: a multiple of NV chosen so that the grid fills whole rounds
 // of one workgroup per CU (a partial last round leaves CUs idle while it runs); among equally
 // balanced choices the larger chunk (fewer workgroups, less per-workgroup setup) wins.
@@ -299,6 +287,18 @@ void launch_exchange_t(hipStream_t s, int W, int M, int N, int K, const double* 
 }
 
 }  // namespace
+
+int device_cus() {
+  static int cus = 0;
+  if (cus == 0) {
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+      cus = prop.multiProcessorCount;
+    if (cus <= 0) cus = 256;
+  }
+  return cus;
+}
 
 int exx_nchunks(int W, int N, int K) {
   const int chunk = choose_chunk(W, K, exx_nv(N), exx_wpb(N));

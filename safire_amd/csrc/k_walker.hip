@@ -438,11 +438,44 @@ void launch_population_control(hipStream_t s, int W, double* wbuf, long ldw, lon
   AFQ_HIP(hipGetLastError());
 }
 
+// BF16 16x16x32: A[i][k] = (i + 1) and B[k][j] = (j + 1) for k == ks only, so D[i][j] =
+// (i + 1)(j + 1). Lane l supplies A[l%16][8*(l/16) + q] and B[8*(l/16) + q][l%16], q = 0..7.
+__global__ void k_probe_mfma_bf16(int ks, double* out) {
+  const int lane = threadIdx.x;
+  const int i = lane & 15, g = lane >> 4;
+  bf16x8 a, b;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int k = 8 * g + q;
+    a[q] = (__bf16)(k == ks ? (float)(i + 1) : 0.0f);
+    b[q] = (__bf16)(k == ks ? (float)(i + 1) : 0.0f);   // column j = lane % 16 = i
+  }
+  f4 acc = {0, 0, 0, 0};
+  acc = mfma_bf16(a, b, acc);
+  for (int r = 0; r < 4; ++r) out[lane * 4 + r] = (double)acc[r];
+}
+
+// int8 16x16x64: lane l supplies bytes 16 (l / 16) + j, j < 16, of row / column l % 16
+__global__ void k_probe_mfma_i8(int ks, double* out) {
+  typedef int i4 __attribute__((ext_vector_type(4)));
+  const int lane = threadIdx.x;
+  const int i = lane & 15, g = lane >> 4;
+  i4 a = {0, 0, 0, 0};
+  if (ks / 16 == g) a[(ks % 16) / 4] = (i + 1) << (8 * (ks % 4));
+  i4 acc = {0, 0, 0, 0};
+  acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, a, acc, 0, 0, 0);
+  for (int r = 0; r < 4; ++r) out[lane * 4 + r] = (double)acc[r];
+}
+
 void launch_probe_mfma(hipStream_t s, int kind, double* out) {
   if (kind == 0)
     hipLaunchKernelGGL(k_probe_mfma<double>, dim3(1), dim3(64), 0, s, out);
-  else
+  else if (kind == 1)
     hipLaunchKernelGGL(k_probe_mfma<float>, dim3(1), dim3(64), 0, s, out);
+  else if (kind <= 3)
+    hipLaunchKernelGGL(k_probe_mfma_bf16, dim3(1), dim3(64), 0, s, kind == 2 ? 0 : 31, out);
+  else
+    hipLaunchKernelGGL(k_probe_mfma_i8, dim3(1), dim3(64), 0, s, kind == 4 ? 0 : 63, out);
   AFQ_HIP(hipGetLastError());
 }
 

@@ -51,6 +51,20 @@ def test_mfma_lane_layout(kind):
             assert d[lane, r] == i + 100 * (j % 4), (lane, r, d[lane])
 
 
+@pytest.mark.parametrize("kind", [2, 3, 4, 5])
+def test_mfma_bf16_lane_layout(kind):
+    """BF16 16x16x32 and int8 16x16x64: A and B pair their k slots lane group by lane group,
+    the accumulator follows the FP32 layout (rows 4g..4g+3)."""
+    out = torch.zeros(64 * 4, dtype=torch.float64, device="cuda")
+    A.check(A.lib().afq_probe_mfma(kind, A.ptr(out), A.stream_ptr()))
+    d = host(out).reshape(64, 4)
+    for lane in range(64):
+        g, j = lane // 16, lane % 16
+        for r in range(4):
+            i = 4 * g + r
+            assert d[lane, r] == (i + 1) * (j + 1), (lane, r, d[lane])
+
+
 def test_det_ops_kat_on_gpu():
     K = load_json("sdet_ops_kat.json")
     Am = cplx(K["A"])
