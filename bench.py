@@ -6,16 +6,20 @@ local-energy step (G -> vbias -> X -> vHS -> P1 exp(iV) P1 -> G, E1, EXX, EJ -> 
 QR re-orthogonalisation every 5 steps. Synthetic seeded inputs; everything resident in HBM
 before the timed region.
 
-Two forms of the same step are timed: the plain FP64 form and the mixed form (Cholesky
-contractions vbias/vHS/exchange in FP32 on MFMA with FP32 accumulation; G, overlaps, LU, QR,
-propagation, weights and energy sums in FP64). `value` is the mixed form's walker-steps/s; the
-line also carries the FP64 figure, the ratio, and the largest per-walker |E_loc(mixed) -
-E_loc(FP64)| on identical walker states. A step = one pass over all walkers.
+Every precision form of the same step is timed (--forms): the plain FP64 form and the mixed
+forms, which change only the Cholesky contractions (vbias and vHS in FP32; the exchange in FP32,
+three-slice BF16 or the int8 Ozaki split); G, overlaps, LU, QR, propagation, weights and energy
+sums stay FP64. Each form runs its timed steps twice from the same start: eagerly with region
+timers (HIP events on the launch stream), then as one captured HIP graph replayed once; the
+graph's final walker buffer must equal the eager run's bit for bit (graph_matches_eager).
+`value` is the graph throughput of the fastest form whose largest per-walker
+|E_loc(form) - E_loc(FP64)| on identical walker states is below 1e-5 Ha; the line carries every
+form's throughput and gate error. A step = one pass over all walkers.
 
 roofline: the exchange-energy kernel (the dominant one), algorithmic FLOP per launch
 W*(8 N^2 M K + 8 N^2 K) divided by its mean duration measured with HIP events on the launch
-stream over the timed region. cpu_baseline: the C++ FP64 oracle (OpenMP over walkers) timed on
-this host on a bounded sample of the same workload.
+stream over the eager timed run. cpu_baseline: the C++ FP64 oracle (OpenMP over walkers) timed
+on this host on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -65,7 +69,7 @@ def run_loop_step(prop, wset, xi_t, Eshift, t, ortho, uniforms):
         prop.population_control(wset, uniforms[t])
 
 
-def run_form(A, torch, s, W, xi_dev, uniforms, warmup, steps, ortho, precision, hybrid):
+def run_form(A, torch, s, W, xi_dev, uniforms, warmup, steps, ortho, precision, hybrid, graph):
     ham = A.HamiltonianOperations(s.h, s.Likn, s.PsiT, 0.0, s.dt)
     ham.set_params(hybrid=int(hybrid))
     ham.set_precision(precision)
@@ -91,7 +95,30 @@ def run_form(A, torch, s, W, xi_dev, uniforms, warmup, steps, ortho, precision, 
     ms = start.elapsed_time(stop)
     timers = ham.read_timers()
     ham.enable_timers(0)
-    return dict(ham=ham, wset=wset, ms=ms, wall_s=wall, timers=timers, Eshift=Eshift)
+    out = dict(ham=ham, wset=wset, ms=ms, eager_ms=ms, wall_s=wall, timers=timers, Eshift=Eshift,
+               graph=None)
+    if graph:
+        # the same steps from the same start, captured once (untimed) and replayed once (timed)
+        gset = A.WalkerSet(ham, W)
+        prop.seed(gset)
+        for t in range(warmup):
+            run_loop_step(prop, gset, xi_dev[t], Eshift, t, ortho, uniforms)
+        torch.cuda.synchronize()
+
+        def steps_fn():
+            for t in range(warmup, warmup + steps):
+                run_loop_step(prop, gset, xi_dev[t], Eshift, t, ortho, uniforms)
+
+        g = A.capture(steps_fn)
+        torch.cuda.synchronize()
+        start.record()
+        g.replay()
+        stop.record()
+        torch.cuda.synchronize()
+        out["ms"] = start.elapsed_time(stop)
+        out["graph"] = bool(torch.equal(gset.buf, wset.buf))   # bit-identical to the eager run
+        del g, gset
+    return out
 
 
 def eloc(A, torch, ham, wset):
@@ -143,6 +170,8 @@ def main(argv=None):
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--forms", default="fp64,fp32,bf16x3,int8")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="time the eager launches instead of one captured graph of the steps")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the walker step runs on a single MI355X here and "
@@ -170,10 +199,12 @@ def main(argv=None):
     for f in args.forms.split(","):
         prec = {"fp64": A.FP64, "fp32": A.FP32, "bf16x3": A.BF16X3, "int8": A.INT8_SPLIT}[f]
         forms[f] = run_form(A, torch, s, W, xi_dev, uniforms, args.warmup, args.steps, ortho,
-                            prec, hybrid=False)
+                            prec, hybrid=False, graph=not args.no_graph)
         r = forms[f]
-        print(f"[bench] {f}: {r['ms'] / args.steps:.3f} ms/step, "
-              f"{W * args.steps / (r['ms'] / 1e3):.0f} walker-steps/s; regions (ms total): "
+        print(f"[bench] {f}: {r['ms'] / args.steps:.3f} ms/step "
+              f"({'graph' if r['graph'] is not None else 'eager'}; eager "
+              f"{r['eager_ms'] / args.steps:.3f}; graph == eager: {r['graph']}), "
+              f"{W * args.steps / (r['ms'] / 1e3):.0f} walker-steps/s; regions (eager, ms total): "
               + ", ".join(f"{k}={v[0]:.2f}" for k, v in r["timers"].items() if v[1]),
               file=sys.stderr, flush=True)
 
@@ -264,6 +295,11 @@ def main(argv=None):
     out["step_tflops"] = round(W * step_flops(M, N, K) / (ms_step / 1e3) / 1e12, 3)
     out["regions_ms_per_step"] = {k: round(v[0] / args.steps, 4) for k, v in head["timers"].items()
                                   if v[1]}
+    out["timing"] = ("one HIP graph of the timed steps, replayed once" if head["graph"] is not None
+                     else "eager launches")
+    out["eager_ms_per_step"] = round(head["eager_ms"] / args.steps, 4)
+    if head["graph"] is not None:
+        out["graph_matches_eager"] = {f: r["graph"] for f, r in forms.items()}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(s, W, ortho, args.cpu_seconds)
     for f in forms.values():

@@ -284,6 +284,18 @@ class det_ops:
                                           stream_ptr(stream)))
 
 
+def capture(fn):
+    """Records the HIP work that fn() enqueues on the current stream into a graph and returns
+    it (torch.cuda.CUDAGraph; replay() runs it). The walker step, Orthogonalize and population
+    control allocate nothing and never synchronise with the host, so a run of steps can be
+    captured once its handle has run eagerly (the first calls set kernel attributes)."""
+    torch = _torch()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        fn()
+    return g
+
+
 class Propagator:
     """AFQMCBasePropagator over a WalkerSet: Propagate (one fused step) and Orthogonalize."""
 

@@ -6,9 +6,26 @@
 #include <cmath>
 #include <complex>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <utility>
 #include <vector>
 
 namespace afq {
+
+void ensure_dynamic_lds(const void* kern, size_t bytes) {
+  if (bytes <= 64 * 1024) return;
+  static std::mutex mu;
+  static std::map<std::pair<int, const void*>, size_t> done;   // (device, kernel) -> bytes
+  int dev = 0;
+  AFQ_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(mu);
+  size_t& cur = done[{dev, kern}];
+  if (cur >= bytes) return;
+  AFQ_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+  cur = bytes;
+}
+
 
 namespace {
 thread_local std::string g_err;

@@ -180,6 +180,11 @@ struct afq_ham_s {
 
 namespace afq {
 
+// Raises a kernel's dynamic-LDS limit to `bytes` (when above 64 KB) once per device. The
+// attribute call is not a stream operation; doing it once keeps it out of every later launch,
+// including launches recorded into a graph.
+void ensure_dynamic_lds(const void* kern, size_t bytes);
+
 // ------------------------------------------------------------------ launchers
 // k_green.hip
 void launch_green(hipStream_t s, int nb, int M, int N, const double* PsiT, const double* Phi,

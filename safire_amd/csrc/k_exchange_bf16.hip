@@ -1,14 +1,14 @@
 // Exchange and Coulomb energy (Real3IndexFactorization::energy_impl, Real3IndexFactorization.hpp:
 // 476-559) with the contraction T_wn = G_w Lnak_n^T on BF16 MFMA (v_mfma_f32_16x16x32_bf16).
 //
-// Accuracy: the energies of a walker are sums over K*N^2 products and reach 1e3-1e5 Ha, so a
-// local energy good to 1e-5 Ha needs the contraction to ~1e-9 relative. Plain FP32 misses that
-// because the rounding of G is the same for every Cholesky vector and adds up coherently. Here
-// both operands are split into three BF16 slices, x = x0 + x1 + x2 (hi, mid, lo: ~24 bits, the
+// Both operands are split into three BF16 slices, x = x0 + x1 + x2 (hi, mid, lo: ~24 bits, the
 // remainder below 2^-24 |x|), and the six slice products with i + j <= 2 are accumulated in
-// FP32; what is dropped (x1 y2, x2 y1, x2 y2 and the remainders) is ~2^-24 relative and not
-// correlated across Cholesky vectors. Complex products take 4 real products (the real part uses
-// -Im(G), negated exactly in BF16).
+// FP32. Complex products take 4 real products (the real part uses -Im(G), negated exactly in
+// BF16). Accuracy: the slices represent each operand to ~2^-24, like FP32 rounding, and the FP32
+// accumulator rounds each T entry to ~2^-24 again; the rounding of G is shared by all Cholesky
+// vectors and adds up coherently in EXX. On walkers whose |EXX| reaches 1e5 Ha the energy error
+// is ~1e-4 Ha, within 3x of the FP32 form: this form does not meet a 1e-5 Ha gate there (the
+// int8 Ozaki split of k_exchange_i8.hip does).
 //
 // Structure follows the FP64 kernel: a workgroup holds WPB walkers (one per wavefront) and a
 // chunk of Cholesky vectors; each wavefront splits its walker's G once into MFMA A-fragments kept
@@ -260,8 +260,7 @@ void launch_exchange_bf16(hipStream_t s, int W, int M, int N, int K, const doubl
   dim3 grid((W + kWPB - 1) / kWPB, nchunk);
   auto go = [&](auto kern) {
     if (lds + sizeof(double) * kWPB * 512 > 64 * 1024)   // dynamic + the static T blocks
-      AFQ_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)lds));
+      ensure_dynamic_lds((const void*)kern, std::max(lds, (size_t)64 * 1024 + 1));
     hipLaunchKernelGGL(kern, grid, dim3(64 * kWPB), lds, s, W, M, N, K, chunk, G, Lb, Kl,
                        exx_part, nchunk);
   };

@@ -33,14 +33,19 @@ typedef int i4 __attribute__((ext_vector_type(4)));
 constexpr int kSG = 5;      // slices of G
 constexpr int kSL = 5;      // slices of Lnak
 constexpr int kOrders = 5;  // accumulator orders 0..4
-constexpr int kWPB = 8;     // walkers (wavefronts) per workgroup
+// walkers (wavefronts) per workgroup: two 4-wavefront workgroups per CU run out of step, so
+// one's FP64 epilogue overlaps the other's MFMAs (one 8-wavefront workgroup keeps all of a
+// CU's wavefronts in step at its group barrier)
+constexpr int kWPB = 4;
+constexpr int kWgPerCu = 2;   // resident workgroups per CU (LDS and registers allow two)
 constexpr int kMaxKC = 2;   // 64-wide k chunks: M <= 128
 constexpr double kBase = 254.0;   // ratio of consecutive slice units
 
 // kept slice products: all of order s + t <= 4
 __host__ __device__ constexpr bool kept(int s, int t) { return s + t < kOrders; }
-// vectors staged per group: LDS holds kStage x 10 slice rows of 64 KC (+16) bytes x 16
-__host__ __device__ constexpr int stage_of(int KC) { return KC == 1 ? 4 : 2; }
+// vectors staged per group: LDS holds 2 buffers x kStage x 10 slice rows of 64 KC (+16) bytes
+// x 16 rows, plus the per-wavefront T blocks
+__host__ __device__ constexpr int stage_of(int KC) { return KC == 1 ? 2 : 1; }
 
 // power-of-two unit of a row whose largest |component| is mx: |x / u| < 127 for every x of the
 // row. Rows below 2^-960 count as zero (their products are far below any energy resolution);
@@ -85,14 +90,17 @@ __device__ inline void wave_lds_sync() {
 }
 
 template <int KC, int NV>
-__global__ __launch_bounds__(64 * kWPB) void k_exchange_i8(
+__global__ __launch_bounds__(64 * kWPB, kWgPerCu * kWPB / 4) void k_exchange_i8(
     int W, int M, int N, int K, int Mp, int kChunk, const double* __restrict__ G,
     const int8_t* __restrict__ Lq, const double* __restrict__ Lu, double* __restrict__ Kl,
     double* __restrict__ exx_part, int nchunk) {
   constexpr int ROWB = 64 * KC + 16;   // staged row stride (bytes): the row plus 16 of padding
   constexpr int kStage = stage_of(KC);
-  __shared__ __attribute__((aligned(16))) int8_t sB[kStage * kSL * 2 * 16 * ROWB];
-  __shared__ double sU[kStage][16];                 // column units of the staged vectors
+  constexpr int kGroupVec = kStage * kSL * 2 * 16 * (4 * KC);   // 16-byte vectors per group
+  constexpr int kPre = (kGroupVec + 64 * kWPB - 1) / (64 * kWPB);  // ... per thread
+  // two staging buffers: the next group is loaded while the current one is computed on
+  __shared__ __attribute__((aligned(16))) int8_t sBuf[2][kStage * kSL * 2 * 16 * ROWB];
+  __shared__ double sUnit[2][kStage][16];           // column units of the staged vectors
   __shared__ double sT[kWPB][16 * 16 * 2];          // per-wavefront T block [a][b] (re,im)
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
@@ -116,9 +124,9 @@ __global__ __launch_bounds__(64 * kWPB) void k_exchange_i8(
         for (int j = 0; j < 16; ++j) {
           const int k = 64 * kc + 16 * g + j;
           if (k < M) {
-            const double m = fmax(fabs(Gw[2 * k]), fabs(Gw[2 * k + 1]));
+            const double xr = fabs(Gw[2 * k]), xi = fabs(Gw[2 * k + 1]);
             // a NaN or Inf anywhere in the row must survive the max (fmax drops NaN)
-            mx = (m <= 1.0e300 && mx <= 1.0e300) ? fmax(mx, m) : INFINITY;
+            mx = (xr <= 1.0e300 && xi <= 1.0e300) ? fmax(mx, fmax(xr, xi)) : INFINITY;
           }
         }
     }
@@ -162,24 +170,52 @@ __global__ __launch_bounds__(64 * kWPB) void k_exchange_i8(
   double* T = sT[wave];
   const int nvec = 4 * KC;          // 16-byte vectors per staged row
 
-  for (int nv0 = nbeg; nv0 < nend; nv0 += kStage) {
-    __syncthreads();
-    for (int e = threadIdx.x; e < kStage * kSL * 2 * 16 * nvec; e += 64 * kWPB) {
+  // group loads: slice rows of vectors nv0 .. nv0 + kStage - 1 into registers (zero outside)
+  uint4 pre[kPre];
+  double preU = 0.0;
+  auto load_group = [&](int nv0) {
+#pragma unroll
+    for (int i = 0; i < kPre; ++i) {
+      const int e = threadIdx.x + i * 64 * kWPB;
       const int kv = e % nvec;
       const int b = (e / nvec) % 16;
       const int tp = (e / (nvec * 16)) % (kSL * 2);
       const int v = e / (nvec * 16 * kSL * 2);
       const int n = nv0 + v;
-      uint4 val = {0u, 0u, 0u, 0u};
-      if (n < nend && b < N && 16 * kv < Mp)
-        val = *reinterpret_cast<const uint4*>(Lq + tp * KNMp + ((long)n * N + b) * Mp + 16 * kv);
-      *reinterpret_cast<uint4*>(sB + ((v * kSL * 2 + tp) * 16 + b) * ROWB + 16 * kv) = val;
+      pre[i] = uint4{0u, 0u, 0u, 0u};
+      if (e < kGroupVec && n < nend && b < N && 16 * kv < Mp)
+        pre[i] = *reinterpret_cast<const uint4*>(Lq + tp * KNMp + ((long)n * N + b) * Mp + 16 * kv);
     }
     if (threadIdx.x < kStage * 16) {
       const int v = threadIdx.x / 16, b = threadIdx.x % 16, n = nv0 + v;
-      sU[v][b] = (n < nend && b < N) ? Lu[(long)n * N + b] : 0.0;
+      preU = (n < nend && b < N) ? Lu[(long)n * N + b] : 0.0;
     }
-    __syncthreads();
+  };
+  auto store_group = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < kPre; ++i) {
+      const int e = threadIdx.x + i * 64 * kWPB;
+      const int kv = e % nvec;
+      const int b = (e / nvec) % 16;
+      const int tp = (e / (nvec * 16)) % (kSL * 2);
+      const int v = e / (nvec * 16 * kSL * 2);
+      if (e < kGroupVec)
+        *reinterpret_cast<uint4*>(sBuf[buf] + ((v * kSL * 2 + tp) * 16 + b) * ROWB + 16 * kv) =
+            pre[i];
+    }
+    if (threadIdx.x < kStage * 16) sUnit[buf][threadIdx.x / 16][threadIdx.x % 16] = preU;
+  };
+
+  if (nbeg < nend) {
+    load_group(nbeg);
+    store_group(0);
+  }
+  __syncthreads();
+  for (int nv0 = nbeg, buf = 0; nv0 < nend; nv0 += kStage, buf ^= 1) {
+    const bool more = nv0 + kStage < nend;
+    if (more) load_group(nv0 + kStage);   // in flight during this group's MFMAs
+    const int8_t* sB = sBuf[buf];
+    const double (*sU)[16] = sUnit[buf];
 
 #pragma unroll 1
     for (int h = 0; h < kStage; h += NV) {
@@ -222,15 +258,31 @@ __global__ __launch_bounds__(64 * kWPB) void k_exchange_i8(
         double tr[4], ti[4];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          double xr = (double)accR[v][kOrders - 1][r], xi = (double)accI[v][kOrders - 1][r];
+          if (KC == 1) {
+            // M <= 64: |acc_o| <= (o + 1) 2 64 127^2, so 254 acc0 + acc1 and 254 acc2 + acc3
+            // stay below 2^31 and pair up exactly in int32 (three conversions, not five)
+            const int pr01 = 254 * accR[v][0][r] + accR[v][1][r];
+            const int pi01 = 254 * accI[v][0][r] + accI[v][1][r];
+            const int pr23 = 254 * accR[v][2][r] + accR[v][3][r];
+            const int pi23 = 254 * accI[v][2][r] + accI[v][3][r];
+            double xr = fma((double)accR[v][4][r], 1.0 / kBase, (double)pr23);
+            double xi = fma((double)accI[v][4][r], 1.0 / kBase, (double)pi23);
+            xr = fma(xr, 1.0 / (kBase * kBase), (double)pr01);
+            xi = fma(xi, 1.0 / (kBase * kBase), (double)pi01);
+            const double u = uRow[r] * uL * (1.0 / kBase);   // pr01 carries one factor 254
+            tr[r] = xr * u;
+            ti[r] = xi * u;
+          } else {
+            double xr = (double)accR[v][kOrders - 1][r], xi = (double)accI[v][kOrders - 1][r];
 #pragma unroll
-          for (int o = kOrders - 2; o >= 0; --o) {
-            xr = fma(xr, 1.0 / kBase, (double)accR[v][o][r]);
-            xi = fma(xi, 1.0 / kBase, (double)accI[v][o][r]);
+            for (int o = kOrders - 2; o >= 0; --o) {
+              xr = fma(xr, 1.0 / kBase, (double)accR[v][o][r]);
+              xi = fma(xi, 1.0 / kBase, (double)accI[v][o][r]);
+            }
+            const double u = uRow[r] * uL;
+            tr[r] = xr * u;
+            ti[r] = xi * u;
           }
-          const double u = uRow[r] * uL;
-          tr[r] = xr * u;
-          ti[r] = xi * u;
         }
         wave_lds_sync();
 #pragma unroll
@@ -248,14 +300,15 @@ __global__ __launch_bounds__(64 * kWPB) void k_exchange_i8(
             exx_re += tr[r] * yr - ti[r] * yi;
             exx_im += tr[r] * yi + ti[r] * yr;
           }
+          // tr(T): the N diagonal entries, summed over lanes 0..15 in four rounds
           double dr = 0.0, di = 0.0;
           if (lane < N) {
             dr = T[2 * (lane * 16 + lane)];
             di = T[2 * (lane * 16 + lane) + 1];
           }
-          for (int off = 32; off > 0; off >>= 1) {
-            dr += __shfl_xor(dr, off, 64);
-            di += __shfl_xor(di, off, 64);
+          for (int off = 8; off > 0; off >>= 1) {
+            dr += __shfl_xor(dr, off, 16);
+            di += __shfl_xor(di, off, 16);
           }
           if (lane == 0 && active) {
             Kl[2 * ((long)w * K + n)] = dr;
@@ -264,6 +317,9 @@ __global__ __launch_bounds__(64 * kWPB) void k_exchange_i8(
         }
       }
     }
+    // the other buffer was last read in the previous group, before the barrier that ended it
+    if (more) store_group(buf ^ 1);
+    __syncthreads();
   }
 
   for (int off = 32; off > 0; off >>= 1) {
@@ -317,7 +373,7 @@ size_t exchange_i8_bytes(int M, int N, int K) {
 // whole rounds of one workgroup per CU, in multiples of the largest staging group
 int exchange_i8_chunk(int W, int K) {
   constexpr int kStage = stage_of(1);
-  const int cus = device_cus();
+  const int cus = device_cus() * kWgPerCu;
   const int wgroups = (W + kWPB - 1) / kWPB;
   int best = kStage;
   double best_eff = -1.0;

@@ -372,9 +372,7 @@ void launch_green(hipStream_t s, int nb, int M, int N, const double* PsiT, const
   const size_t lds = sizeof(cd) * (size_t)M * N * ((E != nullptr && haj != nullptr) ? 3 : 2);
   const bool stage = lds <= 96 * 1024;
   auto go = [&](auto kern) {
-    if (stage && lds > 64 * 1024)
-      AFQ_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)lds));
+    if (stage) ensure_dynamic_lds((const void*)kern, lds);
     hipLaunchKernelGGL(kern, dim3(nb), dim3(64), stage ? lds : 0, s, M, N,
                        reinterpret_cast<const cd*>(PsiT), reinterpret_cast<const cd*>(Phi), ldw,
                        reinterpret_cast<cd*>(G), reinterpret_cast<cd*>(ov), ov_mode, ov_scale,

@@ -5,9 +5,11 @@
 // backward accumulation (zungqr), and Q's columns scaled by sign(Re R_ii): the result is the
 // unique Q whose R has a positive real diagonal.
 //
-// One workgroup per walker; A (the reflectors) and Q live in LDS. Each elimination step reduces
-// a column norm in one wavefront and applies the reflector with one wavefront per trailing
-// column (lanes over rows, shuffle reductions).
+// One workgroup per walker; A (the reflectors) and Q live in LDS with rows padded to N + 1
+// complex numbers, so that the lanes of a column access (consecutive rows) fall into different
+// banks instead of all into one. Each elimination step reduces a column norm in one wavefront and
+// applies the reflector with one wavefront per trailing column (lanes over rows, shuffle
+// reductions).
 #include "afq_internal.hpp"
 
 #include <cfloat>
@@ -30,23 +32,24 @@ __device__ inline cd wave_sum(cd v) {
 __global__ __launch_bounds__(kThreads) void k_qr(int M, int N, cd* __restrict__ Phi, long ldw,
                                                  cd* __restrict__ logdetR) {
   extern __shared__ cd smem[];
-  cd* A = smem;            // [M][N]
-  cd* Q = A + M * N;       // [M][N]
+  const int ld = N + 1;    // padded row stride
+  cd* A = smem;            // [M][ld]
+  cd* Q = A + M * ld;      // [M][ld]
   __shared__ cd tau[64];
   __shared__ double scl[64];
   const int w = blockIdx.x;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   cd* P = Phi + (long)w * ldw;
-  for (int e = tid; e < M * N; e += kThreads) A[e] = P[e];
+  for (int e = tid; e < M * N; e += kThreads) A[(e / N) * ld + e % N] = P[e];
   __syncthreads();
   const int kmax = min(M, N);
 
   for (int j = 0; j < kmax; ++j) {
     if (wave == 0) {
       double x2 = 0.0;
-      for (int i = j + 1 + lane; i < M; i += 64) x2 += cnorm(A[i * N + j]);
+      for (int i = j + 1 + lane; i < M; i += 64) x2 += cnorm(A[i * ld + j]);
       for (int off = 32; off > 0; off >>= 1) x2 += __shfl_xor(x2, off, 64);
-      const cd alpha = A[j * N + j];
+      const cd alpha = A[j * ld + j];
       cd t = cmk(0.0, 0.0), sc = cmk(0.0, 0.0);
       double beta = 0.0;
       const bool identity = (x2 == 0.0 && alpha.im == 0.0);
@@ -57,10 +60,10 @@ __global__ __launch_bounds__(kThreads) void k_qr(int M, int N, cd* __restrict__ 
         sc = cdiv(cmk(1.0, 0.0), cmk(alpha.re - beta, alpha.im));
       }
       for (int i = j + 1 + lane; i < M; i += 64)
-        if (!identity) A[i * N + j] = A[i * N + j] * sc;
+        if (!identity) A[i * ld + j] = A[i * ld + j] * sc;
       if (lane == 0) {
         tau[j] = t;
-        if (!identity) A[j * N + j] = cmk(beta, 0.0);
+        if (!identity) A[j * ld + j] = cmk(beta, 0.0);
       }
     }
     __syncthreads();
@@ -68,45 +71,45 @@ __global__ __launch_bounds__(kThreads) void k_qr(int M, int N, cd* __restrict__ 
     const cd tj = cconj(tau[j]);
     for (int c = j + 1 + wave; c < N; c += kWaves) {
       cd s = cmk(0.0, 0.0);
-      for (int i = j + 1 + lane; i < M; i += 64) s += cconj(A[i * N + j]) * A[i * N + c];
+      for (int i = j + 1 + lane; i < M; i += 64) s += cconj(A[i * ld + j]) * A[i * ld + c];
       s = wave_sum(s);
-      s = (A[j * N + c] + s);
+      s = (A[j * ld + c] + s);
       s = tj * s;
-      if (lane == 0) A[j * N + c] -= s;
-      for (int i = j + 1 + lane; i < M; i += 64) A[i * N + c] -= s * A[i * N + j];
+      if (lane == 0) A[j * ld + c] -= s;
+      for (int i = j + 1 + lane; i < M; i += 64) A[i * ld + c] -= s * A[i * ld + j];
     }
     __syncthreads();
   }
 
-  if (tid < kmax) scl[tid] = (A[tid * N + tid].re < 0.0) ? -1.0 : 1.0;
+  if (tid < kmax) scl[tid] = (A[tid * ld + tid].re < 0.0) ? -1.0 : 1.0;
   if (logdetR != nullptr && tid == 0) {
     cd acc = logdetR[w];
     for (int i = 0; i < kmax; ++i) {
-      const cd r = (A[i * N + i].re < 0.0) ? -A[i * N + i] : A[i * N + i];
+      const cd r = (A[i * ld + i].re < 0.0) ? -A[i * ld + i] : A[i * ld + i];
       acc += clog_(cmk(r.re + DBL_MIN, r.im));
     }
     logdetR[w] = acc;
   }
   for (int e = tid; e < M * N; e += kThreads) {
     const int i = e / N, c = e - i * N;
-    Q[e] = cmk(i == c ? 1.0 : 0.0, 0.0);
+    Q[i * ld + c] = cmk(i == c ? 1.0 : 0.0, 0.0);
   }
   __syncthreads();
   for (int j = kmax - 1; j >= 0; --j) {
     const cd t = tau[j];
     for (int c = j + wave; c < N; c += kWaves) {
       cd s = cmk(0.0, 0.0);
-      for (int i = j + 1 + lane; i < M; i += 64) s += cconj(A[i * N + j]) * Q[i * N + c];
+      for (int i = j + 1 + lane; i < M; i += 64) s += cconj(A[i * ld + j]) * Q[i * ld + c];
       s = wave_sum(s);
-      s = t * (Q[j * N + c] + s);
-      if (lane == 0) Q[j * N + c] -= s;
-      for (int i = j + 1 + lane; i < M; i += 64) Q[i * N + c] -= s * A[i * N + j];
+      s = t * (Q[j * ld + c] + s);
+      if (lane == 0) Q[j * ld + c] -= s;
+      for (int i = j + 1 + lane; i < M; i += 64) Q[i * ld + c] -= s * A[i * ld + j];
     }
     __syncthreads();
   }
   for (int e = tid; e < M * N; e += kThreads) {
-    const int c = e % N;
-    P[e] = (c < kmax) ? scl[c] * Q[e] : Q[e];
+    const int i = e / N, c = e % N;
+    P[e] = (c < kmax) ? scl[c] * Q[i * ld + c] : Q[i * ld + c];
   }
 }
 
@@ -116,11 +119,9 @@ void launch_orthogonalize(hipStream_t s, int nb, int M, int N, double* Phi, long
                           double* logdetR) {
   if (nb <= 0 || (long)M * N == 0) return;
   AFQ_CHECK(N <= 64, AFQ_EUNSUPPORTED, "orthogonalize: N > 64");
-  const size_t lds = 2 * sizeof(cd) * (size_t)M * N;
+  const size_t lds = 2 * sizeof(cd) * (size_t)M * (N + 1);
   AFQ_CHECK(lds <= 160 * 1024 - 2048, AFQ_EUNSUPPORTED, "orthogonalize: walker too large");
-  if (lds > 64 * 1024)
-    AFQ_HIP(hipFuncSetAttribute((const void*)k_qr, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)lds));
+  ensure_dynamic_lds((const void*)k_qr, lds);
   hipLaunchKernelGGL(k_qr, dim3(nb), dim3(kThreads), lds, s, M, N, reinterpret_cast<cd*>(Phi),
                      ldw, reinterpret_cast<cd*>(logdetR));
   AFQ_HIP(hipGetLastError());

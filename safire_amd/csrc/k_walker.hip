@@ -422,9 +422,7 @@ void launch_population_control(hipStream_t s, int W, double* wbuf, long ldw, lon
   if (W <= 0) return;
   const size_t lds = (size_t)W * (sizeof(double) + 3 * sizeof(int));
   if (lds <= 120 * 1024) {
-    if (lds > 64 * 1024)
-      AFQ_HIP(hipFuncSetAttribute((const void*)k_comb_parallel,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    ensure_dynamic_lds((const void*)k_comb_parallel, lds);
     hipLaunchKernelGGL(k_comb_parallel, dim3(1), dim3(256), lds, s, W,
                        reinterpret_cast<const cd*>(wbuf), ldw, poff, r, src, ok, stats);
   } else {
