@@ -5,11 +5,14 @@
 //
 // Tiling for CDNA4: a 256-thread workgroup owns a 64x64 tile of C, each 64-wide wavefront a
 // 32x32 sub-tile made of 2x2 MFMA 16x16x4 tiles with separate real/imag accumulators
-// (4 real MFMAs per complex tile and k-step, 2 when B is real). Along k, lane group g = lane/16
-// takes k = k0 + 4g + s in MFMA step s, so each lane reads 4 consecutive complex numbers
-// (64 bytes) per 16-wide k chunk; A and B use the same permutation, which leaves the sum
-// unchanged. Both FP64 (16x16x4f64) and FP32 (16x16x4f32) MFMA are instantiated; the FP32 form
-// converts A on load and reads B from the FP32 copy of the Cholesky tensor.
+// (4 real MFMAs per complex tile and k-step, 2 when B is real). The workgroup stages each
+// 64 x KS slice of A and of B in LDS once (every thread loads 1/256 of it; rows padded to KS + 1
+// elements so that the 16 rows of an MFMA fragment fall into different banks), double-buffered:
+// the next slice's loads are in flight in registers while the current slice is multiplied, with
+// one barrier per slice. Along k, lane group g = lane/16 takes k = k0 + 4g + s in MFMA step s;
+// A and B use the same permutation, which leaves the sum unchanged. Both FP64 (16x16x4f64) and
+// FP32 (16x16x4f32) MFMA are instantiated; the FP32 form converts A when it is staged and reads
+// B from the FP32 copy of the Cholesky tensor.
 //
 // vbias has few output tiles (W x K = 512 x 320 is 40 tiles) and a long k (N*M), so k is split
 // over grid.z: each split writes its partial tile to a workspace and a second kernel sums the
@@ -22,21 +25,53 @@ namespace {
 
 constexpr int kTile = 64;
 
-template <typename R, bool BC>
+template <typename R, bool BC, int KS>
 __global__ __launch_bounds__(256) void k_gemm_nt(int m, int n, int k, int kchunk, double alpha,
                                                  const double* __restrict__ A, long lda,
                                                  const R* __restrict__ B, long ldb,
                                                  double* __restrict__ C, long ldc,
                                                  long split_stride) {
   typedef typename acc_of<R>::type acc_t;
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
+  constexpr int LDK = KS + 1;          // padded k stride of a staged row (elements)
+  constexpr int NB = BC ? 2 : 1;       // R values per element of B
+  constexpr int KPT = KS / 4;          // k values each thread stages per row and slice
+  __shared__ R sA[2][kTile * LDK * 2];
+  __shared__ R sB[2][kTile * LDK * NB];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
   const int g = lane >> 4, r16 = lane & 15;
-  const int m0 = blockIdx.x * kTile + (wave >> 1) * 32;
-  const int n0 = blockIdx.y * kTile + (wave & 1) * 32;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
   const int kbeg = blockIdx.z * kchunk;
   const int kend = min(k, kbeg + kchunk);
   C += blockIdx.z * split_stride;
+
+  // staging share of this thread: row srow of the A and B tiles, k = k0 + skq .. + KPT - 1
+  const int srow = tid >> 2, skq = (tid & 3) * KPT;
+  const int gm = blockIdx.x * kTile + srow, gn = blockIdx.y * kTile + srow;
+  R pa[2 * KPT], pb[NB * KPT];
+  auto gload = [&](int k0) {
+    const int kb = k0 + skq;
+#pragma unroll
+    for (int j = 0; j < KPT; ++j) {
+      const bool ok = kb + j < kend;
+      const double* ap = A + 2 * ((long)gm * lda + kb + j);
+      pa[2 * j] = (gm < m && ok) ? (R)ap[0] : (R)0;
+      pa[2 * j + 1] = (gm < m && ok) ? (R)ap[1] : (R)0;
+      const R* bp = B + NB * ((long)gn * ldb + kb + j);
+      pb[NB * j] = (gn < n && ok) ? bp[0] : (R)0;
+      if (BC) pb[NB * j + 1] = (gn < n && ok) ? bp[1] : (R)0;
+    }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int j = 0; j < KPT; ++j) {
+      sA[buf][2 * (srow * LDK + skq + j)] = pa[2 * j];
+      sA[buf][2 * (srow * LDK + skq + j) + 1] = pa[2 * j + 1];
+      sB[buf][NB * (srow * LDK + skq + j)] = pb[NB * j];
+      if (BC) sB[buf][NB * (srow * LDK + skq + j) + 1] = pb[NB * j + 1];
+    }
+  };
 
   acc_t cr[2][2], ci[2][2];
 #pragma unroll
@@ -47,67 +82,58 @@ __global__ __launch_bounds__(256) void k_gemm_nt(int m, int n, int k, int kchunk
       ci[a][b] = acc_t{0, 0, 0, 0};
     }
 
-  for (int k0 = kbeg; k0 < kend; k0 += 16) {
-    const int kb = k0 + 4 * g;
-    const bool kfull = kb + 4 <= kend;
-    R ar[2][4], ai[2][4], br[2][4], bi[2][4];
+  if (kbeg < kend) {
+    gload(kbeg);
+    sstore(0);
+  }
+  __syncthreads();
+  for (int k0 = kbeg, buf = 0; k0 < kend; k0 += KS, buf ^= 1) {
+    const bool more = k0 + KS < kend;
+    if (more) gload(k0 + KS);
+    const R* a_s = sA[buf];
+    const R* b_s = sB[buf];
 #pragma unroll
-    for (int t = 0; t < 2; ++t) {
-      const int row = m0 + t * 16 + r16;
-      if (row < m && kfull) {
-        const double2* pa = reinterpret_cast<const double2*>(A + 2 * ((long)row * lda + kb));
+    for (int kk = 0; kk < KS; kk += 16) {
+      R ar[2][4], ai[2][4], br[2][4], bi[2][4];
 #pragma unroll
-        for (int s = 0; s < 4; ++s) {
-          const double2 v = pa[s];
-          ar[t][s] = (R)v.x;
-          ai[t][s] = (R)v.y;
-        }
-      } else {
-        const double* pa = A + 2 * ((long)row * lda + kb);
+      for (int t = 0; t < 2; ++t) {
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
-          const bool ok = row < m && kb + s < kend;
-          ar[t][s] = ok ? (R)pa[2 * s] : (R)0;
-          ai[t][s] = ok ? (R)pa[2 * s + 1] : (R)0;
+          const int kx = kk + 4 * g + s;
+          const int ra = wm + t * 16 + r16, rb = wn + t * 16 + r16;
+          ar[t][s] = a_s[2 * (ra * LDK + kx)];
+          ai[t][s] = a_s[2 * (ra * LDK + kx) + 1];
+          br[t][s] = b_s[NB * (rb * LDK + kx)];
+          bi[t][s] = BC ? b_s[NB * (rb * LDK + kx) + 1] : (R)0;
         }
       }
-      const int col = n0 + t * 16 + r16;
+      // independent accumulators interleaved: 8 (or 4) chains per k-step
 #pragma unroll
       for (int s = 0; s < 4; ++s) {
-        const bool ok = col < n && kb + s < kend;
-        if (BC) {
-          const R* pb = B + 2 * ((long)col * ldb + kb);
-          br[t][s] = ok ? pb[2 * s] : (R)0;
-          bi[t][s] = ok ? pb[2 * s + 1] : (R)0;
-        } else {
-          const R* pb = B + ((long)col * ldb + kb);
-          br[t][s] = ok ? pb[s] : (R)0;
-          bi[t][s] = (R)0;
-        }
-      }
-    }
-    // independent accumulators interleaved: 8 (or 4) chains per k-step
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-#pragma unroll
-      for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b) {
-          cr[a][b] = mfma16x16x4(ar[a][s], br[b][s], cr[a][b]);
-          ci[a][b] = mfma16x16x4(ai[a][s], br[b][s], ci[a][b]);
-        }
-      if (BC) {
 #pragma unroll
         for (int a = 0; a < 2; ++a)
 #pragma unroll
           for (int b = 0; b < 2; ++b) {
-            cr[a][b] = mfma16x16x4(-ai[a][s], bi[b][s], cr[a][b]);
-            ci[a][b] = mfma16x16x4(ar[a][s], bi[b][s], ci[a][b]);
+            cr[a][b] = mfma16x16x4(ar[a][s], br[b][s], cr[a][b]);
+            ci[a][b] = mfma16x16x4(ai[a][s], br[b][s], ci[a][b]);
           }
+        if (BC) {
+#pragma unroll
+          for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+              cr[a][b] = mfma16x16x4(-ai[a][s], bi[b][s], cr[a][b]);
+              ci[a][b] = mfma16x16x4(ar[a][s], bi[b][s], ci[a][b]);
+            }
+        }
       }
     }
+    // the other buffer was last read in the previous slice, before the barrier that ended it
+    if (more) sstore(buf ^ 1);
+    __syncthreads();
   }
 
+  const int m0 = blockIdx.x * kTile + wm, n0 = blockIdx.y * kTile + wn;
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -158,7 +184,7 @@ void launch_gemm_nt_z(hipStream_t s, int precision, int m, int n, int k, double 
   int splits = split_ws ? gemm_splits(m, n, k) : 1;
   if ((long)splits * m * n > split_ws_elems) splits = 1;
   int kchunk = (k + splits - 1) / splits;
-  kchunk = (kchunk + 15) & ~15;   // k chunks on the 16-wide MFMA slices
+  kchunk = (kchunk + 31) & ~31;   // k chunks on whole staged slices (16 or 32 wide)
   splits = (k + kchunk - 1) / kchunk;
   if (splits < 1) splits = 1;
   dim3 grid((m + kTile - 1) / kTile, (n + kTile - 1) / kTile, splits);
@@ -168,17 +194,17 @@ void launch_gemm_nt_z(hipStream_t s, int precision, int m, int n, int k, double 
   const double a = splits > 1 ? 1.0 : alpha;
   if (precision == AFQ_FP64) {
     if (b_complex)
-      hipLaunchKernelGGL((k_gemm_nt<double, true>), grid, dim3(256), 0, s, m, n, k, kchunk, a, A,
+      hipLaunchKernelGGL((k_gemm_nt<double, true, 16>), grid, dim3(256), 0, s, m, n, k, kchunk, a, A,
                          lda, (const double*)B, ldb, out, ldo, sstride);
     else
-      hipLaunchKernelGGL((k_gemm_nt<double, false>), grid, dim3(256), 0, s, m, n, k, kchunk, a,
+      hipLaunchKernelGGL((k_gemm_nt<double, false, 16>), grid, dim3(256), 0, s, m, n, k, kchunk, a,
                          A, lda, (const double*)B, ldb, out, ldo, sstride);
   } else {
     if (b_complex)
-      hipLaunchKernelGGL((k_gemm_nt<float, true>), grid, dim3(256), 0, s, m, n, k, kchunk, a, A,
+      hipLaunchKernelGGL((k_gemm_nt<float, true, 32>), grid, dim3(256), 0, s, m, n, k, kchunk, a, A,
                          lda, (const float*)B, ldb, out, ldo, sstride);
     else
-      hipLaunchKernelGGL((k_gemm_nt<float, false>), grid, dim3(256), 0, s, m, n, k, kchunk, a,
+      hipLaunchKernelGGL((k_gemm_nt<float, false, 32>), grid, dim3(256), 0, s, m, n, k, kchunk, a,
                          A, lda, (const float*)B, ldb, out, ldo, sstride);
   }
   AFQ_HIP(hipGetLastError());
