@@ -56,7 +56,7 @@ enum afq_precision {
                           (x = u (q0 + q1/254 + ... + q4/254^4)); the 15 slice products of
                           order <= 4 accumulate exactly in int32 and are combined in FP64
                           (energy error ~1e-3 of AFQ_FP32's). vbias and vHS as in AFQ_FP32.
-                          Needs N <= 16 and M <= 128 (AFQ_EUNSUPPORTED otherwise)           */
+                          Needs N <= 32 and M <= 128 (AFQ_EUNSUPPORTED otherwise)           */
 };
 
 /* Walker property slots (WalkerConfig.hpp:28-48, offsets WalkerSetBase.cpp:87-117). */
@@ -206,6 +206,26 @@ int afq_weighted_sums(afq_ham ham, int W, const double* d_wbuf, long ldw, double
 int afq_ham_enable_timers(afq_ham ham, int max_records);
 int afq_ham_read_timers(afq_ham ham, double* h_ms, long long* h_counts);
 const char* afq_timer_name(int region);
+
+/* ---------------------------------------------------------------------------------------------
+ * Particle-hole multi-determinant (PHMSD) trial: energy corrections of the excited determinants
+ * (detail/phmsd_impl.hpp). iexcit [ndet][2][nex] int32: the electrons (rows of refc) that are
+ * excited, then the active orbitals they go to; refc [nelec] int32: the reference occupation;
+ * R [nw][ndet][nex][nact] c128: the compact R matrices; wgt [ndet][nw] c128. Out-of-range
+ * indices give NaN results for the walker (never an out-of-bounds access).
+ * ------------------------------------------------------------------------------------------- */
+/* ph_excited_2body_energy_dense_cholesky (phmsd_impl.hpp:29-147): Twina [nw][nelec][nchol][nact]
+ * c128; d_EX[w] -= sum_d wgt eX_d / 2 and d_EJ[w] += sum_d wgt (KE.KE - eJ0) / 2 accumulate;
+ * d_KE [ndet][nw][nchol] c128 is written. nex <= 8, nact <= 256, nelec <= 64. */
+int afq_ph_excited_2body_energy(int nw, int ndet, int nex, int nact, int nelec, int nchol,
+                                const int* d_iexcit, const int* d_refc, const double* d_Twina,
+                                const double* d_R, const double* d_wgt, double* d_EX,
+                                double* d_EJ, double* d_KE, void* stream);
+/* ph_excited_1body_energy (phmsd_impl.hpp:149-183): S [nw][nelec][nact] c128;
+ * d_E[w] += sum_d wgt sum_e (S[p_e] . R_e + S[p_e][v_e] - S[p_e][refc p_e]). */
+int afq_ph_excited_1body_energy(int nw, int ndet, int nex, int nact, int nelec,
+                                const int* d_iexcit, const int* d_refc, const double* d_S,
+                                const double* d_R, const double* d_wgt, double* d_E, void* stream);
 
 /* Lane layout of the MFMA builtins the kernels use, for a self-check; d_out [64][4] f64
  * receives accumulator element r of every lane.

@@ -868,6 +868,142 @@ void orc_system_stats(const orc_system* s, long* eloc_stats3, long* weight_stats
   for (int i = 0; i < 2; ++i) weight_stats2[i] = s->weight_stats[i];
 }
 
+// ---------------------------------------------------------------------------------------------
+// Particle-hole multi-determinant (PHMSD) energy corrections of the excited determinants.
+//
+// ph_excited_2body_energy_dense_cholesky (detail/phmsd_impl.hpp:29-147, HOST branch): for each
+// walker w and excited determinant d, whose excitation iexcit[d] = (electrons p_1..p_nex, i.e.
+// rows of refc, then the orbitals v_1..v_nex they move to), with the caller's walker tensor
+// Twina[w][i][n][a] (electron i, Cholesky vector n, active orbital a) and the compact R[w][d][e][a]
+// of the determinant:
+//   occ(i)  = refc(i), or v_e where i = p_e
+//   F_e     = Twina[w][p_e] R[w][d][e]                      (gemv over a, a vector over n)
+//   eX      = sum_e sum_n F_e^2 + 2 sum_{e<f} sum_n (Twina[p_f] R_e)(Twina[p_e] R_f)
+//           + 2 sum_e sum_j sum_a (sum_n Twina[j][n][a] Twina[p_e][n][occ j]) R_e[a]
+//           + sum_i (sum_n T[i][n][occ i]^2 - T[i][n][ref i]^2)         over changed rows
+//           + 2 sum_{i<j} (sum_n T[i][n][occ j] T[j][n][occ i] - T[i][n][ref j] T[j][n][ref i])
+//             where row i or j changed
+//   KE[d][w][n] = sum_e F_e[n] + sum_i Twina[w][i][n][occ i]
+//   EX[w] -= wgt[d][w] eX / 2;  EJ[w] += wgt[d][w] (sum_n KE^2 - (sum_i T[i][n][ref i])^2) / 2
+// All products are non-conjugated (complex-symmetric).
+void orc_ph_excited_2body(int nw, int ndet, int nex, int nact, int nelec, int nchol,
+                          const int* iexcit, const int* refc, const cd* Twina, const cd* R,
+                          const cd* wgt, cd* EX, cd* EJ, cd* KE) {
+#pragma omp parallel for schedule(dynamic)
+  for (int iw = 0; iw < nw; ++iw) {
+    std::vector<int> occ(nelec);
+    std::vector<cd> Fn(nchol), Fn1(nchol), Fa(nact), Kn(nchol), F0(nchol);
+    auto T = [&](int i, int n, int a) -> cd {
+      return Twina[(((long)iw * nelec + i) * nchol + n) * nact + a];
+    };
+    // T[i] (nchol x nact) times a vector over a
+    auto gemv = [&](int i, const cd* x, cd* y) {
+      for (int n = 0; n < nchol; ++n) {
+        cd acc = 0.0;
+        for (int a = 0; a < nact; ++a) acc += T(i, n, a) * x[a];
+        y[n] = acc;
+      }
+    };
+    cd ex_w = 0.0, ej_w = 0.0;
+    // reference part of the Coulomb term: F0 = sum_i T[i][:][refc i]
+    for (int n = 0; n < nchol; ++n) {
+      cd acc = 0.0;
+      for (int i = 0; i < nelec; ++i) acc += T(i, n, refc[i]);
+      F0[n] = acc;
+    }
+    cd eJ0 = 0.0;
+    for (int n = 0; n < nchol; ++n) eJ0 += F0[n] * F0[n];
+    for (int d = 0; d < ndet; ++d) {
+      const int* ip = iexcit + (long)d * 2 * nex;     // occupied (particles removed)
+      const int* iv = ip + nex;                        // virtual (added)
+      const cd* Rd = R + (((long)iw * ndet + d) * nex) * nact;
+      for (int i = 0; i < nelec; ++i) {
+        occ[i] = refc[i];
+        for (int e = 0; e < nex; ++e)
+          if (ip[e] == i) { occ[i] = iv[e]; break; }
+      }
+      std::fill(Kn.begin(), Kn.end(), cd(0.0));
+      cd eX = 0.0;
+      for (int e = 0; e < nex; ++e) {
+        gemv(ip[e], Rd + (long)e * nact, Fn.data());
+        for (int n = 0; n < nchol; ++n) {
+          eX += Fn[n] * Fn[n];
+          Kn[n] += Fn[n];
+        }
+        for (int f = e + 1; f < nex; ++f) {
+          gemv(ip[f], Rd + (long)e * nact, Fn.data());
+          gemv(ip[e], Rd + (long)f * nact, Fn1.data());
+          cd acc = 0.0;
+          for (int n = 0; n < nchol; ++n) acc += Fn[n] * Fn1[n];
+          eX += 2.0 * acc;
+        }
+        for (int j = 0; j < nelec; ++j) {
+          const int oj = occ[j];
+          for (int a = 0; a < nact; ++a) {
+            cd acc = 0.0;
+            for (int n = 0; n < nchol; ++n) acc += T(j, n, a) * T(ip[e], n, oj);
+            Fa[a] = acc;
+          }
+          cd acc = 0.0;
+          for (int a = 0; a < nact; ++a) acc += Fa[a] * Rd[(long)e * nact + a];
+          eX += 2.0 * acc;
+        }
+      }
+      for (int i = 0; i < nelec; ++i) {
+        const int oi = occ[i], ri = refc[i];
+        if (oi != ri) {
+          cd acc = 0.0;
+          for (int n = 0; n < nchol; ++n) acc += T(i, n, oi) * T(i, n, oi) - T(i, n, ri) * T(i, n, ri);
+          eX += acc;
+        }
+        for (int j = i + 1; j < nelec; ++j) {
+          const int oj = occ[j], rj = refc[j];
+          if (oi != ri || oj != rj) {
+            cd acc = 0.0;
+            for (int n = 0; n < nchol; ++n)
+              acc += T(i, n, oj) * T(j, n, oi) - T(i, n, rj) * T(j, n, ri);
+            eX += 2.0 * acc;
+          }
+        }
+      }
+      for (int i = 0; i < nelec; ++i)
+        for (int n = 0; n < nchol; ++n) Kn[n] += T(i, n, occ[i]);
+      cd kk = 0.0;
+      for (int n = 0; n < nchol; ++n) {
+        kk += Kn[n] * Kn[n];
+        KE[((long)d * nw + iw) * nchol + n] = Kn[n];
+      }
+      const cd wd = wgt[(long)d * nw + iw];
+      ex_w -= 0.5 * wd * eX;
+      ej_w += 0.5 * wd * (kk - eJ0);
+    }
+    EX[iw] += ex_w;
+    EJ[iw] += ej_w;
+  }
+}
+
+// ph_excited_1body_energy (detail/phmsd_impl.hpp:149-183, HOST branch):
+//   E[w] += sum_d wgt[d][w] sum_e ( sum_a S[w][p_e][a] R[w][d][e][a] + S[w][p_e][v_e] - S[w][p_e][refc p_e] )
+void orc_ph_excited_1body(int nw, int ndet, int nex, int nact, int nelec, const int* iexcit,
+                          const int* refc, const cd* S, const cd* R, const cd* wgt, cd* E) {
+  for (int iw = 0; iw < nw; ++iw) {
+    cd acc_w = 0.0;
+    for (int d = 0; d < ndet; ++d) {
+      const int* ip = iexcit + (long)d * 2 * nex;
+      const int* iv = ip + nex;
+      cd e_ = 0.0;
+      for (int e = 0; e < nex; ++e) {
+        const cd* Sp = S + ((long)iw * nelec + ip[e]) * nact;
+        const cd* Re = R + (((long)iw * ndet + d) * nex + e) * nact;
+        for (int a = 0; a < nact; ++a) e_ += Sp[a] * Re[a];
+        e_ += Sp[iv[e]] - Sp[refc[ip[e]]];
+      }
+      acc_w += wgt[(long)d * nw + iw] * e_;
+    }
+    E[iw] += acc_w;
+  }
+}
+
 int orc_num_threads(void) { return omp_get_max_threads(); }
 void orc_set_num_threads(int n) { omp_set_num_threads(n); }
 

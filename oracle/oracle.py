@@ -69,6 +69,9 @@ def _declare(L):
         "orc_bound_weights": (None, [_i, _vp, _l, _l, _d, _d, _d, _vp]),
         "orc_comb": (_i, [_i, _vp, _d, _i, _vp]),
         "orc_population_control": (_i, [_i, _vp, _l, _l, _d, _vp]),
+        "orc_ph_excited_2body": (None, [_i, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp,
+                                        _vp, _vp]),
+        "orc_ph_excited_1body": (None, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
         "orc_system_create": (_vp, [_i, _i, _i, _vp, _vp, _vp, _d, _d]),
         "orc_system_destroy": (None, [_vp]),
         "orc_system_set_params": (None, [_vp, _i, _d, _d, _d, _d, _d, _i, _i, _i]),
@@ -251,6 +254,35 @@ def population_control(buf, poff, r):
     off = np.zeros(W, np.int32)
     rc = lib().orc_population_control(W, _p(buf), buf.shape[1], poff, float(r), _p(off))
     return off, rc
+
+
+def ph_excited_2body(iexcit, refc, Twina, R, wgt):
+    """ph_excited_2body_energy_dense_cholesky (phmsd_impl.hpp:29-147): returns the walker sums
+    (EX[nw], EJ[nw]) and KE[ndet][nw][nchol]. iexcit [ndet][2][nex] (electron rows, then target
+    orbitals), refc [nelec], Twina [nw][nelec][nchol][nact], R [nw][ndet][nex][nact],
+    wgt [ndet][nw]."""
+    nw, ndet, nex, nact = R.shape
+    nelec, nchol = Twina.shape[1], Twina.shape[2]
+    iex = np.ascontiguousarray(iexcit, np.int32)
+    ref = np.ascontiguousarray(refc, np.int32)
+    EX = np.zeros(nw, np.complex128)
+    EJ = np.zeros(nw, np.complex128)
+    KE = np.zeros((ndet, nw, nchol), np.complex128)
+    lib().orc_ph_excited_2body(nw, ndet, nex, nact, nelec, nchol, _p(iex), _p(ref), _p(_c(Twina)),
+                               _p(_c(R)), _p(_c(wgt)), _p(EX), _p(EJ), _p(KE))
+    return EX, EJ, KE
+
+
+def ph_excited_1body(iexcit, refc, S, R, wgt):
+    """ph_excited_1body_energy (phmsd_impl.hpp:149-183): E[nw] = sum_d wgt sum_e (...)."""
+    nw, ndet, nex, nact = R.shape
+    nelec = S.shape[1]
+    iex = np.ascontiguousarray(iexcit, np.int32)
+    ref = np.ascontiguousarray(refc, np.int32)
+    E = np.zeros(nw, np.complex128)
+    lib().orc_ph_excited_1body(nw, ndet, nex, nact, nelec, _p(iex), _p(ref), _p(_c(S)),
+                               _p(_c(R)), _p(_c(wgt)), _p(E))
+    return E
 
 
 # ------------------------------------------------------------------ whole system

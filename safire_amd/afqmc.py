@@ -74,6 +74,9 @@ SIGNATURES = {
     "afq_ham_read_timers": (_i, [_vp, _vp, _vp]),
     "afq_timer_name": (C.c_char_p, [_i]),
     "afq_probe_mfma": (_i, [_i, _vp, _vp]),
+    "afq_ph_excited_2body_energy": (_i, [_i, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp,
+                                         _vp, _vp, _vp]),
+    "afq_ph_excited_1body_energy": (_i, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
 }
 NUM_REGIONS = 9
 

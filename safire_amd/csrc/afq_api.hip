@@ -115,7 +115,7 @@ void run_exchange(afq_ham H, hipStream_t s, int W, const double* G) {
 
 int exchange_chunks(afq_ham H, int W) {
   if (H->precision == AFQ_BF16X3) return exchange_bf16_chunks(W, H->K);
-  if (H->precision == AFQ_INT8_SPLIT) return exchange_i8_chunks(W, H->K);
+  if (H->precision == AFQ_INT8_SPLIT) return exchange_i8_chunks(W, H->N, H->K);
   return exx_nchunks(W, H->N, H->K);
 }
 
@@ -284,7 +284,7 @@ int afq_ham_reserve(afq_ham H, int max_walkers) {
     dalloc(&ws.ov, 2 * W);
     dalloc(&ws.E, 6 * W);
     dalloc(&ws.Kl, 2 * W * K);
-    dalloc(&ws.exx, 2 * W * (size_t)K);   // at most one chunk per Cholesky vector
+    dalloc(&ws.exx, 2 * W * (size_t)(K + 2));   // at most K + 1 tr(T^2) partials per walker
     ws.split_elems = (long)kMaxSplits * W * K;
     dalloc(&ws.split, 2 * (size_t)ws.split_elems);
     dalloc(&ws.cum, W);
@@ -303,7 +303,7 @@ int afq_ham_set_precision(afq_ham H, int precision) {
     AFQ_CHECK(precision != AFQ_BF16X3 || H->Lnak_b != nullptr, AFQ_EUNSUPPORTED,
               "BF16X3 exchange needs N <= 16 and M <= 96 with M % 8 == 0");
     AFQ_CHECK(precision != AFQ_INT8_SPLIT || H->Lnak_q != nullptr, AFQ_EUNSUPPORTED,
-              "INT8_SPLIT exchange needs N <= 16 and M <= 128");
+              "INT8_SPLIT exchange needs N <= 32 and M <= 128");
     H->precision = precision;
   });
 }
@@ -627,6 +627,28 @@ int afq_ham_read_timers(afq_ham H, double* h_ms, long long* h_counts) {
 
 const char* afq_timer_name(int region) {
   return (region >= 0 && region < R_COUNT) ? kRegionNames[region] : "";
+}
+
+int afq_ph_excited_2body_energy(int nw, int ndet, int nex, int nact, int nelec, int nchol,
+                                const int* iexcit, const int* refc, const double* Twina,
+                                const double* R, const double* wgt, double* EX, double* EJ,
+                                double* KE, void* stream) {
+  return guard([&] {
+    AFQ_CHECK(nw >= 0 && ndet >= 0 && nact > 0 && nelec > 0 && nelec <= nact && nchol >= 0,
+              AFQ_EBADSHAPE, "ph 2-body energy: bad sizes");
+    launch_ph_2body(S(stream), nw, ndet, nex, nact, nelec, nchol, iexcit, refc, Twina, R, wgt,
+                    EX, EJ, KE);
+  });
+}
+
+int afq_ph_excited_1body_energy(int nw, int ndet, int nex, int nact, int nelec,
+                                const int* iexcit, const int* refc, const double* Smat,
+                                const double* R, const double* wgt, double* E, void* stream) {
+  return guard([&] {
+    AFQ_CHECK(nw >= 0 && ndet >= 0 && nex >= 0 && nact > 0 && nelec > 0 && nelec <= nact,
+              AFQ_EBADSHAPE, "ph 1-body energy: bad sizes");
+    launch_ph_1body(S(stream), nw, ndet, nex, nact, nelec, iexcit, refc, Smat, R, wgt, E);
+  });
 }
 
 int afq_probe_mfma(int kind, double* out, void* stream) {

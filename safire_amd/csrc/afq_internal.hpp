@@ -211,7 +211,7 @@ void launch_exchange_bf16(hipStream_t s, int W, int M, int N, int K, const doubl
 bool exchange_i8_supported(int M, int N);
 int exchange_i8_row(int M);                        // Mp: padded row length of the slices
 size_t exchange_i8_bytes(int M, int N, int K);     // size of the Lnak slices
-int exchange_i8_chunks(int W, int K);
+int exchange_i8_chunks(int W, int N, int K);   // tr(T^2) partials per walker
 void split_lnak_i8(hipStream_t s, int M, int N, int K, const double* Lnak, int8_t* Lq,
                    double* Lu);
 void launch_exchange_i8(hipStream_t s, int W, int M, int N, int K, const double* G,
@@ -246,5 +246,12 @@ void launch_population_control(hipStream_t s, int W, double* wbuf, long ldw, lon
                                unsigned long long* stats);
 // k_setup.hip
 void setup_hamiltonian(afq_ham_s* H, hipStream_t s);
+// k_phmsd.hip
+void launch_ph_2body(hipStream_t s, int nw, int ndet, int nex, int nact, int nelec, int nchol,
+                     const int* iexcit, const int* refc, const double* Twina, const double* R,
+                     const double* wgt, double* EX, double* EJ, double* KE);
+void launch_ph_1body(hipStream_t s, int nw, int ndet, int nex, int nact, int nelec,
+                     const int* iexcit, const int* refc, const double* S, const double* R,
+                     const double* wgt, double* E);
 
 }  // namespace afq

@@ -82,6 +82,31 @@ __device__ inline i4 mfma_i8(i4 a, i4 b, i4 c) {
   return __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, c, 0, 0, 0);
 }
 
+// v plus the value of the lane `shift` below in the same row of 16 lanes (0 below the row)
+template <int SHIFT>
+__device__ inline double add_row_shr(double v) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = __builtin_amdgcn_update_dpp(0u, (unsigned)u, 0x110 + SHIFT, 0xf, 0xf, true);
+  const unsigned hi =
+      __builtin_amdgcn_update_dpp(0u, (unsigned)(u >> 32), 0x110 + SHIFT, 0xf, 0xf, true);
+  return v + __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// lane 15 of each row of 16 lanes receives the sum of the row (DPP row shifts, no LDS)
+__device__ inline double row_sum16(double v) {
+  v = add_row_shr<1>(v);
+  v = add_row_shr<2>(v);
+  v = add_row_shr<4>(v);
+  return add_row_shr<8>(v);
+}
+
+__device__ inline double readlane_d(double v, int lane) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = __builtin_amdgcn_readlane((unsigned)u, lane);
+  const unsigned hi = __builtin_amdgcn_readlane((unsigned)(u >> 32), lane);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
 // orders LDS traffic of one wavefront: its own writes are visible to its own later reads
 __device__ inline void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -170,21 +195,28 @@ __global__ __launch_bounds__(64 * kWPB, kWgPerCu * kWPB / 4) void k_exchange_i8(
   double* T = sT[wave];
   const int nvec = 4 * KC;          // 16-byte vectors per staged row
 
-  // group loads: slice rows of vectors nv0 .. nv0 + kStage - 1 into registers (zero outside)
+  // group loads: slice rows of vectors nv0 .. nv0 + kStage - 1 into registers (zero outside).
+  // Item e = i * 64 kWPB + tid is (row b, 16-byte column kv) = pos, which is fixed per thread
+  // (16 nvec divides 64 kWPB), and (vector v, slice-part tp) = combo, which is fixed per
+  // wavefront and slot, so only the thread's part of the address lives in vector registers.
+  static_assert((64 * kWPB) % (16 * 4 * KC) == 0, "staging positions must tile the workgroup");
+  const int pos = threadIdx.x % (16 * nvec);
+  const int pb = pos / nvec, pkv = pos % nvec;
+  const bool pos_ok = pb < N && 16 * pkv < Mp;
+  const long gpos = (long)pb * Mp + 16 * pkv;
+  const int lpos = pb * ROWB + 16 * pkv;
   uint4 pre[kPre];
   double preU = 0.0;
   auto load_group = [&](int nv0) {
 #pragma unroll
     for (int i = 0; i < kPre; ++i) {
-      const int e = threadIdx.x + i * 64 * kWPB;
-      const int kv = e % nvec;
-      const int b = (e / nvec) % 16;
-      const int tp = (e / (nvec * 16)) % (kSL * 2);
-      const int v = e / (nvec * 16 * kSL * 2);
+      const int combo =
+          __builtin_amdgcn_readfirstlane((i * 64 * kWPB + (int)threadIdx.x) / (16 * nvec));
+      const int tp = combo % (kSL * 2), v = combo / (kSL * 2);
       const int n = nv0 + v;
       pre[i] = uint4{0u, 0u, 0u, 0u};
-      if (e < kGroupVec && n < nend && b < N && 16 * kv < Mp)
-        pre[i] = *reinterpret_cast<const uint4*>(Lq + tp * KNMp + ((long)n * N + b) * Mp + 16 * kv);
+      if (combo < kStage * kSL * 2 && n < nend && pos_ok)
+        pre[i] = *reinterpret_cast<const uint4*>(Lq + tp * KNMp + (long)n * N * Mp + gpos);
     }
     if (threadIdx.x < kStage * 16) {
       const int v = threadIdx.x / 16, b = threadIdx.x % 16, n = nv0 + v;
@@ -194,14 +226,10 @@ __global__ __launch_bounds__(64 * kWPB, kWgPerCu * kWPB / 4) void k_exchange_i8(
   auto store_group = [&](int buf) {
 #pragma unroll
     for (int i = 0; i < kPre; ++i) {
-      const int e = threadIdx.x + i * 64 * kWPB;
-      const int kv = e % nvec;
-      const int b = (e / nvec) % 16;
-      const int tp = (e / (nvec * 16)) % (kSL * 2);
-      const int v = e / (nvec * 16 * kSL * 2);
-      if (e < kGroupVec)
-        *reinterpret_cast<uint4*>(sBuf[buf] + ((v * kSL * 2 + tp) * 16 + b) * ROWB + 16 * kv) =
-            pre[i];
+      const int combo =
+          __builtin_amdgcn_readfirstlane((i * 64 * kWPB + (int)threadIdx.x) / (16 * nvec));
+      if (combo < kStage * kSL * 2)   // combo = v * 2 kSL + tp: rows of slice-part combo
+        *reinterpret_cast<uint4*>(sBuf[buf] + combo * 16 * ROWB + lpos) = pre[i];
     }
     if (threadIdx.x < kStage * 16) sUnit[buf][threadIdx.x / 16][threadIdx.x % 16] = preU;
   };
@@ -261,10 +289,11 @@ __global__ __launch_bounds__(64 * kWPB, kWgPerCu * kWPB / 4) void k_exchange_i8(
           if (KC == 1) {
             // M <= 64: |acc_o| <= (o + 1) 2 64 127^2, so 254 acc0 + acc1 and 254 acc2 + acc3
             // stay below 2^31 and pair up exactly in int32 (three conversions, not five)
-            const int pr01 = 254 * accR[v][0][r] + accR[v][1][r];
-            const int pi01 = 254 * accI[v][0][r] + accI[v][1][r];
-            const int pr23 = 254 * accR[v][2][r] + accR[v][3][r];
-            const int pi23 = 254 * accI[v][2][r] + accI[v][3][r];
+            // (24-bit multiplies: |acc0|, |acc2| < 2^23)
+            const int pr01 = __mul24(254, accR[v][0][r]) + accR[v][1][r];
+            const int pi01 = __mul24(254, accI[v][0][r]) + accI[v][1][r];
+            const int pr23 = __mul24(254, accR[v][2][r]) + accR[v][3][r];
+            const int pi23 = __mul24(254, accI[v][2][r]) + accI[v][3][r];
             double xr = fma((double)accR[v][4][r], 1.0 / kBase, (double)pr23);
             double xi = fma((double)accI[v][4][r], 1.0 / kBase, (double)pi23);
             xr = fma(xr, 1.0 / (kBase * kBase), (double)pr01);
@@ -300,17 +329,15 @@ __global__ __launch_bounds__(64 * kWPB, kWgPerCu * kWPB / 4) void k_exchange_i8(
             exx_re += tr[r] * yr - ti[r] * yi;
             exx_im += tr[r] * yi + ti[r] * yr;
           }
-          // tr(T): the N diagonal entries, summed over lanes 0..15 in four rounds
+          // tr(T): the N diagonal entries in lanes 0..15, summed into lane 15 by DPP row shifts
           double dr = 0.0, di = 0.0;
           if (lane < N) {
             dr = T[2 * (lane * 16 + lane)];
             di = T[2 * (lane * 16 + lane) + 1];
           }
-          for (int off = 8; off > 0; off >>= 1) {
-            dr += __shfl_xor(dr, off, 16);
-            di += __shfl_xor(di, off, 16);
-          }
-          if (lane == 0 && active) {
+          dr = row_sum16(dr);
+          di = row_sum16(di);
+          if (lane == 15 && active) {
             Kl[2 * ((long)w * K + n)] = dr;
             Kl[2 * ((long)w * K + n) + 1] = di;
           }
@@ -329,6 +356,195 @@ __global__ __launch_bounds__(64 * kWPB, kWgPerCu * kWPB / 4) void k_exchange_i8(
   if (lane == 0 && active) {
     exx_part[2 * ((long)w * nchunk + c)] = exx_re;
     exx_part[2 * ((long)w * nchunk + c) + 1] = exx_im;
+  }
+}
+
+// N in 17..32: T_wn is 2 x 2 tiles of 16 x 16. A 4-wavefront workgroup holds 2 walkers; the
+// two wavefronts of a walker own its row tiles ta = 0, 1 (G rows 16 ta .. 16 ta + 15 sliced into
+// registers) and compute both column tiles of their rows. Each vector's T goes to a per-walker
+// 32 x 32 LDS block; after a barrier every wavefront takes tr(T^2) over its rows and wavefront 0
+// of the walker takes tr(T) from the diagonal. The two row tiles write separate tr(T^2) partials
+// (slots 2c and 2c + 1 of the walker). One vector is staged at a time (32 rows per slice-part);
+// LDS is ~79 KB at M = 128, so two workgroups share a CU.
+template <int KC>
+__global__ __launch_bounds__(256, 2) void k_exchange_i8_t2(
+    int W, int M, int N, int K, int Mp, int kChunk, const double* __restrict__ G,
+    const int8_t* __restrict__ Lq, const double* __restrict__ Lu, double* __restrict__ Kl,
+    double* __restrict__ exx_part, int nslots) {
+  constexpr int ROWB = 64 * KC + 16;
+  constexpr int kRows = 32;
+  __shared__ __attribute__((aligned(16))) int8_t sB[kSL * 2 * kRows * ROWB];
+  __shared__ double sU[kRows];
+  __shared__ double sT[2][kRows * kRows * 2];      // per walker T [a][b] (re, im)
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int g = lane >> 4, r16 = lane & 15;
+  const int wl = wave >> 1, ta = wave & 1;
+  const int w_raw = blockIdx.x * 2 + wl;
+  const bool active = w_raw < W;
+  const int w = active ? w_raw : W - 1;
+  const int c = blockIdx.y;
+  const int nbeg = c * kChunk;
+  const int nend = min(K, nbeg + kChunk);
+  const long KNMp = (long)K * N * Mp;
+  const int arow = ta * 16 + r16;                   // this lane's row of G
+
+  i4 ar[KC][kSG], ai[KC][kSG], nai[KC][kSG];
+  double uG;
+  {
+    const double* Gw = G + 2 * ((long)w * N + arow) * M;
+    double mx = 0.0;
+    if (arow < N) {
+      for (int kc = 0; kc < KC; ++kc)
+        for (int j = 0; j < 16; ++j) {
+          const int k = 64 * kc + 16 * g + j;
+          if (k < M) {
+            const double xr = fabs(Gw[2 * k]), xi = fabs(Gw[2 * k + 1]);
+            mx = (xr <= 1.0e300 && xi <= 1.0e300) ? fmax(mx, fmax(xr, xi)) : INFINITY;
+          }
+        }
+    }
+    mx = fmax(mx, __shfl_xor(mx, 16, 64));
+    mx = fmax(mx, __shfl_xor(mx, 32, 64));
+    bool zero;
+    uG = row_unit(mx, zero);
+    const bool finite = uG == uG;
+    const double inv = (finite && !zero) ? 1.0 / uG : 0.0;
+#pragma unroll
+    for (int kc = 0; kc < KC; ++kc) {
+#pragma unroll
+      for (int d = 0; d < 4; ++d) {
+        int qr[4][kSG], qi[4][kSG];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int k = 64 * kc + 16 * g + 4 * d + j;
+          double xr = 0.0, xi = 0.0;
+          if (arow < N && k < M && inv != 0.0) {
+            xr = Gw[2 * k] * inv;
+            xi = Gw[2 * k + 1] * inv;
+          }
+          slice<kSG>(xr, qr[j]);
+          slice<kSG>(xi, qi[j]);
+        }
+#pragma unroll
+        for (int s = 0; s < kSG; ++s) {
+          ar[kc][s][d] = (int)pack4(qr[0][s], qr[1][s], qr[2][s], qr[3][s]);
+          ai[kc][s][d] = (int)pack4(qi[0][s], qi[1][s], qi[2][s], qi[3][s]);
+          nai[kc][s][d] = (int)pack4(-qi[0][s], -qi[1][s], -qi[2][s], -qi[3][s]);
+        }
+      }
+    }
+  }
+  double uRow[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) uRow[r] = __shfl(uG, 4 * g + r, 64);
+
+  double exx_re = 0.0, exx_im = 0.0;
+  double* T = sT[wl];
+  const int nvec = 4 * KC;
+
+  for (int n = nbeg; n < nend; ++n) {
+    __syncthreads();   // the previous vector's LDS reads are done
+    for (int e = threadIdx.x; e < kSL * 2 * kRows * nvec; e += 256) {
+      const int kv = e % nvec;
+      const int b = (e / nvec) % kRows;
+      const int tp = e / (nvec * kRows);
+      uint4 val = {0u, 0u, 0u, 0u};
+      if (b < N && 16 * kv < Mp)
+        val = *reinterpret_cast<const uint4*>(Lq + tp * KNMp + ((long)n * N + b) * Mp + 16 * kv);
+      *reinterpret_cast<uint4*>(sB + (tp * kRows + b) * ROWB + 16 * kv) = val;
+    }
+    if (threadIdx.x < kRows)
+      sU[threadIdx.x] = (int)threadIdx.x < N ? Lu[(long)n * N + threadIdx.x] : 0.0;
+    __syncthreads();
+
+    i4 accR[2][kOrders], accI[2][kOrders];
+#pragma unroll
+    for (int tb = 0; tb < 2; ++tb)
+#pragma unroll
+      for (int o = 0; o < kOrders; ++o) {
+        accR[tb][o] = i4{0, 0, 0, 0};
+        accI[tb][o] = i4{0, 0, 0, 0};
+      }
+#pragma unroll
+    for (int kc = 0; kc < KC; ++kc)
+#pragma unroll
+      for (int tb = 0; tb < 2; ++tb)
+#pragma unroll
+        for (int t = 0; t < kSL; ++t) {
+          const int8_t* base = sB + ((t * 2) * kRows + tb * 16 + r16) * ROWB + 64 * kc + 16 * g;
+          const i4 br = *reinterpret_cast<const i4*>(base);
+          const i4 bi = *reinterpret_cast<const i4*>(base + kRows * ROWB);
+#pragma unroll
+          for (int s = 0; s < kSG; ++s)
+            if (kept(s, t)) {
+              accR[tb][s + t] = mfma_i8(ar[kc][s], br, accR[tb][s + t]);
+              accI[tb][s + t] = mfma_i8(ar[kc][s], bi, accI[tb][s + t]);
+            }
+#pragma unroll
+          for (int s = 0; s < kSG; ++s)
+            if (kept(s, t)) {
+              accR[tb][s + t] = mfma_i8(nai[kc][s], bi, accR[tb][s + t]);
+              accI[tb][s + t] = mfma_i8(ai[kc][s], br, accI[tb][s + t]);
+            }
+        }
+
+    // T entries of rows 16 ta + 4g + r, columns 16 tb + r16, assembled in FP64
+    double tr[2][4], ti[2][4];
+#pragma unroll
+    for (int tb = 0; tb < 2; ++tb) {
+      const double uL = sU[tb * 16 + r16];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        double xr = (double)accR[tb][kOrders - 1][r], xi = (double)accI[tb][kOrders - 1][r];
+#pragma unroll
+        for (int o = kOrders - 2; o >= 0; --o) {
+          xr = fma(xr, 1.0 / kBase, (double)accR[tb][o][r]);
+          xi = fma(xi, 1.0 / kBase, (double)accI[tb][o][r]);
+        }
+        const double u = uRow[r] * uL;
+        tr[tb][r] = xr * u;
+        ti[tb][r] = xi * u;
+        const int a = ta * 16 + 4 * g + r, b = tb * 16 + r16;
+        T[2 * (a * kRows + b)] = tr[tb][r];
+        T[2 * (a * kRows + b) + 1] = ti[tb][r];
+      }
+    }
+    __syncthreads();   // both row tiles of T are in LDS
+#pragma unroll
+    for (int tb = 0; tb < 2; ++tb)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int a = ta * 16 + 4 * g + r, b = tb * 16 + r16;
+        const double yr = T[2 * (b * kRows + a)], yi = T[2 * (b * kRows + a) + 1];
+        exx_re += tr[tb][r] * yr - ti[tb][r] * yi;
+        exx_im += tr[tb][r] * yi + ti[tb][r] * yr;
+      }
+    if (ta == 0) {
+      // tr(T): diagonal entries a < N in lanes 0..31, summed by rows of 16 and combined
+      double dr = 0.0, di = 0.0;
+      if (lane < N && lane < kRows) {
+        dr = T[2 * (lane * kRows + lane)];
+        di = T[2 * (lane * kRows + lane) + 1];
+      }
+      dr = row_sum16(dr);
+      di = row_sum16(di);
+      const double kr = readlane_d(dr, 15) + readlane_d(dr, 31);
+      const double ki = readlane_d(di, 15) + readlane_d(di, 31);
+      if (lane == 0 && active) {
+        Kl[2 * ((long)w * K + n)] = kr;
+        Kl[2 * ((long)w * K + n) + 1] = ki;
+      }
+    }
+  }
+
+  for (int off = 32; off > 0; off >>= 1) {
+    exx_re += __shfl_xor(exx_re, off, 64);
+    exx_im += __shfl_xor(exx_im, off, 64);
+  }
+  if (lane == 0 && active) {
+    exx_part[2 * ((long)w * nslots + 2 * c + ta)] = exx_re;
+    exx_part[2 * ((long)w * nslots + 2 * c + ta) + 1] = exx_im;
   }
 }
 
@@ -362,7 +578,7 @@ __global__ void k_split_lnak_i8(int rows, int M, int Mp, long KNMp, const double
 
 }  // namespace
 
-bool exchange_i8_supported(int M, int N) { return N <= 16 && M <= 64 * kMaxKC; }
+bool exchange_i8_supported(int M, int N) { return N <= 32 && M <= 64 * kMaxKC; }
 
 int exchange_i8_row(int M) { return (M + 15) / 16 * 16; }
 
@@ -370,15 +586,18 @@ size_t exchange_i8_bytes(int M, int N, int K) {
   return (size_t)kSL * 2 * K * N * exchange_i8_row(M);
 }
 
-// whole rounds of one workgroup per CU, in multiples of the largest staging group
-int exchange_i8_chunk(int W, int K) {
-  constexpr int kStage = stage_of(1);
+// whole rounds of the resident workgroups (two per CU), in multiples of the staging group; for
+// N > 16 (two walkers per workgroup, one vector staged at a time) chunks of at least 2 vectors,
+// so that the 2 tr(T^2) partials per chunk fit the K partials per walker of the workspace
+int exchange_i8_chunk(int W, int N, int K) {
+  const int step = N > 16 ? 2 : stage_of(1);
+  const int wpb = N > 16 ? 2 : kWPB;
   const int cus = device_cus() * kWgPerCu;
-  const int wgroups = (W + kWPB - 1) / kWPB;
-  int best = kStage;
+  const int wgroups = (W + wpb - 1) / wpb;
+  int best = step;
   double best_eff = -1.0;
-  for (int j = 1; j <= 8; ++j) {
-    const int chunk = kStage * j;
+  for (int j = 1; j <= 16; ++j) {
+    const int chunk = step * j;
     const long nwg = (long)wgroups * ((K + chunk - 1) / chunk);
     const long rounds = (nwg + cus - 1) / cus;
     const double eff = (double)wgroups * K / ((double)rounds * cus * chunk);
@@ -390,9 +609,10 @@ int exchange_i8_chunk(int W, int K) {
   return best;
 }
 
-int exchange_i8_chunks(int W, int K) {
-  const int chunk = exchange_i8_chunk(W, K);
-  return (K + chunk - 1) / chunk;
+// tr(T^2) partials per walker: one per chunk, two (one per row tile) for N > 16
+int exchange_i8_chunks(int W, int N, int K) {
+  const int chunk = exchange_i8_chunk(W, N, K);
+  return ((K + chunk - 1) / chunk) * (N > 16 ? 2 : 1);
 }
 
 void split_lnak_i8(hipStream_t s, int M, int N, int K, const double* Lnak, int8_t* Lq,
@@ -408,17 +628,28 @@ void launch_exchange_i8(hipStream_t s, int W, int M, int N, int K, const double*
                         const int8_t* Lq, const double* Lu, double* Kl, double* exx_part) {
   if (W <= 0 || K <= 0) return;
   AFQ_CHECK(exchange_i8_supported(M, N), AFQ_EUNSUPPORTED,
-            "int8 exchange: needs N <= 16 and M <= 128");
-  const int chunk = exchange_i8_chunk(W, K);
+            "int8 exchange: needs N <= 32 and M <= 128");
+  const int chunk = exchange_i8_chunk(W, N, K);
   const int nchunk = (K + chunk - 1) / chunk;
   const int Mp = exchange_i8_row(M);
-  dim3 grid((W + kWPB - 1) / kWPB, nchunk);
-  if (M <= 64)
-    hipLaunchKernelGGL((k_exchange_i8<1, 2>), grid, dim3(64 * kWPB), 0, s, W, M, N, K, Mp, chunk,
-                       G, Lq, Lu, Kl, exx_part, nchunk);
-  else
-    hipLaunchKernelGGL((k_exchange_i8<2, 1>), grid, dim3(64 * kWPB), 0, s, W, M, N, K, Mp, chunk,
-                       G, Lq, Lu, Kl, exx_part, nchunk);
+  if (N > 16) {
+    dim3 grid((W + 1) / 2, nchunk);
+    const int nslots = 2 * nchunk;
+    if (M <= 64)
+      hipLaunchKernelGGL((k_exchange_i8_t2<1>), grid, dim3(256), 0, s, W, M, N, K, Mp, chunk, G,
+                         Lq, Lu, Kl, exx_part, nslots);
+    else
+      hipLaunchKernelGGL((k_exchange_i8_t2<2>), grid, dim3(256), 0, s, W, M, N, K, Mp, chunk, G,
+                         Lq, Lu, Kl, exx_part, nslots);
+  } else {
+    dim3 grid((W + kWPB - 1) / kWPB, nchunk);
+    if (M <= 64)
+      hipLaunchKernelGGL((k_exchange_i8<1, 2>), grid, dim3(64 * kWPB), 0, s, W, M, N, K, Mp,
+                         chunk, G, Lq, Lu, Kl, exx_part, nchunk);
+    else
+      hipLaunchKernelGGL((k_exchange_i8<2, 1>), grid, dim3(64 * kWPB), 0, s, W, M, N, K, Mp,
+                         chunk, G, Lq, Lu, Kl, exx_part, nchunk);
+  }
   AFQ_HIP(hipGetLastError());
 }
 

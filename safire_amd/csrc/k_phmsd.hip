@@ -1,0 +1,217 @@
+// Particle-hole multi-determinant (PHMSD) energy corrections of the excited determinants,
+// detail/phmsd_impl.hpp:
+//   ph_excited_2body_energy_dense_cholesky (29-147): EX[w] -= sum_d wgt[d][w] eX_d / 2,
+//     EJ[w] += sum_d wgt[d][w] (sum_n KE[d][w][n]^2 - eJ0_w) / 2, KE[d][w][n] written
+//   ph_excited_1body_energy (149-183): E[w] += sum_d wgt[d][w] e_d
+// for excitations iexcit[d] = (electron rows p_1..p_nex of refc, then their target orbitals),
+// the walker tensor Twina[w][i][n][a] (electron i, Cholesky vector n, active orbital a) and the
+// compact R[w][d][e][a] of each determinant.
+//
+// One workgroup per walker walks the determinants in order; its threads split the Cholesky index
+// n, so every term of eX (all of them are sums over n) and KE[d][w][n] come out per thread
+// without a reduction per determinant. The weight of a determinant is a scalar, so each thread
+// keeps weighted running sums and the workgroup reduces them once, in a fixed order (no atomics:
+// results do not depend on scheduling). R[w][d] and the determinant's occupation are staged in
+// LDS per determinant. Every index read from the excitation lists is clamped into range before
+// it addresses Twina; a determinant with an out-of-range index makes the walker's results NaN.
+#include "afq_internal.hpp"
+
+namespace afq {
+
+namespace {
+
+constexpr int kPhThreads = 256;
+constexpr int kPhMaxEx = 8;        // excitation rank handled per determinant
+constexpr int kPhMaxAct = 256;     // active orbitals
+constexpr int kPhMaxElec = 64;
+
+__device__ inline cd block_sum(cd v, cd* red) {
+  const int t = threadIdx.x;
+  for (int off = 32; off > 0; off >>= 1) {
+    v.re += __shfl_xor(v.re, off, 64);
+    v.im += __shfl_xor(v.im, off, 64);
+  }
+  __syncthreads();
+  if ((t & 63) == 0) red[t >> 6] = v;
+  __syncthreads();
+  cd s = cmk(0.0, 0.0);
+  for (int i = 0; i < kPhThreads / 64; ++i) s += red[i];   // fixed order
+  __syncthreads();
+  return s;
+}
+
+template <int NEX>
+__global__ __launch_bounds__(kPhThreads) void k_ph_2body(
+    int nw, int ndet, int nact, int nelec, int nchol, const int* __restrict__ iexcit,
+    const int* __restrict__ refc, const cd* __restrict__ Twina, const cd* __restrict__ R,
+    const cd* __restrict__ wgt, cd* __restrict__ EX, cd* __restrict__ EJ, cd* __restrict__ KE) {
+  __shared__ cd sR[NEX * kPhMaxAct];
+  __shared__ int sOcc[kPhMaxElec], sRef[kPhMaxElec], sP[NEX];
+  __shared__ int sBad;
+  __shared__ cd red[kPhThreads / 64];
+  const int w = blockIdx.x;
+  const int t = threadIdx.x;
+  const cd* Tw = Twina + (long)w * nelec * nchol * nact;
+  auto T = [&](int i, int n, int a) { return Tw[((long)i * nchol + n) * nact + a]; };
+  if (t == 0) {
+    int bad = 0;
+    for (int i = 0; i < nelec; ++i) bad |= (refc[i] < 0 || refc[i] >= nact);
+    sBad = bad;
+  }
+  for (int i = t; i < nelec; i += kPhThreads) sRef[i] = min(max(refc[i], 0), nact - 1);
+  __syncthreads();
+  const bool bad_ref = sBad != 0;
+
+  // eJ0 = sum_n (sum_i T[i][n][refc i])^2, as thread partials over n
+  cd ej0 = cmk(0.0, 0.0);
+  for (int n = t; n < nchol; n += kPhThreads) {
+    cd f0 = cmk(0.0, 0.0);
+    for (int i = 0; i < nelec; ++i) f0 += T(i, n, sRef[i]);
+    ej0 += f0 * f0;
+  }
+
+  cd ex_acc = cmk(0.0, 0.0), ej_acc = cmk(0.0, 0.0);
+  for (int d = 0; d < ndet; ++d) {
+    const int* ex = iexcit + (long)d * 2 * NEX;
+    __syncthreads();   // the previous determinant's LDS reads are done
+    for (int e = t; e < NEX * nact; e += kPhThreads)
+      sR[e] = R[(((long)w * ndet + d) * NEX) * nact + e];
+    if (t < NEX) sP[t] = min(max(ex[t], 0), nelec - 1);
+    if (t == 0) {
+      int bad = bad_ref ? 1 : 0;
+      for (int e = 0; e < NEX; ++e)
+        bad |= (ex[e] < 0 || ex[e] >= nelec || ex[NEX + e] < 0 || ex[NEX + e] >= nact);
+      sBad = bad;
+    }
+    for (int i = t; i < nelec; i += kPhThreads) {
+      int o = sRef[i];
+      for (int e = 0; e < NEX; ++e)
+        if (ex[e] == i) {
+          o = min(max(ex[NEX + e], 0), nact - 1);
+          break;
+        }
+      sOcc[i] = o;
+    }
+    __syncthreads();
+
+    cd eX = cmk(0.0, 0.0), kk = cmk(0.0, 0.0);
+    for (int n = t; n < nchol; n += kPhThreads) {
+      // Y_ef = sum_a T[p_e][n][a] R_f[a]: the diagonal terms and the symmetric pairs
+      cd kn = cmk(0.0, 0.0);
+#pragma unroll
+      for (int e = 0; e < NEX; ++e) {
+        cd yee = cmk(0.0, 0.0);
+        for (int a = 0; a < nact; ++a) yee += T(sP[e], n, a) * sR[e * nact + a];
+        eX += yee * yee;
+        kn += yee;
+#pragma unroll
+        for (int f = e + 1; f < NEX; ++f) {
+          cd yfe = cmk(0.0, 0.0), yef = cmk(0.0, 0.0);
+          for (int a = 0; a < nact; ++a) {
+            yfe += T(sP[f], n, a) * sR[e * nact + a];
+            yef += T(sP[e], n, a) * sR[f * nact + a];
+          }
+          eX += 2.0 * (yfe * yef);
+        }
+        // R_e against the occupied columns: sum_j T[p_e][n][occ j] (T[j][n][:] . R_e)
+        for (int j = 0; j < nelec; ++j) {
+          cd z = cmk(0.0, 0.0);
+          for (int a = 0; a < nact; ++a) z += T(j, n, a) * sR[e * nact + a];
+          eX += 2.0 * (T(sP[e], n, sOcc[j]) * z);
+        }
+      }
+      // rows whose occupation changed
+      for (int i = 0; i < nelec; ++i) {
+        const int oi = sOcc[i], ri = sRef[i];
+        const cd tio = T(i, n, oi), tir = T(i, n, ri);
+        if (oi != ri) eX += tio * tio - tir * tir;
+        for (int j = i + 1; j < nelec; ++j) {
+          const int oj = sOcc[j], rj = sRef[j];
+          if (oi != ri || oj != rj)
+            eX += 2.0 * (T(i, n, oj) * T(j, n, oi) - T(i, n, rj) * T(j, n, ri));
+        }
+        kn += tio;
+      }
+      KE[((long)d * nw + w) * nchol + n] = kn;
+      kk += kn * kn;
+    }
+    const cd wd = wgt[(long)d * nw + w];
+    if (sBad) eX = cmk(__builtin_nan(""), __builtin_nan(""));
+    ex_acc += wd * eX;
+    ej_acc += wd * (kk - ej0);
+  }
+  const cd sx = block_sum(ex_acc, red);
+  const cd sj = block_sum(ej_acc, red);
+  if (t == 0) {
+    EX[w] -= 0.5 * sx;
+    EJ[w] += 0.5 * sj;
+  }
+}
+
+// E[w] += sum_d wgt[d][w] sum_e (S[w][p_e] . R[w][d][e] + S[w][p_e][v_e] - S[w][p_e][refc p_e])
+__global__ __launch_bounds__(kPhThreads) void k_ph_1body(
+    int nw, int ndet, int nex, int nact, int nelec, const int* __restrict__ iexcit,
+    const int* __restrict__ refc, const cd* __restrict__ S, const cd* __restrict__ R,
+    const cd* __restrict__ wgt, cd* __restrict__ E) {
+  __shared__ cd red[kPhThreads / 64];
+  const int w = blockIdx.x;
+  cd acc = cmk(0.0, 0.0);
+  for (int d = threadIdx.x; d < ndet; d += kPhThreads) {
+    const int* ex = iexcit + (long)d * 2 * nex;
+    cd e_ = cmk(0.0, 0.0);
+    for (int e = 0; e < nex; ++e) {
+      const bool ok = ex[e] >= 0 && ex[e] < nelec && ex[nex + e] >= 0 && ex[nex + e] < nact;
+      const int p = ok ? ex[e] : 0, v = ok ? ex[nex + e] : 0;
+      const int rp = (ok && refc[p] >= 0 && refc[p] < nact) ? refc[p] : -1;
+      const cd* Sp = S + ((long)w * nelec + p) * nact;
+      const cd* Re = R + (((long)w * ndet + d) * nex + e) * nact;
+      for (int a = 0; a < nact; ++a) e_ += Sp[a] * Re[a];
+      if (rp < 0) e_ = cmk(__builtin_nan(""), __builtin_nan(""));
+      else e_ += Sp[v] - Sp[rp];
+    }
+    acc += wgt[(long)d * nw + w] * e_;
+  }
+  const cd s = block_sum(acc, red);
+  if (threadIdx.x == 0) E[w] += s;
+}
+
+}  // namespace
+
+void launch_ph_2body(hipStream_t s, int nw, int ndet, int nex, int nact, int nelec, int nchol,
+                     const int* iexcit, const int* refc, const double* Twina, const double* R,
+                     const double* wgt, double* EX, double* EJ, double* KE) {
+  if (nw <= 0) return;
+  AFQ_CHECK(nex >= 1 && nex <= kPhMaxEx, AFQ_EUNSUPPORTED, "ph 2-body energy: nex must be 1..8");
+  AFQ_CHECK(nact <= kPhMaxAct && nelec <= kPhMaxElec, AFQ_EUNSUPPORTED,
+            "ph 2-body energy: nact <= 256 and nelec <= 64");
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3(nw), dim3(kPhThreads), 0, s, nw, ndet, nact, nelec, nchol,
+                       iexcit, refc, reinterpret_cast<const cd*>(Twina),
+                       reinterpret_cast<const cd*>(R), reinterpret_cast<const cd*>(wgt),
+                       reinterpret_cast<cd*>(EX), reinterpret_cast<cd*>(EJ),
+                       reinterpret_cast<cd*>(KE));
+  };
+  switch (nex) {
+    case 1: go(k_ph_2body<1>); break;
+    case 2: go(k_ph_2body<2>); break;
+    case 3: go(k_ph_2body<3>); break;
+    case 4: go(k_ph_2body<4>); break;
+    case 5: go(k_ph_2body<5>); break;
+    case 6: go(k_ph_2body<6>); break;
+    case 7: go(k_ph_2body<7>); break;
+    default: go(k_ph_2body<8>); break;
+  }
+  AFQ_HIP(hipGetLastError());
+}
+
+void launch_ph_1body(hipStream_t s, int nw, int ndet, int nex, int nact, int nelec,
+                     const int* iexcit, const int* refc, const double* S, const double* R,
+                     const double* wgt, double* E) {
+  if (nw <= 0) return;
+  hipLaunchKernelGGL(k_ph_1body, dim3(nw), dim3(kPhThreads), 0, s, nw, ndet, nex, nact, nelec,
+                     iexcit, refc, reinterpret_cast<const cd*>(S), reinterpret_cast<const cd*>(R),
+                     reinterpret_cast<const cd*>(wgt), reinterpret_cast<cd*>(E));
+  AFQ_HIP(hipGetLastError());
+}
+
+}  // namespace afq

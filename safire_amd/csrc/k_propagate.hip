@@ -19,19 +19,6 @@ namespace {
 
 constexpr int kWaves = 4;
 
-// A operand fragment of one 16-wide k-chunk: A[row][k0 + 4g + s], s < 4
-__device__ inline void load_a(const cd* __restrict__ A, long lda, int M, int row, int k0, int g,
-                              double (&ar)[4], double (&ai)[4]) {
-#pragma unroll
-  for (int s = 0; s < 4; ++s) {
-    const int kx = k0 + 4 * g + s;
-    const bool ok = row < M && kx < M;
-    const cd a = ok ? A[(long)row * lda + kx] : cmk(0.0, 0.0);
-    ar[s] = a.re;
-    ai[s] = a.im;
-  }
-}
-
 // acc(tiles of this wave) = A[rows][0..M) * sB[0..M)[cols]; A in LDS transposed (A_LDS:
 // A[row][k] at A[k * lda + row], zero padded to MB) or in memory (prefetched one k-chunk ahead)
 template <int TPW, bool A_LDS>
@@ -41,6 +28,9 @@ __device__ inline void mm_tiles(int M, int MB, int NB, int ldb, const cd* __rest
   const int lane = threadIdx.x & 63;
   const int g = lane >> 4, r16 = lane & 15;
   const int ntc = NB / 16;
+  // strides of the four k values a lane group reads per chunk (no multiplies in the k loop)
+  const int lb1 = ldb, lb2 = 2 * ldb, lb3 = 3 * ldb;
+  const long la1 = lda, la2 = 2 * lda, la3 = 3 * lda;
 #pragma unroll
   for (int q = 0; q < TPW; ++q) {
     accr[q] = d4{0, 0, 0, 0};
@@ -51,30 +41,34 @@ __device__ inline void mm_tiles(int M, int MB, int NB, int ldb, const cd* __rest
     if (q < ntw) {
       const int ti = tile_ids[q] / ntc, tj = tile_ids[q] % ntc;
       const int row = ti * 16 + r16;
+      // A fragment of chunk k0: A[row][k0 + 4g + s]; in LDS A is transposed (A[k][row])
+      const cd* ap = A_LDS ? A + (long)(4 * g) * lda + row : A + (long)row * lda + 4 * g;
+      const bool arow = row < M;
+      const cd* bp = sB + (4 * g) * ldb + tj * 16 + r16;
       double ar[4], ai[4], nr[4], ni[4];
-      if (A_LDS) {
-        // rows and columns beyond M are zero in LDS
-      } else {
-        load_a(A, lda, M, row, 0, g, ar, ai);
-      }
-      for (int k0 = 0; k0 < MB; k0 += 16) {
-        if (A_LDS) {
-#pragma unroll
-          for (int s = 0; s < 4; ++s) {
-            const cd a = A[(long)(k0 + 4 * g + s) * lda + row];
-            ar[s] = a.re;
-            ai[s] = a.im;
-          }
-        } else if (k0 + 16 < MB) {
-          load_a(A, lda, M, row, k0 + 16, g, nr, ni);
-        }
-        double br[4], bi[4];
+      auto load_mem = [&](const cd* p, int k0, double (&xr)[4], double (&xi)[4]) {
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
-          const cd b = sB[(k0 + 4 * g + s) * ldb + tj * 16 + r16];
-          br[s] = b.re;
-          bi[s] = b.im;
+          const bool ok = arow && k0 + 4 * g + s < M;
+          const cd x = ok ? p[s] : cmk(0.0, 0.0);
+          xr[s] = x.re;
+          xi[s] = x.im;
         }
+      };
+      if (!A_LDS) load_mem(ap, 0, ar, ai);
+      for (int k0 = 0; k0 < MB; k0 += 16) {
+        if (A_LDS) {
+          const cd x0 = ap[0], x1 = ap[la1], x2 = ap[la2], x3 = ap[la3];
+          ar[0] = x0.re; ai[0] = x0.im; ar[1] = x1.re; ai[1] = x1.im;
+          ar[2] = x2.re; ai[2] = x2.im; ar[3] = x3.re; ai[3] = x3.im;
+          ap += 16 * lda;
+        } else if (k0 + 16 < MB) {
+          load_mem(ap + 16, k0 + 16, nr, ni);
+        }
+        const cd y0 = bp[0], y1 = bp[lb1], y2 = bp[lb2], y3 = bp[lb3];
+        bp += 16 * ldb;
+        const double br[4] = {y0.re, y1.re, y2.re, y3.re};
+        const double bi[4] = {y0.im, y1.im, y2.im, y3.im};
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
           accr[q] = mfma16x16x4(ar[s], br[s], accr[q]);
@@ -86,6 +80,7 @@ __device__ inline void mm_tiles(int M, int MB, int NB, int ldb, const cd* __rest
           acci[q] = mfma16x16x4(ai[s], br[s], acci[q]);
         }
         if (!A_LDS) {
+          ap += 16;
 #pragma unroll
           for (int s = 0; s < 4; ++s) {
             ar[s] = nr[s];
@@ -114,8 +109,10 @@ __device__ inline void store_tiles(cd* sB, int NB, int ldb, int ntw, const int* 
   }
 }
 
+// LDS (80 KB per workgroup with V_w staged) allows two workgroups per CU, i.e. two wavefronts
+// per SIMD, so the register budget is that of two wavefronts (no accumulators parked in AGPRs)
 template <int TPW, bool V_LDS>
-__global__ __launch_bounds__(64 * kWaves) void k_propagate(int M, int N, cd* __restrict__ Phi,
+__global__ __launch_bounds__(64 * kWaves, 2) void k_propagate(int M, int N, cd* __restrict__ Phi,
                                                           long ldw, const cd* __restrict__ P1,
                                                           const cd* __restrict__ V, int order) {
   extern __shared__ cd smem[];   // [MB][NB] current term, then (V_LDS) [MB][MB] V_w transposed

@@ -38,10 +38,13 @@ def states(s, W, amp, seed):
 
 
 @pytest.mark.parametrize("form", list(FORMS))
-@pytest.mark.parametrize("shape", [(24, 6, 96), (40, 8, 200), (64, 10, 320), (64, 16, 320)])
+@pytest.mark.parametrize("shape", [(24, 6, 96), (40, 8, 200), (64, 10, 320), (64, 16, 320),
+                                   (40, 20, 120), (64, 32, 96), (128, 24, 60), (128, 32, 40)])
 def test_reduced_form_energy(form, shape):
     prec, bound = FORMS[form]
     M, N, K = shape
+    if form == "bf16x3" and N > 16:
+        pytest.skip("the BF16X3 exchange covers N <= 16")
     s = synth.make_system(M, N, K, seed=M + N)
     W = 48
     ham = A.HamiltonianOperations(s.h, s.Likn, s.PsiT, 0.0, s.dt)
@@ -90,9 +93,9 @@ def test_reduced_form_contractions(form):
     ham.close()
 
 
-@pytest.mark.parametrize("prec", [A.BF16X3, A.INT8_SPLIT])
-def test_split_forms_reject_unsupported_shapes(prec):
-    s = synth.make_system(40, 20, 60, seed=2)      # N > 16
+@pytest.mark.parametrize("prec,M,N", [(A.BF16X3, 40, 20), (A.INT8_SPLIT, 136, 20)])
+def test_split_forms_reject_unsupported_shapes(prec, M, N):
+    s = synth.make_system(M, N, 60, seed=2)        # N beyond 16 (BF16X3), M beyond 128 (int8)
     ham = A.HamiltonianOperations(s.h, s.Likn, s.PsiT, 0.0, s.dt)
     with pytest.raises(A.AfqError):
         ham.set_precision(prec)
