@@ -170,6 +170,8 @@ def main(argv=None):
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--forms", default="fp64,fp32,bf16x3,int8")
+    ap.add_argument("--no-hybrid", action="store_true",
+                    help="skip the hybrid-weight (no per-step local energy) leg")
     ap.add_argument("--no-graph", action="store_true",
                     help="time the eager launches instead of one captured graph of the steps")
     args = ap.parse_args(argv)
@@ -300,6 +302,19 @@ def main(argv=None):
     out["eager_ms_per_step"] = round(head["eager_ms"] / args.steps, 4)
     if head["graph"] is not None:
         out["graph_matches_eager"] = {f: r["graph"] for f, r in forms.items()}
+    if not args.no_hybrid:
+        # the reference's default step: hybrid weights, no local energy per step
+        hyb = {}
+        for f in sorted({"fp64", mixed}):
+            prec = {"fp64": A.FP64, "fp32": A.FP32, "bf16x3": A.BF16X3, "int8": A.INT8_SPLIT}[f]
+            r = run_form(A, torch, s, W, xi_dev, uniforms, args.warmup, args.steps, ortho, prec,
+                         hybrid=True, graph=not args.no_graph)
+            hyb[f] = {"walker_steps_per_s": round(W * args.steps / (r["ms"] / 1e3), 1),
+                      "ms_per_step": round(r["ms"] / args.steps, 4)}
+            r["ham"].close()
+            print(f"[bench] hybrid {f}: {r['ms'] / args.steps:.3f} ms/step", file=sys.stderr,
+                  flush=True)
+        out["hybrid_form"] = hyb
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(s, W, ortho, args.cpu_seconds)
     for f in forms.values():

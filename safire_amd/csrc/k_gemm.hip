@@ -3,10 +3,11 @@
 //   vbias  (Real3IndexFactorization.hpp:350-361): A = G [W][N*M], B = Lnak [K][N*M]  (complex)
 //   vHS    (Real3IndexFactorization.hpp:297-313): A = X [W][K],   B = Likn [M*M][K]   (real)
 //
-// Tiling for CDNA4: a 256-thread workgroup owns a 64x64 tile of C, each 64-wide wavefront a
-// 32x32 sub-tile made of 2x2 MFMA 16x16x4 tiles with separate real/imag accumulators
-// (4 real MFMAs per complex tile and k-step, 2 when B is real). The workgroup stages each
-// 64 x KS slice of A and of B in LDS once (every thread loads 1/256 of it; rows padded to KS + 1
+// Tiling for CDNA4: a 256-thread workgroup owns a 64 x TN tile of C (TN = 64, or 128 for the
+// FP32 form with real B), each 64-wide wavefront a 32 x TN/2 sub-tile made of 2 x TN/32 MFMA
+// 16x16x4 tiles with separate real/imag accumulators (4 real MFMAs per complex tile and k-step,
+// 2 when B is real). The workgroup stages each 64 x KS slice of A and TN x KS slice of B in LDS
+// once (every thread loads 1/256 of it; rows padded to KS + 1
 // elements so that the 16 rows of an MFMA fragment fall into different banks), double-buffered:
 // the next slice's loads are in flight in registers while the current slice is multiplied, with
 // one barrier per slice. Along k, lane group g = lane/16 takes k = k0 + 4g + s in MFMA step s;
@@ -25,7 +26,7 @@ namespace {
 
 constexpr int kTile = 64;
 
-template <typename R, bool BC, int KS>
+template <typename R, bool BC, int KS, int TN>
 __global__ __launch_bounds__(256) void k_gemm_nt(int m, int n, int k, int kchunk, double alpha,
                                                  const double* __restrict__ A, long lda,
                                                  const R* __restrict__ B, long ldb,
@@ -35,21 +36,23 @@ __global__ __launch_bounds__(256) void k_gemm_nt(int m, int n, int k, int kchunk
   constexpr int LDK = KS + 1;          // padded k stride of a staged row (elements)
   constexpr int NB = BC ? 2 : 1;       // R values per element of B
   constexpr int KPT = KS / 4;          // k values each thread stages per row and slice
+  constexpr int NTW = TN / 32;         // 16-wide MFMA column tiles per wavefront
+  constexpr int BROWS = TN / 64;       // B rows each thread stages per slice
   __shared__ R sA[2][kTile * LDK * 2];
-  __shared__ R sB[2][kTile * LDK * NB];
+  __shared__ R sB[2][TN * LDK * NB];
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int g = lane >> 4, r16 = lane & 15;
-  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * (TN / 2);
   const int kbeg = blockIdx.z * kchunk;
   const int kend = min(k, kbeg + kchunk);
   C += blockIdx.z * split_stride;
 
   // staging share of this thread: row srow of the A and B tiles, k = k0 + skq .. + KPT - 1
   const int srow = tid >> 2, skq = (tid & 3) * KPT;
-  const int gm = blockIdx.x * kTile + srow, gn = blockIdx.y * kTile + srow;
-  R pa[2 * KPT], pb[NB * KPT];
+  const int gm = blockIdx.x * kTile + srow;
+  R pa[2 * KPT], pb[BROWS][NB * KPT];
   auto gload = [&](int k0) {
     const int kb = k0 + skq;
 #pragma unroll
@@ -58,9 +61,13 @@ __global__ __launch_bounds__(256) void k_gemm_nt(int m, int n, int k, int kchunk
       const double* ap = A + 2 * ((long)gm * lda + kb + j);
       pa[2 * j] = (gm < m && ok) ? (R)ap[0] : (R)0;
       pa[2 * j + 1] = (gm < m && ok) ? (R)ap[1] : (R)0;
-      const R* bp = B + NB * ((long)gn * ldb + kb + j);
-      pb[NB * j] = (gn < n && ok) ? bp[0] : (R)0;
-      if (BC) pb[NB * j + 1] = (gn < n && ok) ? bp[1] : (R)0;
+#pragma unroll
+      for (int h = 0; h < BROWS; ++h) {
+        const int gn = blockIdx.y * TN + srow + 64 * h;
+        const R* bp = B + NB * ((long)gn * ldb + kb + j);
+        pb[h][NB * j] = (gn < n && ok) ? bp[0] : (R)0;
+        if (BC) pb[h][NB * j + 1] = (gn < n && ok) ? bp[1] : (R)0;
+      }
     }
   };
   auto sstore = [&](int buf) {
@@ -68,16 +75,20 @@ __global__ __launch_bounds__(256) void k_gemm_nt(int m, int n, int k, int kchunk
     for (int j = 0; j < KPT; ++j) {
       sA[buf][2 * (srow * LDK + skq + j)] = pa[2 * j];
       sA[buf][2 * (srow * LDK + skq + j) + 1] = pa[2 * j + 1];
-      sB[buf][NB * (srow * LDK + skq + j)] = pb[NB * j];
-      if (BC) sB[buf][NB * (srow * LDK + skq + j) + 1] = pb[NB * j + 1];
+#pragma unroll
+      for (int h = 0; h < BROWS; ++h) {
+        const int row = srow + 64 * h;
+        sB[buf][NB * (row * LDK + skq + j)] = pb[h][NB * j];
+        if (BC) sB[buf][NB * (row * LDK + skq + j) + 1] = pb[h][NB * j + 1];
+      }
     }
   };
 
-  acc_t cr[2][2], ci[2][2];
+  acc_t cr[2][NTW], ci[2][NTW];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b) {
+    for (int b = 0; b < NTW; ++b) {
       cr[a][b] = acc_t{0, 0, 0, 0};
       ci[a][b] = acc_t{0, 0, 0, 0};
     }
@@ -94,15 +105,19 @@ __global__ __launch_bounds__(256) void k_gemm_nt(int m, int n, int k, int kchunk
     const R* b_s = sB[buf];
 #pragma unroll
     for (int kk = 0; kk < KS; kk += 16) {
-      R ar[2][4], ai[2][4], br[2][4], bi[2][4];
+      R ar[2][4], ai[2][4], br[NTW][4], bi[NTW][4];
 #pragma unroll
-      for (int t = 0; t < 2; ++t) {
+      for (int s = 0; s < 4; ++s) {
+        const int kx = kk + 4 * g + s;
 #pragma unroll
-        for (int s = 0; s < 4; ++s) {
-          const int kx = kk + 4 * g + s;
-          const int ra = wm + t * 16 + r16, rb = wn + t * 16 + r16;
+        for (int t = 0; t < 2; ++t) {
+          const int ra = wm + t * 16 + r16;
           ar[t][s] = a_s[2 * (ra * LDK + kx)];
           ai[t][s] = a_s[2 * (ra * LDK + kx) + 1];
+        }
+#pragma unroll
+        for (int t = 0; t < NTW; ++t) {
+          const int rb = wn + t * 16 + r16;
           br[t][s] = b_s[NB * (rb * LDK + kx)];
           bi[t][s] = BC ? b_s[NB * (rb * LDK + kx) + 1] : (R)0;
         }
@@ -113,7 +128,7 @@ __global__ __launch_bounds__(256) void k_gemm_nt(int m, int n, int k, int kchunk
 #pragma unroll
         for (int a = 0; a < 2; ++a)
 #pragma unroll
-          for (int b = 0; b < 2; ++b) {
+          for (int b = 0; b < NTW; ++b) {
             cr[a][b] = mfma16x16x4(ar[a][s], br[b][s], cr[a][b]);
             ci[a][b] = mfma16x16x4(ai[a][s], br[b][s], ci[a][b]);
           }
@@ -121,7 +136,7 @@ __global__ __launch_bounds__(256) void k_gemm_nt(int m, int n, int k, int kchunk
 #pragma unroll
           for (int a = 0; a < 2; ++a)
 #pragma unroll
-            for (int b = 0; b < 2; ++b) {
+            for (int b = 0; b < NTW; ++b) {
               cr[a][b] = mfma16x16x4(-ai[a][s], bi[b][s], cr[a][b]);
               ci[a][b] = mfma16x16x4(ar[a][s], bi[b][s], ci[a][b]);
             }
@@ -133,11 +148,11 @@ __global__ __launch_bounds__(256) void k_gemm_nt(int m, int n, int k, int kchunk
     __syncthreads();
   }
 
-  const int m0 = blockIdx.x * kTile + wm, n0 = blockIdx.y * kTile + wn;
+  const int m0 = blockIdx.x * kTile + wm, n0 = blockIdx.y * TN + wn;
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b) {
+    for (int b = 0; b < NTW; ++b) {
       const int col = n0 + b * 16 + r16;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
@@ -187,25 +202,28 @@ void launch_gemm_nt_z(hipStream_t s, int precision, int m, int n, int k, double 
   kchunk = (kchunk + 31) & ~31;   // k chunks on whole staged slices (16 or 32 wide)
   splits = (k + kchunk - 1) / kchunk;
   if (splits < 1) splits = 1;
-  dim3 grid((m + kTile - 1) / kTile, (n + kTile - 1) / kTile, splits);
+  // FP32 with real B (vHS) takes 64 x 128 tiles: half the re-reads of A per output column (the
+  // FP64 form of that tile needs more than 128 VGPRs + 128 AGPRs and would run one wave per SIMD)
+  const int tn = (b_complex || precision == AFQ_FP64) ? 64 : 128;
+  dim3 grid((m + kTile - 1) / kTile, (n + tn - 1) / tn, splits);
   double* out = splits > 1 ? split_ws : C;
   const long ldo = splits > 1 ? n : ldc;
   const long sstride = splits > 1 ? 2L * m * n : 0;
   const double a = splits > 1 ? 1.0 : alpha;
   if (precision == AFQ_FP64) {
     if (b_complex)
-      hipLaunchKernelGGL((k_gemm_nt<double, true, 16>), grid, dim3(256), 0, s, m, n, k, kchunk, a, A,
-                         lda, (const double*)B, ldb, out, ldo, sstride);
+      hipLaunchKernelGGL((k_gemm_nt<double, true, 16, 64>), grid, dim3(256), 0, s, m, n, k, kchunk,
+                         a, A, lda, (const double*)B, ldb, out, ldo, sstride);
     else
-      hipLaunchKernelGGL((k_gemm_nt<double, false, 16>), grid, dim3(256), 0, s, m, n, k, kchunk, a,
-                         A, lda, (const double*)B, ldb, out, ldo, sstride);
+      hipLaunchKernelGGL((k_gemm_nt<double, false, 16, 64>), grid, dim3(256), 0, s, m, n, k,
+                         kchunk, a, A, lda, (const double*)B, ldb, out, ldo, sstride);
   } else {
     if (b_complex)
-      hipLaunchKernelGGL((k_gemm_nt<float, true, 32>), grid, dim3(256), 0, s, m, n, k, kchunk, a, A,
-                         lda, (const float*)B, ldb, out, ldo, sstride);
+      hipLaunchKernelGGL((k_gemm_nt<float, true, 32, 64>), grid, dim3(256), 0, s, m, n, k, kchunk,
+                         a, A, lda, (const float*)B, ldb, out, ldo, sstride);
     else
-      hipLaunchKernelGGL((k_gemm_nt<float, false, 32>), grid, dim3(256), 0, s, m, n, k, kchunk, a,
-                         A, lda, (const float*)B, ldb, out, ldo, sstride);
+      hipLaunchKernelGGL((k_gemm_nt<float, false, 32, 128>), grid, dim3(256), 0, s, m, n, k,
+                         kchunk, a, A, lda, (const float*)B, ldb, out, ldo, sstride);
   }
   AFQ_HIP(hipGetLastError());
   if (splits > 1) {

@@ -106,6 +106,38 @@ def test_singular_overlap_gives_zero_G():
     np.testing.assert_allclose(host(ov), ovref, rtol=1e-14)
 
 
+@pytest.mark.parametrize("M,N", [(24, 5), (64, 16), (40, 12)])
+def test_mixed_density_matrix_pivoting(M, N):
+    """Overlap matrices that force row exchanges and ties in the pivot search: permutation
+    matrices (one candidate per column, a swap at most steps) and small-integer matrices, whose
+    |re|+|im| values tie often. Singular integer draws are kept: G is then zero, like getri_or_zero."""
+    rng = np.random.default_rng(7 * M + N)
+    nb = 24
+    PsiT = np.zeros((N, M), np.complex128)
+    PsiT[:, :N] = np.eye(N)
+    Phi = rand_c(rng, nb, M, N)
+    for w in range(nb):
+        if w % 2 == 0:
+            Phi[w, :N] = np.eye(N)[rng.permutation(N)] * (1 + 1j * (w % 4 == 0))
+        else:
+            Phi[w, :N] = rng.integers(-1, 2, (N, N)) + 1j * rng.integers(-1, 2, (N, N))
+    Phi[2, 1] = Phi[2, 0]     # a repeated row: an exactly zero pivot part-way through
+    Gref, ovref = O.mixed_density_matrix(PsiT, Phi)
+    assert not np.any(Gref[2])
+    G = torch.zeros((nb, N, M), dtype=torch.complex128, device="cuda")
+    ov = torch.zeros(nb, dtype=torch.complex128, device="cuda")
+    A.det_ops.MixedDensityMatrix(dev(PsiT), dev(Phi), G, ov)
+    Gh, ovh = host(G), host(ov)
+    for w in range(nb):
+        if not np.any(Gref[w]):
+            assert not np.any(Gh[w]), w
+        else:
+            assert relerr(Gh[w], Gref[w]) < 1e-9, w
+        d = ovh[w] - ovref[w]
+        d = d.real + 1j * ((d.imag + np.pi) % (2 * np.pi) - np.pi)
+        assert abs(d) < 1e-9 * max(1.0, abs(ovref[w])), (w, ovh[w], ovref[w])
+
+
 @pytest.mark.parametrize("M,N,nb", [(8, 3, 2), (24, 6, 17), (64, 16, 5), (128, 32, 3)])
 def test_propagate_parity(M, N, nb):
     rng = np.random.default_rng(7 + M)
