@@ -31,7 +31,9 @@ __global__ __launch_bounds__(64 * WPB) void k_exchange(int W, int M, int N, int 
   constexpr int NB = 16 * NT;
   constexpr int LDB = 17;                 // padded k stride of the staged Lnak slice
   __shared__ R sB[NV * NB * LDB * 2];     // [v][a][k] (re,im)
-  __shared__ double sT[WPB][NB * NB * 2]; // per-wavefront T block [b][a] (re,im)
+  // per-wavefront T block [b][a] (re, im), rows padded by one entry (distinct banks for the
+  // transposed read)
+  __shared__ __attribute__((aligned(16))) double sT[WPB][NB * (NB + 1) * 2];
 
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
@@ -143,8 +145,8 @@ __global__ __launch_bounds__(64 * WPB) void k_exchange(int W, int M, int N, int 
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             const int row = a * 16 + acc_row<R>(g, r), col = b * 16 + r16;
-            T[2 * (row * NB + col)] = (double)tr_[v][a][b][r];
-            T[2 * (row * NB + col) + 1] = (double)ti_[v][a][b][r];
+            *reinterpret_cast<double2*>(T + 2 * (row * (NB + 1) + col)) =
+                double2{(double)tr_[v][a][b][r], (double)ti_[v][a][b][r]};
           }
       __syncthreads();
       const int n = nv0 + v;
@@ -157,14 +159,17 @@ __global__ __launch_bounds__(64 * WPB) void k_exchange(int W, int M, int N, int 
             for (int r = 0; r < 4; ++r) {
               const int row = a * 16 + acc_row<R>(g, r), col = b * 16 + r16;
               const double xr = (double)tr_[v][a][b][r], xi = (double)ti_[v][a][b][r];
-              const double yr = T[2 * (col * NB + row)], yi = T[2 * (col * NB + row) + 1];
+              const double2 y =
+                  *reinterpret_cast<const double2*>(T + 2 * (col * (NB + 1) + row));
+              const double yr = y.x, yi = y.y;
               exx_re += xr * yr - xi * yi;
               exx_im += xr * yi + xi * yr;
             }
         double dr = 0.0, di = 0.0;
         if (lane < N) {
-          dr = T[2 * (lane * NB + lane)];
-          di = T[2 * (lane * NB + lane) + 1];
+          const double2 d = *reinterpret_cast<const double2*>(T + 2 * (lane * (NB + 1) + lane));
+          dr = d.x;
+          di = d.y;
         }
         for (int off = 32; off > 0; off >>= 1) {
           dr += __shfl_xor(dr, off, 64);

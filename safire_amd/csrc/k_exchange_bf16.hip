@@ -50,7 +50,9 @@ __global__ __launch_bounds__(64 * kWPB) void k_exchange_bf16(
     const __bf16* __restrict__ Lb, double* __restrict__ Kl, double* __restrict__ exx_part,
     int nchunk) {
   extern __shared__ __bf16 sB[];                    // [kNV][3][2][16][row]
-  __shared__ double sT[kWPB][16 * 16 * 2];          // per-wavefront T block [b][a] (re,im)
+  // per-wavefront T block [b][a] (re, im), rows padded to 17 entries (distinct banks for the
+  // transposed read)
+  __shared__ __attribute__((aligned(16))) double sT[kWPB][16 * 17 * 2];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int g = lane >> 4, r16 = lane & 15;
@@ -159,8 +161,8 @@ __global__ __launch_bounds__(64 * kWPB) void k_exchange_bf16(
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int rw = acc_row<float>(g, r), cl = r16;
-        T[2 * (rw * 16 + cl)] = (double)tr[v][r];
-        T[2 * (rw * 16 + cl) + 1] = (double)ti[v][r];
+        *reinterpret_cast<double2*>(T + 2 * (rw * 17 + cl)) =
+            double2{(double)tr[v][r], (double)ti[v][r]};
       }
       __syncthreads();
       const int n = nv0 + v;
@@ -169,14 +171,16 @@ __global__ __launch_bounds__(64 * kWPB) void k_exchange_bf16(
         for (int r = 0; r < 4; ++r) {
           const int rw = acc_row<float>(g, r), cl = r16;
           const double xr = (double)tr[v][r], xi = (double)ti[v][r];
-          const double yr = T[2 * (cl * 16 + rw)], yi = T[2 * (cl * 16 + rw) + 1];
+          const double2 y = *reinterpret_cast<const double2*>(T + 2 * (cl * 17 + rw));
+          const double yr = y.x, yi = y.y;
           exx_re += xr * yr - xi * yi;
           exx_im += xr * yi + xi * yr;
         }
         double dr = 0.0, di = 0.0;
         if (lane < N) {
-          dr = T[2 * (lane * 16 + lane)];
-          di = T[2 * (lane * 16 + lane) + 1];
+          const double2 d = *reinterpret_cast<const double2*>(T + 2 * (lane * 17 + lane));
+          dr = d.x;
+          di = d.y;
         }
         for (int off = 32; off > 0; off >>= 1) {
           dr += __shfl_xor(dr, off, 64);

@@ -126,7 +126,9 @@ __global__ __launch_bounds__(64 * kWPB, kWgPerCu * kWPB / 4) void k_exchange_i8(
   // two staging buffers: the next group is loaded while the current one is computed on
   __shared__ __attribute__((aligned(16))) int8_t sBuf[2][kStage * kSL * 2 * 16 * ROWB];
   __shared__ double sUnit[2][kStage][16];           // column units of the staged vectors
-  __shared__ double sT[kWPB][16 * 16 * 2];          // per-wavefront T block [a][b] (re,im)
+  // per-wavefront T block [a][b] (re, im), rows padded to 17 entries: the transposed read
+  // (16 lanes down a column) then falls into distinct banks
+  __shared__ __attribute__((aligned(16))) double sT[kWPB][16 * 17 * 2];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int g = lane >> 4, r16 = lane & 15;
@@ -317,23 +319,24 @@ __global__ __launch_bounds__(64 * kWPB, kWgPerCu * kWPB / 4) void k_exchange_i8(
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int rw = acc_row<float>(g, r);
-          T[2 * (rw * 16 + r16)] = tr[r];
-          T[2 * (rw * 16 + r16) + 1] = ti[r];
+          *reinterpret_cast<double2*>(T + 2 * (rw * 17 + r16)) = double2{tr[r], ti[r]};
         }
         wave_lds_sync();
         if (n < nend) {
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             const int rw = acc_row<float>(g, r);
-            const double yr = T[2 * (r16 * 16 + rw)], yi = T[2 * (r16 * 16 + rw) + 1];
+            const double2 y = *reinterpret_cast<const double2*>(T + 2 * (r16 * 17 + rw));
+            const double yr = y.x, yi = y.y;
             exx_re += tr[r] * yr - ti[r] * yi;
             exx_im += tr[r] * yi + ti[r] * yr;
           }
           // tr(T): the N diagonal entries in lanes 0..15, summed into lane 15 by DPP row shifts
           double dr = 0.0, di = 0.0;
           if (lane < N) {
-            dr = T[2 * (lane * 16 + lane)];
-            di = T[2 * (lane * 16 + lane) + 1];
+            const double2 d = *reinterpret_cast<const double2*>(T + 2 * (lane * 17 + lane));
+            dr = d.x;
+            di = d.y;
           }
           dr = row_sum16(dr);
           di = row_sum16(di);
@@ -375,7 +378,11 @@ __global__ __launch_bounds__(256, 2) void k_exchange_i8_t2(
   constexpr int kRows = 32;
   __shared__ __attribute__((aligned(16))) int8_t sB[kSL * 2 * kRows * ROWB];
   __shared__ double sU[kRows];
-  __shared__ double sT[2][kRows * kRows * 2];      // per walker T [a][b] (re, im)
+  // per walker T [a][b] (re, im). For KC = 1 entry (x, y) sits in column y ^ (x % 16) of row x,
+  // so that 16 lanes reading down a column (the transposed read) hit 16 different banks; at
+  // KC = 2 the kernel has no registers to spare for it (it would spill)
+  __shared__ double sT[2][kRows * kRows * 2];
+  auto tcol = [](int x, int y) { return KC == 1 ? (y ^ (x & 15)) : y; };
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int g = lane >> 4, r16 = lane & 15;
@@ -506,8 +513,8 @@ __global__ __launch_bounds__(256, 2) void k_exchange_i8_t2(
         tr[tb][r] = xr * u;
         ti[tb][r] = xi * u;
         const int a = ta * 16 + 4 * g + r, b = tb * 16 + r16;
-        T[2 * (a * kRows + b)] = tr[tb][r];
-        T[2 * (a * kRows + b) + 1] = ti[tb][r];
+        T[2 * (a * kRows + tcol(a, b))] = tr[tb][r];
+        T[2 * (a * kRows + tcol(a, b)) + 1] = ti[tb][r];
       }
     }
     __syncthreads();   // both row tiles of T are in LDS
@@ -516,7 +523,8 @@ __global__ __launch_bounds__(256, 2) void k_exchange_i8_t2(
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int a = ta * 16 + 4 * g + r, b = tb * 16 + r16;
-        const double yr = T[2 * (b * kRows + a)], yi = T[2 * (b * kRows + a) + 1];
+        const double yr = T[2 * (b * kRows + tcol(b, a))];
+        const double yi = T[2 * (b * kRows + tcol(b, a)) + 1];
         exx_re += tr[tb][r] * yr - ti[tb][r] * yi;
         exx_im += tr[tb][r] * yi + ti[tb][r] * yr;
       }
@@ -524,8 +532,8 @@ __global__ __launch_bounds__(256, 2) void k_exchange_i8_t2(
       // tr(T): diagonal entries a < N in lanes 0..31, summed by rows of 16 and combined
       double dr = 0.0, di = 0.0;
       if (lane < N && lane < kRows) {
-        dr = T[2 * (lane * kRows + lane)];
-        di = T[2 * (lane * kRows + lane) + 1];
+        dr = T[2 * (lane * kRows + tcol(lane, lane))];
+        di = T[2 * (lane * kRows + tcol(lane, lane)) + 1];
       }
       dr = row_sum16(dr);
       di = row_sum16(di);
