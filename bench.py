@@ -39,6 +39,11 @@ PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0, "int8": 5000.0}
 # executed MFMA flops per algorithmic flop of the exchange contraction
 EXECUTED_PER_ALGO = {"fp64": 1.0, "fp32": 1.0, "bf16x3": 6.0, "int8": 15.0}
 GATE_HA = 1e-5
+# HBM bytes per launch of each form's exchange kernel (FETCH_SIZE and WRITE_SIZE counter runs
+# scaled by a copy kernel of known size), as recorded in profiles/walker_step_c2_hbm_traffic.md;
+# null for a workload or form without such a measurement
+TRAFFIC_BYTES = {("c2", "int8"): 38.48e6, ("c2", "fp64"): 53.25e6}
+TRAFFIC_SOURCE = "profiles/walker_step_c2_hbm_traffic.md"
 
 
 def exchange_flops(W, M, N, K):
@@ -273,7 +278,11 @@ def main(argv=None):
                    "form": f"{mixed} Cholesky contractions, fp64 G/LU/QR/propagation/weights",
                    "parallelism": "single GPU"},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak,
-                     "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": None,
+                     "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
+                     "traffic": TRAFFIC_BYTES.get((args.workload, mixed)),
+                     "traffic_unit": "bytes per launch",
+                     "traffic_source": (TRAFFIC_SOURCE if (args.workload, mixed) in TRAFFIC_BYTES
+                                        else None),
                      "kernel": f"k_exchange ({mixed} MFMA)",
                      "flop_per_launch": exchange_flops(W, M, N, K),
                      "mean_launch_ms": round(ex_avg, 5),
