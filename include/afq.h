@@ -194,6 +194,17 @@ int afq_population_control(afq_ham ham, int W, double* d_wbuf, long ldw, double 
  * PSEUDO_ELOC; d_out is c128 [4]. Deterministic order. */
 int afq_weighted_sums(afq_ham ham, int W, const double* d_wbuf, long ldw, double* d_out,
                       void* stream);
+/* The walker sums of one EnergyEstimator::measure (EnergyEstimator.h:51-100) with
+ * weight_statistics (Measurements.hpp:130-149) and the growth / mixed averages
+ * (averagePseudoEnergy, averageEnergy: average_energy.hpp:53-79), in a fixed order:
+ * d_out[0..2] = sum_w E_w,c w_w for c = E1, EXX, EJ (no conjugation), [3] = sum w,
+ * [4] = sum exp(OVLP_w) w_w, [5] = (sum |w|, 0), [6] = (sum |w|^2, 0),
+ * [7] = sum w exp(dt PSEUDO_ELOC), [8] = sum w (E1+EXX+EJ) from the walker slots; c128 [9].
+ * d_E (c128 [W][3]) and d_ov (c128 [W]) replace the walkers' E1/EXX/EJ and OVLP in [0..2] and
+ * [4] when given (the estimator recomputing the energy, EnergyEstimator.h:73-74); NULL takes
+ * the walker slots (walkers carrying their energy). dt is the handle's time step. */
+int afq_estimator_sums(afq_ham ham, int W, const double* d_wbuf, long ldw, const double* d_E,
+                       const double* d_ov, double* d_out, void* stream);
 
 /* Region timers (the AFQMCTimers regions of AFQMCTimer.h:25-63, timed on the GPU): while
  * enabled, afq_step and afq_orthogonalize record a pair of HIP events on their stream around

@@ -285,6 +285,59 @@ def ph_excited_1body(iexcit, refc, S, R, wgt):
     return E
 
 
+# ------------------------------------------------------------------ estimators
+def estimator_measure(weights, E, ovlp, step, dt):
+    """One EnergyEstimator::measure on one rank (EnergyEstimator.h:51-92): the local energies
+    [W, 3] contracted with the weights by gemv(transpose(E), w) (no conjugation), the overlap
+    dot(exp(ovlp), w), each divided by sum w (MeasurementOutput, Measurements.hpp:96-128);
+    ProjectionTime = (step + 1) dt; weight_statistics (Measurements.hpp:130-149)."""
+    w = np.asarray(weights, np.complex128)
+    E = np.asarray(E, np.complex128)
+    den = complex(sum(w[i] for i in range(w.size)))
+    avg = [sum(E[i, c] * w[i] for i in range(w.size)) for c in range(3)]
+    ov = sum(np.exp(ovlp[i]) * w[i] for i in range(w.size))
+    s_abs = sum(abs(w[i]) for i in range(w.size))
+    s_abs2 = sum(abs(w[i]) ** 2 for i in range(w.size))
+    return {"Energy": (avg[0] + avg[1] + avg[2]) / den, "OnebodyEnergy": avg[0] / den,
+            "ExchangeEnergy": avg[1] / den, "CoulombEnergy": avg[2] / den, "Overlap": ov / den,
+            "ProjectionTime": (step + 1) * dt,
+            "EffectiveNumWalkers": (s_abs * s_abs / s_abs2).real, "Phase": den / s_abs,
+            "TotalWeight": den}
+
+
+def average_pseudo_energy(weights, pseudo, dt):
+    """averagePseudoEnergy (average_energy.hpp:53-63): log(sum w exp(dt E) / sum w).real / dt."""
+    num = sum(weights[i] * np.exp(dt * pseudo[i]) for i in range(len(weights)))
+    x = (num / sum(weights)).real
+    # std::log: NaN below 0, -inf at 0
+    return float(np.log(x) / dt) if x > 0 else (-np.inf if x == 0 else np.nan)
+
+
+def average_energy(weights, E1, EXX, EJ):
+    """averageEnergy (average_energy.hpp:67-79): sum w (E1 + EXX + EJ) / sum w, real part."""
+    num = sum(weights[i] * (E1[i] + EXX[i] + EJ[i]) for i in range(len(weights)))
+    return float((num / sum(weights)).real)
+
+
+def accumulator_bins(samples, binsize):
+    """Accumulator::add (Accumulator.hpp:86-101): each bin is the running sum of binsize samples
+    divided by binsize; only complete bins are returned (collect_full_bins, 12-33)."""
+    bins, cur, fill = [], None, 0
+    for x in samples:
+        x = np.asarray(x)
+        cur = x * 0 + x if cur is None else cur + x
+        fill += 1
+        if fill >= binsize:
+            bins.append(cur / binsize)
+            cur, fill = x * 0, 0
+    return bins
+
+
+def eshift_relaxation_factor(equilibration_steps):
+    """parameter_defaults.cpp:310-318."""
+    return 1.0 - np.exp(-10.0 / equilibration_steps) if equilibration_steps > 1 else 1.0
+
+
 # ------------------------------------------------------------------ whole system
 class System:
     """Closed-shell NOMSD (one determinant) + Real3Index Hamiltonian + base propagator."""

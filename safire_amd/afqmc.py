@@ -70,6 +70,7 @@ SIGNATURES = {
     "afq_orthogonalize": (_i, [_vp, _i, _vp, _l, _vp]),
     "afq_seed_properties": (_i, [_vp, _i, _vp, _l, _vp]),
     "afq_weighted_sums": (_i, [_vp, _i, _vp, _l, _vp, _vp]),
+    "afq_estimator_sums": (_i, [_vp, _i, _vp, _l, _vp, _vp, _vp, _vp]),
     "afq_ham_enable_timers": (_i, [_vp, _i]),
     "afq_ham_read_timers": (_i, [_vp, _vp, _vp]),
     "afq_timer_name": (C.c_char_p, [_i]),
@@ -326,4 +327,14 @@ class Propagator:
         out = torch.zeros(4, dtype=torch.complex128, device="cuda")
         check(lib().afq_weighted_sums(self.ham.handle, wset.W, ptr(wset.buf), wset.ldw, ptr(out),
                                       stream_ptr(stream)))
+        return out
+
+    def estimator_sums(self, wset: WalkerSet, E=None, ov=None, stream=None):
+        """c128 [9] on the device: sum E1 w, sum EXX w, sum EJ w, sum w, sum exp(OVLP) w, sum |w|,
+        sum |w|^2, sum w exp(dt PSEUDO_ELOC), sum w (E1+EXX+EJ); E [W,3] / ov [W] replace the
+        walkers' energy and overlap slots in the first five when given."""
+        torch = _torch()
+        out = torch.zeros(9, dtype=torch.complex128, device="cuda")
+        check(lib().afq_estimator_sums(self.ham.handle, wset.W, ptr(wset.buf), wset.ldw, ptr(E),
+                                       ptr(ov), ptr(out), stream_ptr(stream)))
         return out

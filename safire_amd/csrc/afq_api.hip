@@ -585,6 +585,15 @@ int afq_weighted_sums(afq_ham H, int W, const double* wbuf, long ldw, double* ou
   });
 }
 
+int afq_estimator_sums(afq_ham H, int W, const double* wbuf, long ldw, const double* E,
+                       const double* ov, double* out, void* stream) {
+  return guard([&] {
+    check_ham(H);
+    check_walkers(H, W, ldw);
+    launch_estimator_sums(S(stream), W, wbuf, ldw, (long)H->M * H->N, H->dt, E, ov, out);
+  });
+}
+
 int afq_ham_enable_timers(afq_ham H, int max_records) {
   return guard([&] {
     check_ham(H);

@@ -240,6 +240,8 @@ void launch_set_properties(hipStream_t s, int W, double* wbuf, long ldw, long po
                            const double* ov, const double* E, int seed_pseudo);
 void launch_weighted_sums(hipStream_t s, int W, const double* wbuf, long ldw, long poff,
                           double dt, double* out);
+void launch_estimator_sums(hipStream_t s, int W, const double* wbuf, long ldw, long poff,
+                           double dt, const double* E, const double* ov, double* out);
 void launch_probe_mfma(hipStream_t s, int kind, double* out);
 void launch_population_control(hipStream_t s, int W, double* wbuf, long ldw, long poff,
                                double r, double* cum, int* off, int* src, int* ok,

@@ -31,7 +31,7 @@ __global__ __launch_bounds__(256) void k_gemm_nt(int m, int n, int k, int kchunk
                                                  const double* __restrict__ A, long lda,
                                                  const R* __restrict__ B, long ldb,
                                                  double* __restrict__ C, long ldc,
-                                                 long split_stride) {
+                                                 long split_stride, int nmajor) {
   typedef typename acc_of<R>::type acc_t;
   constexpr int LDK = KS + 1;          // padded k stride of a staged row (elements)
   constexpr int NB = BC ? 2 : 1;       // R values per element of B
@@ -45,13 +45,17 @@ __global__ __launch_bounds__(256) void k_gemm_nt(int m, int n, int k, int kchunk
   const int wave = tid >> 6;
   const int g = lane >> 4, r16 = lane & 15;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * (TN / 2);
+  // nmajor: the column tile is the fastest grid index, so consecutive workgroups (which the
+  // dispatcher deals round-robin to the 8 XCDs) share A tiles and each XCD's L2 holds 1/8 of B
+  const int tile_m = nmajor ? blockIdx.y : blockIdx.x;
+  const int tile_n = nmajor ? blockIdx.x : blockIdx.y;
   const int kbeg = blockIdx.z * kchunk;
   const int kend = min(k, kbeg + kchunk);
   C += blockIdx.z * split_stride;
 
   // staging share of this thread: row srow of the A and B tiles, k = k0 + skq .. + KPT - 1
   const int srow = tid >> 2, skq = (tid & 3) * KPT;
-  const int gm = blockIdx.x * kTile + srow;
+  const int gm = tile_m * kTile + srow;
   R pa[2 * KPT], pb[BROWS][NB * KPT];
   auto gload = [&](int k0) {
     const int kb = k0 + skq;
@@ -63,7 +67,7 @@ __global__ __launch_bounds__(256) void k_gemm_nt(int m, int n, int k, int kchunk
       pa[2 * j + 1] = (gm < m && ok) ? (R)ap[1] : (R)0;
 #pragma unroll
       for (int h = 0; h < BROWS; ++h) {
-        const int gn = blockIdx.y * TN + srow + 64 * h;
+        const int gn = tile_n * TN + srow + 64 * h;
         const R* bp = B + NB * ((long)gn * ldb + kb + j);
         pb[h][NB * j] = (gn < n && ok) ? bp[0] : (R)0;
         if (BC) pb[h][NB * j + 1] = (gn < n && ok) ? bp[1] : (R)0;
@@ -148,7 +152,7 @@ __global__ __launch_bounds__(256) void k_gemm_nt(int m, int n, int k, int kchunk
     __syncthreads();
   }
 
-  const int m0 = blockIdx.x * kTile + wm, n0 = blockIdx.y * TN + wn;
+  const int m0 = tile_m * kTile + wm, n0 = tile_n * TN + wn;
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -205,7 +209,10 @@ void launch_gemm_nt_z(hipStream_t s, int precision, int m, int n, int k, double 
   // FP32 with real B (vHS) takes 64 x 128 tiles: half the re-reads of A per output column (the
   // FP64 form of that tile needs more than 128 VGPRs + 128 AGPRs and would run one wave per SIMD)
   const int tn = (b_complex || precision == AFQ_FP64) ? 64 : 128;
-  dim3 grid((m + kTile - 1) / kTile, (n + tn - 1) / tn, splits);
+  // real B (vHS, the large Cholesky tensor): column tiles fastest (XCD-local B)
+  const int nmajor = b_complex ? 0 : 1;
+  const unsigned mt = (m + kTile - 1) / kTile, nt = (n + tn - 1) / tn;
+  dim3 grid(nmajor ? nt : mt, nmajor ? mt : nt, splits);
   double* out = splits > 1 ? split_ws : C;
   const long ldo = splits > 1 ? n : ldc;
   const long sstride = splits > 1 ? 2L * m * n : 0;
@@ -213,17 +220,17 @@ void launch_gemm_nt_z(hipStream_t s, int precision, int m, int n, int k, double 
   if (precision == AFQ_FP64) {
     if (b_complex)
       hipLaunchKernelGGL((k_gemm_nt<double, true, 16, 64>), grid, dim3(256), 0, s, m, n, k, kchunk,
-                         a, A, lda, (const double*)B, ldb, out, ldo, sstride);
+                         a, A, lda, (const double*)B, ldb, out, ldo, sstride, nmajor);
     else
       hipLaunchKernelGGL((k_gemm_nt<double, false, 16, 64>), grid, dim3(256), 0, s, m, n, k,
-                         kchunk, a, A, lda, (const double*)B, ldb, out, ldo, sstride);
+                         kchunk, a, A, lda, (const double*)B, ldb, out, ldo, sstride, nmajor);
   } else {
     if (b_complex)
       hipLaunchKernelGGL((k_gemm_nt<float, true, 32, 64>), grid, dim3(256), 0, s, m, n, k, kchunk,
-                         a, A, lda, (const float*)B, ldb, out, ldo, sstride);
+                         a, A, lda, (const float*)B, ldb, out, ldo, sstride, nmajor);
     else
       hipLaunchKernelGGL((k_gemm_nt<float, false, 32, 128>), grid, dim3(256), 0, s, m, n, k,
-                         kchunk, a, A, lda, (const float*)B, ldb, out, ldo, sstride);
+                         kchunk, a, A, lda, (const float*)B, ldb, out, ldo, sstride, nmajor);
   }
   AFQ_HIP(hipGetLastError());
   if (splits > 1) {

@@ -5,7 +5,8 @@
 //   k_walker_update  walker_update + bound_local_energy + bound_walker_weights
 //                    (WalkerSetUpdate.hpp:46-218); one thread per walker
 //   k_set_properties OVLP/E1/EXX/EJ/PSEUDO_ELOC seeding (NOMSD.cpp:69-80, execute.cpp:172-184)
-//   k_weighted_sums  the sums of EnergyEstimator::measure and averagePseudoEnergy
+//   k_weighted_sums  the sums of averagePseudoEnergy and averageEnergy
+//   k_estimator_sums the sums of EnergyEstimator::measure and weight_statistics
 #include "afq_internal.hpp"
 
 #include <cfloat>
@@ -172,6 +173,60 @@ __global__ __launch_bounds__(256) void k_weighted_sums(int W, const cd* __restri
     for (int q = 0; q < 4; ++q) red[wave][q] = s[q];
   __syncthreads();
   if (tid < 4) {
+    cd t = cmk(0, 0);
+    for (int qw = 0; qw < 4; ++qw) t += red[qw][tid];
+    out[tid] = t;
+  }
+}
+
+// The walker sums of one estimator measurement, in a fixed order (one workgroup):
+//   EnergyEstimator::measure (EnergyEstimator.h:51-100): sum_w E_w,c w_w for c = E1, EXX, EJ
+//     (gemv of the transposed energies with the weights, no conjugation), sum_w w_w, and the
+//     overlap sum_w exp(OVLP_w) w_w; E and OVLP come from the walker slots (walkers carrying
+//     their energy) or from E [W][3] / ov [W] when given;
+//   weight_statistics (Measurements.hpp:130-149): sum |w|, sum |w|^2;
+//   averagePseudoEnergy / averageEnergy (average_energy.hpp:53-79): sum w exp(dt PSEUDO_ELOC),
+//     sum w (E1 + EXX + EJ) from the walker slots.
+__global__ __launch_bounds__(256) void k_estimator_sums(int W, const cd* __restrict__ wbuf,
+                                                        long ldw, long poff, double dt,
+                                                        const cd* __restrict__ E,
+                                                        const cd* __restrict__ ov,
+                                                        cd* __restrict__ out) {
+  constexpr int NS = 9;
+  __shared__ cd red[4][NS];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  cd s[NS];
+#pragma unroll
+  for (int q = 0; q < NS; ++q) s[q] = cmk(0, 0);
+  for (int i = tid; i < W; i += 256) {
+    const cd* p = wbuf + (long)i * ldw + poff;
+    const cd w = p[AFQ_P_WEIGHT];
+    const cd e1 = E ? E[3L * i] : p[AFQ_P_E1];
+    const cd exx = E ? E[3L * i + 1] : p[AFQ_P_EXX];
+    const cd ej = E ? E[3L * i + 2] : p[AFQ_P_EJ];
+    const cd o = ov ? ov[i] : p[AFQ_P_OVLP];
+    const double aw = cabs_(w);
+    s[0] += e1 * w;
+    s[1] += exx * w;
+    s[2] += ej * w;
+    s[3] += w;
+    s[4] += cexp_(o) * w;
+    s[5].re += aw;
+    s[6].re += aw * aw;
+    s[7] += w * cexp_(dt * p[AFQ_P_PSEUDO_ELOC]);
+    s[8] += w * (p[AFQ_P_E1] + p[AFQ_P_EXX] + p[AFQ_P_EJ]);
+  }
+#pragma unroll
+  for (int q = 0; q < NS; ++q)
+    for (int off = 32; off > 0; off >>= 1) {
+      s[q].re += __shfl_xor(s[q].re, off, 64);
+      s[q].im += __shfl_xor(s[q].im, off, 64);
+    }
+  if (lane == 0)
+#pragma unroll
+    for (int q = 0; q < NS; ++q) red[wave][q] = s[q];
+  __syncthreads();
+  if (tid < NS) {
     cd t = cmk(0, 0);
     for (int qw = 0; qw < 4; ++qw) t += red[qw][tid];
     out[tid] = t;
@@ -412,6 +467,15 @@ void launch_weighted_sums(hipStream_t s, int W, const double* wbuf, long ldw, lo
                           double dt, double* out) {
   hipLaunchKernelGGL(k_weighted_sums, dim3(1), dim3(256), 0, s, W,
                      reinterpret_cast<const cd*>(wbuf), ldw, poff, dt,
+                     reinterpret_cast<cd*>(out));
+  AFQ_HIP(hipGetLastError());
+}
+
+void launch_estimator_sums(hipStream_t s, int W, const double* wbuf, long ldw, long poff,
+                           double dt, const double* E, const double* ov, double* out) {
+  hipLaunchKernelGGL(k_estimator_sums, dim3(1), dim3(256), 0, s, W,
+                     reinterpret_cast<const cd*>(wbuf), ldw, poff, dt,
+                     reinterpret_cast<const cd*>(E), reinterpret_cast<const cd*>(ov),
                      reinterpret_cast<cd*>(out));
   AFQ_HIP(hipGetLastError());
 }
