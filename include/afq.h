@@ -27,6 +27,7 @@ extern "C" {
 #endif
 
 typedef struct afq_ham_s* afq_ham;
+typedef struct afq_rng_t* afq_rng;
 
 enum afq_status {
   AFQ_OK = 0,
@@ -237,6 +238,27 @@ int afq_ph_excited_2body_energy(int nw, int ndet, int nex, int nact, int nelec, 
 int afq_ph_excited_1body_energy(int nw, int ndet, int nex, int nact, int nelec,
                                 const int* d_iexcit, const int* d_refc, const double* d_S,
                                 const double* d_R, const double* d_wgt, double* d_E, void* stream);
+
+/* ---------------------------------------------------------------------------------------------
+ * Random numbers in step with the reference's host generators (field-for-field parity mode).
+ * ------------------------------------------------------------------------------------------- */
+/* split_seed (utilities/Random.cpp:7-20): splitmix64 finalizer of
+ * seed << 32 | (stream & 255) << 24 | rank. The reference seeds the walker generator with
+ * stream 0 and the field generator with stream 1 (execute.cpp:124-125). */
+unsigned long long afq_split_seed(int seed, int rank, unsigned stream);
+/* HostRandomGenerator (utilities/Random.hpp:36-49): std::mt19937 seeded with seed (kept modulo
+ * 2^32, as the engine does). Host memory; destroy with afq_rng_destroy. */
+int afq_rng_create(unsigned long long seed, afq_rng* out);
+void afq_rng_destroy(afq_rng rng);
+/* n raw 32-bit outputs of the engine into host memory. */
+int afq_rng_raw(afq_rng rng, long n, unsigned* out);
+/* sampleUniformFields (utilities/Random.hpp:42-48): n draws of
+ * std::uniform_real_distribution<double>(0, 1) (libstdc++: two engine outputs per draw) into
+ * host memory, in order. */
+int afq_rng_uniform(afq_rng rng, long n, double* out);
+/* probit (AFQMC/Utilities/probit.h:57-135) of n uniforms on the device: d_x[i] = probit(d_u[i]),
+ * the N(0,1) fields of construct_X (construct_X_kernel.hpp:71). d_x may alias d_u. */
+int afq_probit(long n, const double* d_u, double* d_x, void* stream);
 
 /* Lane layout of the MFMA builtins the kernels use, for a self-check; d_out [64][4] f64
  * receives accumulator element r of every lane.

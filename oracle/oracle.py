@@ -285,6 +285,61 @@ def ph_excited_1body(iexcit, refc, S, R, wgt):
     return E
 
 
+# ------------------------------------------------------------------ random numbers
+_ACKLAM_A = (-3.969683028665376e+01, 2.209460984245205e+02, -2.759285104469687e+02,
+             1.383577518672690e+02, -3.066479806614716e+01, 2.506628277459239e+00)
+_ACKLAM_B = (-5.447609879822406e+01, 1.615858368580409e+02, -1.556989798598866e+02,
+             6.680131188771972e+01, -1.328068155288572e+01)
+_ACKLAM_C = (-7.784894002430293e-03, -3.223964580411365e-01, -2.400758277161838e+00,
+             -2.549732539343734e+00, 4.374664141464968e+00, 2.938163982698783e+00)
+_ACKLAM_D = (7.784695709041462e-03, 3.224671290700398e-01, 2.445134137142996e+00,
+             3.754408661907416e+00)
+
+
+def probit(p):
+    """Acklam's inverse normal CDF as probit.h:57-135 evaluates it (P. J. Acklam 2002): central
+    rational function for 0.02425 <= p <= 0.97575, tail functions outside, 0 outside [0, 1]."""
+    p = np.asarray(p, np.float64)
+    a, b, c, d = _ACKLAM_A, _ACKLAM_B, _ACKLAM_C, _ACKLAM_D
+    out = np.zeros_like(p)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        lo = (p > 0) & (p < 0.02425)
+        q = np.sqrt(-2 * np.log(p[lo]))
+        out[lo] = ((((((c[0] * q + c[1]) * q + c[2]) * q + c[3]) * q + c[4]) * q + c[5]) /
+                   ((((d[0] * q + d[1]) * q + d[2]) * q + d[3]) * q + 1))
+        hi = (p > 0.97575) & (p < 1)
+        q = np.sqrt(-2 * np.log(1 - p[hi]))
+        out[hi] = -((((((c[0] * q + c[1]) * q + c[2]) * q + c[3]) * q + c[4]) * q + c[5]) /
+                    ((((d[0] * q + d[1]) * q + d[2]) * q + d[3]) * q + 1))
+        mid = (p >= 0.02425) & (p <= 0.97575)
+        q = p[mid] - 0.5
+        r = q * q
+        out[mid] = ((((((a[0] * r + a[1]) * r + a[2]) * r + a[3]) * r + a[4]) * r + a[5]) * q /
+                    (((((b[0] * r + b[1]) * r + b[2]) * r + b[3]) * r + b[4]) * r + 1))
+    out[p == 0] = -np.inf
+    out[p == 1] = np.inf
+    return out
+
+
+def canonical_uniform(raw32):
+    """std::uniform_real_distribution<double>(0, 1) over std::mt19937 as libstdc++ computes it
+    (generate_canonical<double, 53>): outputs x1, x2 in turn give (x1 + x2 2^32) / 2^64 in double
+    arithmetic, the largest double below 1 if that rounds to 1."""
+    x = np.asarray(raw32, np.float64).reshape(-1, 2)
+    u = (x[:, 0] + x[:, 1] * 4294967296.0) / 18446744073709551616.0
+    return np.where(u >= 1.0, np.nextafter(1.0, 0.0), u)
+
+
+def split_seed(seed, rank, stream):
+    """utilities/Random.cpp:7-20: splitmix64 finalizer of seed << 32 | stream << 24 | rank."""
+    m = (1 << 64) - 1
+    x = ((seed & 0xffffffff) << 32 | (stream & 0xff) << 24 | rank) & m
+    x = (x + 0x9e3779b97f4a7c15) & m
+    x = ((x ^ (x >> 30)) * 0xbf58476d1ce4e5b9) & m
+    x = ((x ^ (x >> 27)) * 0x94d049bb133111eb) & m
+    return x ^ (x >> 31)
+
+
 # ------------------------------------------------------------------ estimators
 def estimator_measure(weights, E, ovlp, step, dt):
     """One EnergyEstimator::measure on one rank (EnergyEstimator.h:51-92): the local energies

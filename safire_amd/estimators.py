@@ -180,18 +180,21 @@ def run_afqmc(prop: "A.Propagator", wset: "A.WalkerSet", fields, uniforms, steps
               Eshift: float, estimators=(), meas: Measurements | None = None,
               walker_ortho_interval: int = 5, population_control_interval: int = 10,
               equilibration_steps: int = 0, relaxation_factor: float | None = None):
-    """The loop of run_afqmc (run_afqmc.cpp:69-91) on one GPU. fields[t] is step t's [W, K]
-    field array on the device, uniforms[t] the comb's uniform for step t. Returns
-    (Eshift, growth energies per step)."""
+    """The loop of run_afqmc (run_afqmc.cpp:69-91) on one GPU. fields[t] (or fields(t) when
+    fields is callable) is step t's [W, K] field array on the device, uniforms[t] (or
+    uniforms(t)) the comb's uniform for step t. Returns (Eshift, growth energies per step)."""
     f = eshift_relaxation_factor(equilibration_steps) if relaxation_factor is None else relaxation_factor
     dt = prop.ham.dt
     growth = []
     for step in range(steps):
-        prop.Propagate(wset, fields[step], Eshift)
+        prop.Propagate(wset, fields(step) if callable(fields) else fields[step], Eshift)
         if step % walker_ortho_interval == 0:
             prop.Orthogonalize(wset)
         if step % population_control_interval == 0:
-            prop.population_control(wset, float(uniforms[step]))
+            # a callable draws its uniform only here, one per population control, as the
+            # reference's walker generator does
+            prop.population_control(wset, float(uniforms(step) if callable(uniforms)
+                                                 else uniforms[step]))
         sums = prop.estimator_sums(wset).cpu().numpy()
         e = average_pseudo_energy(sums, dt)
         if step >= equilibration_steps:
