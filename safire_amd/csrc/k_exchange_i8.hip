@@ -207,18 +207,47 @@ __global__ __launch_bounds__(64 * kWPB, kWgPerCu * kWPB / 4) void k_exchange_i8(
   const bool pos_ok = pb < N && 16 * pkv < Mp;
   const long gpos = (long)pb * Mp + 16 * pkv;
   const int lpos = pb * ROWB + 16 * pkv;
+  // per slot i: combo = v * 2 kSL + tp (wave-uniform) and the uniform part of its address,
+  // tp (K N Mp) + v (N Mp); a group only adds (nv0 N Mp) and the thread's gpos. KC = 1 keeps
+  // them in registers across groups; KC = 2 has no registers to spare and recomputes them.
+  const long NMp = (long)N * Mp;
+  int combo_k[kPre], vsl_k[kPre], ubase_k[kPre];   // offsets fit in 32 bits (checked at launch)
+  auto slot = [&](int i, int& combo, int& v, int& ubase) {
+    if constexpr (KC == 1) {
+      combo = combo_k[i];
+      v = vsl_k[i];
+      ubase = ubase_k[i];
+    } else {
+      combo = __builtin_amdgcn_readfirstlane((i * 64 * kWPB + (int)threadIdx.x) / (16 * nvec));
+      v = combo / (kSL * 2);
+      ubase = (int)((combo % (kSL * 2)) * KNMp) + v * (int)NMp;
+    }
+  };
+  if constexpr (KC == 1) {
+#pragma unroll
+    for (int i = 0; i < kPre; ++i) {
+      combo_k[i] =
+          __builtin_amdgcn_readfirstlane((i * 64 * kWPB + (int)threadIdx.x) / (16 * nvec));
+      vsl_k[i] = combo_k[i] / (kSL * 2);
+      ubase_k[i] = (int)((combo_k[i] % (kSL * 2)) * KNMp) + vsl_k[i] * (int)NMp;
+    }
+  }
   uint4 pre[kPre];
   double preU = 0.0;
   auto load_group = [&](int nv0) {
+    const int8_t* Lg = Lq + (long)nv0 * NMp + gpos;
 #pragma unroll
     for (int i = 0; i < kPre; ++i) {
-      const int combo =
-          __builtin_amdgcn_readfirstlane((i * 64 * kWPB + (int)threadIdx.x) / (16 * nvec));
-      const int tp = combo % (kSL * 2), v = combo / (kSL * 2);
-      const int n = nv0 + v;
+      int combo, v, ubase;
+      slot(i, combo, v, ubase);
       pre[i] = uint4{0u, 0u, 0u, 0u};
-      if (combo < kStage * kSL * 2 && n < nend && pos_ok)
-        pre[i] = *reinterpret_cast<const uint4*>(Lq + tp * KNMp + (long)n * N * Mp + gpos);
+      if (combo < kStage * kSL * 2 && nv0 + v < nend && pos_ok) {
+        if constexpr (KC == 1)
+          pre[i] = *reinterpret_cast<const uint4*>(Lg + ubase);
+        else
+          pre[i] = *reinterpret_cast<const uint4*>(Lq + (combo % (kSL * 2)) * KNMp +
+                                                    (long)(nv0 + v) * NMp + gpos);
+      }
     }
     if (threadIdx.x < kStage * 16) {
       const int v = threadIdx.x / 16, b = threadIdx.x % 16, n = nv0 + v;
@@ -228,9 +257,9 @@ __global__ __launch_bounds__(64 * kWPB, kWgPerCu * kWPB / 4) void k_exchange_i8(
   auto store_group = [&](int buf) {
 #pragma unroll
     for (int i = 0; i < kPre; ++i) {
-      const int combo =
-          __builtin_amdgcn_readfirstlane((i * 64 * kWPB + (int)threadIdx.x) / (16 * nvec));
-      if (combo < kStage * kSL * 2)   // combo = v * 2 kSL + tp: rows of slice-part combo
+      int combo, v, ubase;
+      slot(i, combo, v, ubase);
+      if (combo < kStage * kSL * 2)   // rows of slice-part combo
         *reinterpret_cast<uint4*>(sBuf[buf] + combo * 16 * ROWB + lpos) = pre[i];
     }
     if (threadIdx.x < kStage * 16) sUnit[buf][threadIdx.x / 16][threadIdx.x % 16] = preU;
@@ -637,6 +666,8 @@ void launch_exchange_i8(hipStream_t s, int W, int M, int N, int K, const double*
   if (W <= 0 || K <= 0) return;
   AFQ_CHECK(exchange_i8_supported(M, N), AFQ_EUNSUPPORTED,
             "int8 exchange: needs N <= 32 and M <= 128");
+  AFQ_CHECK((long)kSL * 2 * K * N * exchange_i8_row(M) < (1L << 31), AFQ_EUNSUPPORTED,
+            "int8 exchange: the Cholesky slices exceed 2 GiB");
   const int chunk = exchange_i8_chunk(W, N, K);
   const int nchunk = (K + chunk - 1) / chunk;
   const int Mp = exchange_i8_row(M);
