@@ -391,32 +391,51 @@ __global__ __launch_bounds__(64 * kWPB, kWgPerCu * kWPB / 4) void k_exchange_i8(
   }
 }
 
-// N in 17..32: T_wn is 2 x 2 tiles of 16 x 16. A 4-wavefront workgroup holds 2 walkers; the
+// N in 17..32: T_wn is 2 x 2 tiles of 16 x 16. An 8-wavefront workgroup holds 4 walkers; the
 // two wavefronts of a walker own its row tiles ta = 0, 1 (G rows 16 ta .. 16 ta + 15 sliced into
 // registers) and compute both column tiles of their rows. Each vector's T goes to a per-walker
 // 32 x 32 LDS block; after a barrier every wavefront takes tr(T^2) over its rows and wavefront 0
 // of the walker takes tr(T) from the diagonal. The two row tiles write separate tr(T^2) partials
-// (slots 2c and 2c + 1 of the walker). One vector is staged at a time (32 rows per slice-part);
-// LDS is ~79 KB at M = 128, so two workgroups share a CU.
+// (slots 2c and 2c + 1 of the walker).
+//
+// Staging: the int8 slices of one vector (10 slice-parts x 32 rows x 64 KC bytes, 20 or 40 KB)
+// are copied from global memory straight into LDS (global_load_lds_dwordx4, no registers), double
+// buffered, so the next vector's copy runs under this vector's MFMAs and epilogue. Rows are not
+// padded; the 16-byte chunk kv of row b sits at position kv ^ f(b) of its row (XOR swizzle), which
+// spreads the 16 rows an MFMA fragment reads over distinct banks. Rows b >= N and chunks past the
+// end of a global row copy clamped (in-bounds) data: their column unit is 0 and the matching G
+// entries are 0, so they contribute nothing. LDS: 2 x 40 KB + 4 x 16 KB at M = 128 (one
+// workgroup, two wavefronts per SIMD).
+constexpr int kT2Walkers = 4;
+
 template <int KC>
-__global__ __launch_bounds__(256, 2) void k_exchange_i8_t2(
+__device__ inline int t2_swz(int b) {
+  return KC == 2 ? ((b >> 1) & 7) : ((b >> 2) & 3);
+}
+
+template <int KC>
+__global__ __launch_bounds__(128 * kT2Walkers, 2) void k_exchange_i8_t2(
     int W, int M, int N, int K, int Mp, int kChunk, const double* __restrict__ G,
     const int8_t* __restrict__ Lq, const double* __restrict__ Lu, double* __restrict__ Kl,
     double* __restrict__ exx_part, int nslots) {
-  constexpr int ROWB = 64 * KC + 16;
   constexpr int kRows = 32;
-  __shared__ __attribute__((aligned(16))) int8_t sB[kSL * 2 * kRows * ROWB];
-  __shared__ double sU[kRows];
-  // per walker T [a][b] (re, im). For KC = 1 entry (x, y) sits in column y ^ (x % 16) of row x,
-  // so that 16 lanes reading down a column (the transposed read) hit 16 different banks; at
-  // KC = 2 the kernel has no registers to spare for it (it would spill)
-  __shared__ double sT[2][kRows * kRows * 2];
-  auto tcol = [](int x, int y) { return KC == 1 ? (y ^ (x & 15)) : y; };
+  constexpr int ROWB = 64 * KC;                  // staged row (bytes), unpadded
+  constexpr int kPlane = kRows * ROWB;           // one slice-part of one vector
+  constexpr int kVecBytes = kSL * 2 * kPlane;
+  constexpr int kWindows = kVecBytes / 1024;     // 1 KB copied per wavefront instruction
+  constexpr int kWaves = 2 * kT2Walkers;
+  static_assert(kVecBytes % 1024 == 0, "a vector's slices are whole 1 KB windows");
+  __shared__ __attribute__((aligned(16))) int8_t sB[2][kVecBytes];
+  __shared__ double sU[2][kRows];
+  // per walker T [a][b] (re, im); entry (x, y) sits in column y ^ (x % 16) of row x, so that 16
+  // lanes reading down a column (the transposed read) hit 16 different banks
+  __shared__ double sT[kT2Walkers][kRows * kRows * 2];
+  auto tcol = [](int x, int y) { return y ^ (x & 15); };
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int g = lane >> 4, r16 = lane & 15;
   const int wl = wave >> 1, ta = wave & 1;
-  const int w_raw = blockIdx.x * 2 + wl;
+  const int w_raw = blockIdx.x * kT2Walkers + wl;
   const bool active = w_raw < W;
   const int w = active ? w_raw : W - 1;
   const int c = blockIdx.y;
@@ -477,85 +496,106 @@ __global__ __launch_bounds__(256, 2) void k_exchange_i8_t2(
 
   double exx_re = 0.0, exx_im = 0.0;
   double* T = sT[wl];
-  const int nvec = 4 * KC;
 
-  for (int n = nbeg; n < nend; ++n) {
-    __syncthreads();   // the previous vector's LDS reads are done
-    for (int e = threadIdx.x; e < kSL * 2 * kRows * nvec; e += 256) {
-      const int kv = e % nvec;
-      const int b = (e / nvec) % kRows;
-      const int tp = e / (nvec * kRows);
-      uint4 val = {0u, 0u, 0u, 0u};
-      if (b < N && 16 * kv < Mp)
-        val = *reinterpret_cast<const uint4*>(Lq + tp * KNMp + ((long)n * N + b) * Mp + 16 * kv);
-      *reinterpret_cast<uint4*>(sB + (tp * kRows + b) * ROWB + 16 * kv) = val;
+  // copy vector n's slices into sB[buf]: window win of the vector is LDS bytes
+  // [1024 win, 1024 win + 1024), lane l writes bytes 16 l .. 16 l + 15 of it
+  auto stage = [&](int n, int buf) {
+    for (int win = wave; win < kWindows; win += kWaves) {
+      const int o = win * 1024 + 16 * lane;
+      const int tp = o / kPlane;
+      const int b = (o % kPlane) / ROWB;
+      const int kv = ((o % ROWB) >> 4) ^ t2_swz<KC>(b);
+      const int bg = min(b, N - 1);
+      const int kg = min(16 * kv, Mp - 16);
+      const int8_t* src = Lq + tp * KNMp + ((long)n * N + bg) * Mp + kg;
+      __builtin_amdgcn_global_load_lds(
+          (const void __attribute__((address_space(1)))*)src,
+          (void __attribute__((address_space(3)))*)(sB[buf] + win * 1024), 16, 0, 0);
     }
-    if (threadIdx.x < kRows)
-      sU[threadIdx.x] = (int)threadIdx.x < N ? Lu[(long)n * N + threadIdx.x] : 0.0;
+  };
+  double preU = 0.0;
+  auto load_units = [&](int n) {
+    if (threadIdx.x < kRows) preU = (int)threadIdx.x < N ? Lu[(long)n * N + threadIdx.x] : 0.0;
+  };
+
+  if (nbeg < nend) {
+    stage(nbeg, 0);
+    load_units(nbeg);
+    if (threadIdx.x < kRows) sU[0][threadIdx.x] = preU;
+  }
+  for (int n = nbeg, buf = 0; n < nend; ++n, buf ^= 1) {
+    // vector n is in sB[buf] (this thread's copies done, then everyone's); every wavefront is
+    // past iteration n - 1, so sB[buf ^ 1], sU[buf ^ 1] and the T blocks are free again
+    __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0)
     __syncthreads();
+    const bool more = n + 1 < nend;
+    if (more) {
+      stage(n + 1, buf ^ 1);
+      load_units(n + 1);
+    }
+    const int8_t* sBb = sB[buf];
 
-    i4 accR[2][kOrders], accI[2][kOrders];
-#pragma unroll
-    for (int tb = 0; tb < 2; ++tb)
-#pragma unroll
-      for (int o = 0; o < kOrders; ++o) {
-        accR[tb][o] = i4{0, 0, 0, 0};
-        accI[tb][o] = i4{0, 0, 0, 0};
-      }
-#pragma unroll
-    for (int kc = 0; kc < KC; ++kc)
-#pragma unroll
-      for (int tb = 0; tb < 2; ++tb)
-#pragma unroll
-        for (int t = 0; t < kSL; ++t) {
-          const int8_t* base = sB + ((t * 2) * kRows + tb * 16 + r16) * ROWB + 64 * kc + 16 * g;
-          const i4 br = *reinterpret_cast<const i4*>(base);
-          const i4 bi = *reinterpret_cast<const i4*>(base + kRows * ROWB);
-#pragma unroll
-          for (int s = 0; s < kSG; ++s)
-            if (kept(s, t)) {
-              accR[tb][s + t] = mfma_i8(ar[kc][s], br, accR[tb][s + t]);
-              accI[tb][s + t] = mfma_i8(ar[kc][s], bi, accI[tb][s + t]);
-            }
-#pragma unroll
-          for (int s = 0; s < kSG; ++s)
-            if (kept(s, t)) {
-              accR[tb][s + t] = mfma_i8(nai[kc][s], bi, accR[tb][s + t]);
-              accI[tb][s + t] = mfma_i8(ai[kc][s], br, accI[tb][s + t]);
-            }
-        }
-
-    // T entries of rows 16 ta + 4g + r, columns 16 tb + r16, assembled in FP64
-    double tr[2][4], ti[2][4];
+    // the two column tiles one after the other: each tile's T entries (rows 16 ta + 4g + r,
+    // columns 16 tb + r16) are assembled in FP64 and written to the walker's T block
 #pragma unroll
     for (int tb = 0; tb < 2; ++tb) {
-      const double uL = sU[tb * 16 + r16];
+      i4 accR[kOrders], accI[kOrders];
+#pragma unroll
+      for (int o = 0; o < kOrders; ++o) {
+        accR[o] = i4{0, 0, 0, 0};
+        accI[o] = i4{0, 0, 0, 0};
+      }
+      const int row = tb * 16 + r16;
+#pragma unroll
+      for (int kc = 0; kc < KC; ++kc) {
+        const int chunk = (((4 * kc + g) ^ t2_swz<KC>(row)) << 4);
+#pragma unroll
+        for (int t = 0; t < kSL; ++t) {
+          const int8_t* base = sBb + ((t * 2) * kRows + row) * ROWB + chunk;
+          const i4 br = *reinterpret_cast<const i4*>(base);
+          const i4 bi = *reinterpret_cast<const i4*>(base + kPlane);
+#pragma unroll
+          for (int s = 0; s < kSG; ++s)
+            if (kept(s, t)) {
+              accR[s + t] = mfma_i8(ar[kc][s], br, accR[s + t]);
+              accI[s + t] = mfma_i8(ar[kc][s], bi, accI[s + t]);
+            }
+#pragma unroll
+          for (int s = 0; s < kSG; ++s)
+            if (kept(s, t)) {
+              accR[s + t] = mfma_i8(nai[kc][s], bi, accR[s + t]);
+              accI[s + t] = mfma_i8(ai[kc][s], br, accI[s + t]);
+            }
+        }
+      }
+      const double uL = sU[buf][row];
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        double xr = (double)accR[tb][kOrders - 1][r], xi = (double)accI[tb][kOrders - 1][r];
+        double xr = (double)accR[kOrders - 1][r], xi = (double)accI[kOrders - 1][r];
 #pragma unroll
         for (int o = kOrders - 2; o >= 0; --o) {
-          xr = fma(xr, 1.0 / kBase, (double)accR[tb][o][r]);
-          xi = fma(xi, 1.0 / kBase, (double)accI[tb][o][r]);
+          xr = fma(xr, 1.0 / kBase, (double)accR[o][r]);
+          xi = fma(xi, 1.0 / kBase, (double)accI[o][r]);
         }
         const double u = uRow[r] * uL;
-        tr[tb][r] = xr * u;
-        ti[tb][r] = xi * u;
-        const int a = ta * 16 + 4 * g + r, b = tb * 16 + r16;
-        T[2 * (a * kRows + tcol(a, b))] = tr[tb][r];
-        T[2 * (a * kRows + tcol(a, b)) + 1] = ti[tb][r];
+        const int a = ta * 16 + 4 * g + r;
+        T[2 * (a * kRows + tcol(a, row))] = xr * u;
+        T[2 * (a * kRows + tcol(a, row)) + 1] = xi * u;
       }
     }
+    if (more && threadIdx.x < kRows) sU[buf ^ 1][threadIdx.x] = preU;
     __syncthreads();   // both row tiles of T are in LDS
 #pragma unroll
     for (int tb = 0; tb < 2; ++tb)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int a = ta * 16 + 4 * g + r, b = tb * 16 + r16;
+        const double xr = T[2 * (a * kRows + tcol(a, b))];
+        const double xi = T[2 * (a * kRows + tcol(a, b)) + 1];
         const double yr = T[2 * (b * kRows + tcol(b, a))];
         const double yi = T[2 * (b * kRows + tcol(b, a)) + 1];
-        exx_re += tr[tb][r] * yr - ti[tb][r] * yi;
-        exx_im += tr[tb][r] * yi + ti[tb][r] * yr;
+        exx_re += xr * yr - xi * yi;
+        exx_im += xr * yi + xi * yr;
       }
     if (ta == 0) {
       // tr(T): diagonal entries a < N in lanes 0..31, summed by rows of 16 and combined
@@ -628,8 +668,8 @@ size_t exchange_i8_bytes(int M, int N, int K) {
 // so that the 2 tr(T^2) partials per chunk fit the K partials per walker of the workspace
 int exchange_i8_chunk(int W, int N, int K) {
   const int step = N > 16 ? 2 : stage_of(1);
-  const int wpb = N > 16 ? 2 : kWPB;
-  const int cus = device_cus() * kWgPerCu;
+  const int wpb = N > 16 ? kT2Walkers : kWPB;
+  const int cus = device_cus() * (N > 16 ? 1 : kWgPerCu);
   const int wgroups = (W + wpb - 1) / wpb;
   int best = step;
   double best_eff = -1.0;
@@ -672,14 +712,14 @@ void launch_exchange_i8(hipStream_t s, int W, int M, int N, int K, const double*
   const int nchunk = (K + chunk - 1) / chunk;
   const int Mp = exchange_i8_row(M);
   if (N > 16) {
-    dim3 grid((W + 1) / 2, nchunk);
+    dim3 grid((W + kT2Walkers - 1) / kT2Walkers, nchunk);
     const int nslots = 2 * nchunk;
     if (M <= 64)
-      hipLaunchKernelGGL((k_exchange_i8_t2<1>), grid, dim3(256), 0, s, W, M, N, K, Mp, chunk, G,
-                         Lq, Lu, Kl, exx_part, nslots);
+      hipLaunchKernelGGL((k_exchange_i8_t2<1>), grid, dim3(128 * kT2Walkers), 0, s, W, M, N, K,
+                         Mp, chunk, G, Lq, Lu, Kl, exx_part, nslots);
     else
-      hipLaunchKernelGGL((k_exchange_i8_t2<2>), grid, dim3(256), 0, s, W, M, N, K, Mp, chunk, G,
-                         Lq, Lu, Kl, exx_part, nslots);
+      hipLaunchKernelGGL((k_exchange_i8_t2<2>), grid, dim3(128 * kT2Walkers), 0, s, W, M, N, K,
+                         Mp, chunk, G, Lq, Lu, Kl, exx_part, nslots);
   } else {
     dim3 grid((W + kWPB - 1) / kWPB, nchunk);
     if (M <= 64)
