@@ -166,12 +166,83 @@ def cpu_baseline(s, W, ortho, seconds):
                        f"(QR every {ortho}), C++ FP64 oracle, OpenMP over walkers, {el:.1f} s")}
 
 
+def bench_batch(args, A, torch, synth):
+    """c4: 32 independent systems of uneven size. Each form runs every system's steps one system
+    at a time (eager launches: the reference's way, one job per system), then as one captured
+    graph with the systems spread over --streams concurrent capture streams (the walkers must
+    match the one-at-a-time run bit for bit). value = total walker-steps/s of the batched graph
+    of the fastest form within the accuracy gate (the int8 exchange's gate is checked per system
+    by the c2/c3 legs and the tests; here both forms are timed)."""
+    from safire_amd.batch import SystemBatch
+    systems, steps, ortho = synth.batch_workload()
+    steps = args.steps if args.steps != 100 else steps
+    res = {}
+    for name, prec in (("fp64", A.FP64), ("int8", A.INT8_SPLIT)):
+        b = SystemBatch(systems, steps=steps, ortho=ortho, precision=prec)
+        b.run_sequential()                                  # warm-up (kernel attributes)
+        b.reset()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        b.run_sequential()
+        torch.cuda.synchronize()
+        seq = time.perf_counter() - t0
+        ref = [it["wset"].buf.clone() for it in b.items]
+        b.reset()
+        g = b.capture(streams=args.streams)
+        b.reset()
+        g = b.capture(streams=args.streams)
+        torch.cuda.synchronize()
+        start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        start.record()
+        g.replay()
+        stop.record()
+        torch.cuda.synchronize()
+        bat = start.elapsed_time(stop) / 1e3
+        same = all(torch.equal(it["wset"].buf, r) for it, r in zip(b.items, ref))
+        ws = b.walker_steps()
+        res[name] = {"batched_walker_steps_per_s": round(ws / bat, 1),
+                     "one_at_a_time_walker_steps_per_s": round(ws / seq, 1),
+                     "batched_ms_per_step": round(bat * 1e3 / steps, 4),
+                     "one_at_a_time_ms_per_step": round(seq * 1e3 / steps, 4),
+                     "batched_matches_one_at_a_time": same}
+        print(f"[bench] c4 {name}: batched graph {bat * 1e3 / steps:.3f} ms/step, one at a time "
+              f"{seq * 1e3 / steps:.3f} ms/step, identical walkers: {same}", file=sys.stderr,
+              flush=True)
+        del g
+        b.close()
+    head = "int8" if res["int8"]["batched_walker_steps_per_s"] >= res["fp64"]["batched_walker_steps_per_s"] else "fp64"
+    Ms = sorted({s.M for s, _ in systems})
+    out = {
+        "metric": METRIC, "value": res[head]["batched_walker_steps_per_s"],
+        "unit": "walker-steps/s", "n_gpus": 1, "steps": steps, "warmup": 1,
+        "ms_per_step": res[head]["batched_ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": f"{head} contractions / fp64",
+        "data": "synthetic",
+        "config": {"workload": f"c4: {len(systems)} independent systems, NMO in {Ms}, "
+                               f"N = NMO/4, Nchol = 5 NMO, walkers 32-256 "
+                               f"({sum(W for _, W in systems)} in all), QR every {ortho}",
+                   "form": "one HIP graph per run, systems on "
+                           f"{args.streams} concurrent capture streams",
+                   "parallelism": "single GPU"},
+        "forms": res,
+        "batched_over_one_at_a_time": round(res[head]["batched_walker_steps_per_s"] /
+                                            res[head]["one_at_a_time_walker_steps_per_s"], 3),
+        "mixed_over_fp64": round(res["int8"]["batched_walker_steps_per_s"] /
+                                 res["fp64"]["batched_walker_steps_per_s"], 3),
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4"])
+    ap.add_argument("--streams", type=int, default=4,
+                    help="c4: capture streams the systems are spread over")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--forms", default="fp64,fp32,bf16x3,int8")
@@ -192,6 +263,8 @@ def main(argv=None):
     from safire_amd import afqmc as A
     from safire_amd import synth
 
+    if args.workload == "c4":
+        return bench_batch(args, A, torch, synth)
     s, W, _, ortho = synth.workload(args.workload)
     M, N, K = s.M, s.N, s.K
     nst = max(args.warmup + args.steps, 31)   # the accuracy gate replays 31 steps

@@ -62,6 +62,22 @@ def make_system(M: int, N: int, K: int, seed: int, decay: float = 24.0, scale: f
                   name=name)
 
 
+def batch_workload(count: int = 32, seed: int = SEEDS["c4"]) -> tuple[list, int, int]:
+    """The uneven batch (c4): `count` independent systems with NMO drawn uniformly from
+    {16, 24, ..., 96}, N = round(NMO / 4) electrons per spin, Nchol = 5 NMO, walkers uniform in
+    [32, 256], each with its own seeded H1, Cholesky vectors (decay 24) and RHF trial.
+    Returns ([(system, walkers), ...], steps, ortho_interval)."""
+    rng = np.random.default_rng([seed, 0])
+    out = []
+    for i in range(count):
+        M = int(rng.choice(np.arange(16, 97, 8)))
+        N = max(1, int(round(M / 4)))
+        W = int(rng.integers(32, 257))
+        s = make_system(M, N, 5 * M, seed * 1000 + i, decay=24.0, name=f"c4.{i}")
+        out.append((s, W))
+    return out, 50, 5
+
+
 def workload(name: str) -> tuple[System, int, int, int]:
     """Returns (system, walkers, steps, ortho_interval) of a named single-system workload."""
     M, N, K, W, steps, decay, ortho = WORKLOADS[name]
