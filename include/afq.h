@@ -260,6 +260,24 @@ int afq_rng_uniform(afq_rng rng, long n, double* out);
  * the N(0,1) fields of construct_X (construct_X_kernel.hpp:71). d_x may alias d_u. */
 int afq_probit(long n, const double* d_u, double* d_x, void* stream);
 
+/* calculate_overlaps (phmsd_helpers.hpp:111-195) for the rank-nex excitations of one spin:
+ * d_ov[d][w] = det A, A(p, q) = T[w][e_(nex+p)][e_q] with e = iexcit[d]; T [nw][nact][nelec]
+ * c128 (the walker's active-space matrix); 0 for a singular A. nex <= 8. */
+int afq_ph_overlaps(int nw, int ndet, int nex, int nact, int nelec, const int* d_iexcit,
+                    const double* d_T, double* d_ov, void* stream);
+/* get_compact_ph_R_matrices (phmsd_helpers.hpp:317-383): d_R [nw][ndet][nex][nact] c128,
+ * R(p, orbs(i)) = sum_q Q(p, q) (delta(i, e_q) - T[e_(nex+q)][i]) with Q = A^-1 and orbs the
+ * excited configuration (refc with e_q -> e_(nex+q)); zero elsewhere and where A is singular. */
+int afq_ph_compact_R(int nw, int ndet, int nex, int nact, int nelec, const int* d_iexcit,
+                     const int* d_refc, const double* d_T, double* d_R, void* stream);
+/* calculate_R (phmsd_helpers.hpp:200-262) for one shell, accumulated into d_R [nw][nelec][nact]
+ * c128: R[w][i][orbs_d(i)] += wgt[d][w] and R[w][e_p][:] += wgt[d][w] Rc[w][d][p][:] for every
+ * determinant with d_ov[d][w] != 0, in determinant order; nex = 0 adds the reference
+ * determinant's R[w][i][refc(i)] += wgt[0][w]. Call it for nex = 0, 1, 2, ... in turn. */
+int afq_ph_accumulate_R(int nw, int ndet, int nex, int nact, int nelec, const int* d_iexcit,
+                        const int* d_refc, const double* d_Rc, const double* d_ov,
+                        const double* d_wgt, double* d_R, void* stream);
+
 /* Lane layout of the MFMA builtins the kernels use, for a self-check; d_out [64][4] f64
  * receives accumulator element r of every lane.
  * kind 0 / 1: FP64 / FP32 16x16x4, D = A B with A[i][k] = i + 100 k, B[k][j] = (k == j % 4).

@@ -1004,6 +1004,119 @@ void orc_ph_excited_1body(int nw, int ndet, int nex, int nact, int nelec, const 
   }
 }
 
+// ---------------------------------------------------------------- PHMSD overlaps and R
+// One rank-nex shell: e = iexcit[d] (electrons e_0..e_{nex-1}, target orbitals after them);
+// T [nw][nact][nelec]; A(p, q) = T[w][e_{nex+p}][e_q].
+static bool ph_factor(int nex, int nact, int nelec, const int* e, const cd* Tw, cd* Q,
+                      int* ipiv, cd& ov) {
+  for (int p = 0; p < nex; ++p)
+    for (int q = 0; q < nex; ++q) Q[p * nex + q] = Tw[(long)e[nex + p] * nelec + e[q]];
+  getrf(nex, Q, ipiv);
+  log_det_from_getrf(nex, Q, ipiv, ov);     // ov = log det (reference: exp(log det))
+  ov = std::exp(ov);
+  return std::abs(ov) != 0.0;
+}
+
+// calculate_overlaps (phmsd_helpers.hpp:111-195), one shell: ov[d][w] = det A
+void orc_ph_overlaps(int nw, int ndet, int nex, int nact, int nelec, const int* iexcit,
+                     const cd* T, cd* ov) {
+  for (int d = 0; d < ndet; ++d)
+    for (int w = 0; w < nw; ++w) {
+      std::vector<cd> Q(nex * nex);
+      std::vector<int> ipiv(nex);
+      cd o;
+      ph_factor(nex, nact, nelec, iexcit + (long)d * 2 * nex, T + (long)w * nact * nelec,
+                Q.data(), ipiv.data(), o);
+      // a zero pivot gives log(DBL_MIN)-sized values in log_det; the determinant itself is 0
+      bool sing = false;
+      for (int j = 0; j < nex; ++j) sing |= Q[j * nex + j] == cd(0.0);
+      ov[(long)d * nw + w] = sing ? cd(0.0) : o;
+    }
+}
+
+// get_compact_ph_R_matrices (phmsd_helpers.hpp:317-383): R [nw][ndet][nex][nact], zeroed first
+void orc_ph_compact_R(int nw, int ndet, int nex, int nact, int nelec, const int* iexcit,
+                      const int* refc, const cd* T, cd* R) {
+  std::fill(R, R + (long)nw * ndet * nex * nact, cd(0.0));
+  std::vector<cd> Q(nex * nex);
+  std::vector<int> ipiv(nex), orbs(nelec);
+  for (int w = 0; w < nw; ++w) {
+    const cd* Tw = T + (long)w * nact * nelec;
+    for (int d = 0; d < ndet; ++d) {
+      const int* e = iexcit + (long)d * 2 * nex;
+      cd* Rd = R + ((long)w * ndet + d) * nex * nact;
+      for (int i = 0; i < nelec; ++i) {
+        orbs[i] = refc[i];
+        for (int p = 0; p < nex; ++p)
+          if (e[p] == i) {
+            orbs[i] = e[nex + p];
+            break;
+          }
+      }
+      cd ov;
+      ph_factor(nex, nact, nelec, e, Tw, Q.data(), ipiv.data(), ov);
+      bool sing = false;
+      for (int j = 0; j < nex; ++j) sing |= Q[j * nex + j] == cd(0.0);
+      if (sing) continue;
+      getri_or_zero(nex, Q.data(), ipiv.data());
+      for (int p = 0; p < nex; ++p)
+        for (int q = 0; q < nex; ++q) {
+          const cd Ipq = Q[p * nex + q];
+          const cd* Tq = Tw + (long)e[q + nex] * nelec;
+          for (int i = 0; i < nelec; ++i) Rd[(long)p * nact + orbs[i]] -= Ipq * Tq[i];
+          Rd[(long)p * nact + orbs[e[q]]] += Ipq;
+        }
+    }
+  }
+}
+
+// calculate_R (phmsd_helpers.hpp:200-262), host path, for shells nex = 1..nshell with the
+// excitation lists concatenated (shell k holds ndets[k] determinants, 2 k ints each) and the
+// weights wgt [1 + sum ndets][nw] (row 0 the reference): R [nw][nelec][nact], zeroed first
+void orc_ph_calculate_R(int nw, int nshell, const int* ndets, int nact, int nelec,
+                        const int* iexcit, const int* refc, const cd* T, const cd* wgt, cd* R) {
+  std::fill(R, R + (long)nw * nelec * nact, cd(0.0));
+  for (int i = 0; i < nelec; ++i)
+    for (int w = 0; w < nw; ++w) R[((long)w * nelec + i) * nact + refc[i]] += wgt[w];
+  std::vector<int> orbs(nelec);
+  const int* ex = iexcit;
+  for (int nex = 1, nd = 1; nex <= nshell; ++nex) {
+    std::vector<cd> Q(nex * nex);
+    std::vector<int> ipiv(nex);
+    for (int k = 0; k < ndets[nex - 1]; ++k, ++nd, ex += 2 * nex) {
+      for (int i = 0; i < nelec; ++i) {
+        orbs[i] = refc[i];
+        for (int p = 0; p < nex; ++p)
+          if (ex[p] == i) {
+            orbs[i] = ex[nex + p];
+            break;
+          }
+      }
+      for (int w = 0; w < nw; ++w) {
+        const cd* Tw = T + (long)w * nact * nelec;
+        cd ov;
+        ph_factor(nex, nact, nelec, ex, Tw, Q.data(), ipiv.data(), ov);
+        bool sing = false;
+        for (int j = 0; j < nex; ++j) sing |= Q[j * nex + j] == cd(0.0);
+        if (sing) continue;
+        getri_or_zero(nex, Q.data(), ipiv.data());
+        const cd wd = wgt[(long)nd * nw + w];
+        cd* Rw = R + (long)w * nelec * nact;
+        for (int i = 0; i < nelec; ++i) Rw[(long)i * nact + orbs[i]] += wd;
+        for (int p = 0; p < nex; ++p) {
+          cd* Rp = Rw + (long)ex[p] * nact;
+          for (int q = 0; q < nex; ++q) {
+            const cd Ipq = Q[p * nex + q];
+            const cd* Tq = Tw + (long)ex[q + nex] * nelec;
+            for (int i = 0; i < nelec; ++i) Rp[orbs[i]] -= wd * Ipq * Tq[i];
+            Rp[orbs[ex[q]]] += wd * Ipq;
+          }
+        }
+      }
+    }
+  }
+}
+
 int orc_num_threads(void) { return omp_get_max_threads(); }
 void orc_set_num_threads(int n) { omp_set_num_threads(n); }
 

@@ -72,6 +72,9 @@ def _declare(L):
         "orc_ph_excited_2body": (None, [_i, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp,
                                         _vp, _vp]),
         "orc_ph_excited_1body": (None, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
+        "orc_ph_overlaps": (None, [_i, _i, _i, _i, _i, _vp, _vp, _vp]),
+        "orc_ph_compact_R": (None, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _vp]),
+        "orc_ph_calculate_R": (None, [_i, _i, _vp, _i, _i, _vp, _vp, _vp, _vp, _vp]),
         "orc_system_create": (_vp, [_i, _i, _i, _vp, _vp, _vp, _d, _d]),
         "orc_system_destroy": (None, [_vp]),
         "orc_system_set_params": (None, [_vp, _i, _d, _d, _d, _d, _d, _i, _i, _i]),
@@ -283,6 +286,42 @@ def ph_excited_1body(iexcit, refc, S, R, wgt):
     lib().orc_ph_excited_1body(nw, ndet, nex, nact, nelec, _p(iex), _p(ref), _p(_c(S)),
                                _p(_c(R)), _p(_c(wgt)), _p(E))
     return E
+
+
+def ph_overlaps(iexcit, T):
+    """calculate_overlaps (phmsd_helpers.hpp:111-195), one shell: ov [ndet][nw]."""
+    ndet, two_nex = iexcit.shape
+    nw, nact, nelec = T.shape
+    iex = np.ascontiguousarray(iexcit, np.int32)
+    ov = np.zeros((ndet, nw), np.complex128)
+    lib().orc_ph_overlaps(nw, ndet, two_nex // 2, nact, nelec, _p(iex), _p(_c(T)), _p(ov))
+    return ov
+
+
+def ph_compact_R(iexcit, refc, T):
+    """get_compact_ph_R_matrices (phmsd_helpers.hpp:317-383): R [nw][ndet][nex][nact]."""
+    ndet, two_nex = iexcit.shape
+    nex = two_nex // 2
+    nw, nact, nelec = T.shape
+    iex = np.ascontiguousarray(iexcit, np.int32)
+    ref = np.ascontiguousarray(refc, np.int32)
+    R = np.zeros((nw, ndet, nex, nact), np.complex128)
+    lib().orc_ph_compact_R(nw, ndet, nex, nact, nelec, _p(iex), _p(ref), _p(_c(T)), _p(R))
+    return R
+
+
+def ph_calculate_R(shells, refc, T, wgt):
+    """calculate_R (phmsd_helpers.hpp:200-262): shells[k] = iexcit [ndet_k][2 (k + 1)] of rank
+    k + 1; wgt [1 + sum ndet_k][nw] (row 0 the reference); R [nw][nelec][nact]."""
+    nw, nact, nelec = T.shape
+    ndets = np.array([len(x) for x in shells], np.int32)
+    iex = np.ascontiguousarray(np.concatenate([np.asarray(x, np.int32).reshape(-1)
+                                               for x in shells]), np.int32)
+    ref = np.ascontiguousarray(refc, np.int32)
+    R = np.zeros((nw, nelec, nact), np.complex128)
+    lib().orc_ph_calculate_R(nw, len(shells), _p(ndets), nact, nelec, _p(iex), _p(ref),
+                             _p(_c(T)), _p(_c(wgt)), _p(R))
+    return R
 
 
 # ------------------------------------------------------------------ random numbers

@@ -78,6 +78,9 @@ SIGNATURES = {
     "afq_ph_excited_2body_energy": (_i, [_i, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp,
                                          _vp, _vp, _vp]),
     "afq_ph_excited_1body_energy": (_i, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "afq_ph_overlaps": (_i, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _vp]),
+    "afq_ph_compact_R": (_i, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp]),
+    "afq_ph_accumulate_R": (_i, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "afq_split_seed": (C.c_ulonglong, [_i, _i, C.c_uint]),
     "afq_rng_create": (_i, [C.c_ulonglong, C.POINTER(_vp)]),
     "afq_rng_destroy": (None, [_vp]),

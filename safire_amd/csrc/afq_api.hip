@@ -660,6 +660,34 @@ int afq_ph_excited_1body_energy(int nw, int ndet, int nex, int nact, int nelec,
   });
 }
 
+int afq_ph_overlaps(int nw, int ndet, int nex, int nact, int nelec, const int* iexcit,
+                    const double* T, double* ov, void* stream) {
+  return guard([&] {
+    AFQ_CHECK(nw >= 0 && ndet >= 0 && nex >= 1 && nact > 0 && nelec > 0 && nelec <= nact,
+              AFQ_EBADSHAPE, "ph overlaps: bad sizes");
+    launch_ph_overlaps(S(stream), nw, ndet, nex, nact, nelec, iexcit, T, ov);
+  });
+}
+
+int afq_ph_compact_R(int nw, int ndet, int nex, int nact, int nelec, const int* iexcit,
+                     const int* refc, const double* T, double* R, void* stream) {
+  return guard([&] {
+    AFQ_CHECK(nw >= 0 && ndet >= 0 && nex >= 1 && nact > 0 && nelec > 0 && nelec <= nact,
+              AFQ_EBADSHAPE, "ph R matrices: bad sizes");
+    launch_ph_compact_R(S(stream), nw, ndet, nex, nact, nelec, iexcit, refc, T, R);
+  });
+}
+
+int afq_ph_accumulate_R(int nw, int ndet, int nex, int nact, int nelec, const int* iexcit,
+                        const int* refc, const double* Rc, const double* ov, const double* wgt,
+                        double* R, void* stream) {
+  return guard([&] {
+    AFQ_CHECK(nw >= 0 && ndet >= 0 && nex >= 0 && nact > 0 && nelec > 0 && nelec <= nact,
+              AFQ_EBADSHAPE, "ph R accumulation: bad sizes");
+    launch_ph_accumulate_R(S(stream), nw, ndet, nex, nact, nelec, iexcit, refc, Rc, ov, wgt, R);
+  });
+}
+
 int afq_probe_mfma(int kind, double* out, void* stream) {
   return guard([&] {
     AFQ_CHECK(kind >= 0 && kind <= 5, AFQ_EINVAL,

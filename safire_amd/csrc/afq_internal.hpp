@@ -255,5 +255,12 @@ void launch_ph_2body(hipStream_t s, int nw, int ndet, int nex, int nact, int nel
 void launch_ph_1body(hipStream_t s, int nw, int ndet, int nex, int nact, int nelec,
                      const int* iexcit, const int* refc, const double* S, const double* R,
                      const double* wgt, double* E);
+void launch_ph_overlaps(hipStream_t s, int nw, int ndet, int nex, int nact, int nelec,
+                        const int* iexcit, const double* T, double* ov);
+void launch_ph_compact_R(hipStream_t s, int nw, int ndet, int nex, int nact, int nelec,
+                         const int* iexcit, const int* refc, const double* T, double* R);
+void launch_ph_accumulate_R(hipStream_t s, int nw, int ndet, int nex, int nact, int nelec,
+                            const int* iexcit, const int* refc, const double* Rc,
+                            const double* ov, const double* wgt, double* R);
 
 }  // namespace afq

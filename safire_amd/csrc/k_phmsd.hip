@@ -175,6 +175,183 @@ __global__ __launch_bounds__(kPhThreads) void k_ph_1body(
   if (threadIdx.x == 0) E[w] += s;
 }
 
+// ---- excitation overlaps and R matrices (phmsd_helpers.hpp) ----
+//
+// For the rank-nex excitations of one spin, iexcit [ndet][2 nex] holds the excited electrons
+// e_0..e_{nex-1} and their target active orbitals e_nex..e_{2nex-1}; T [nw][nact][nel] is the
+// walker's active-space matrix. Per determinant and walker A(p, q) = T[e_{nex+p}][e_q]:
+//   overlap ratio = det A                                   (calculate_overlaps, 111-195)
+//   compact R(p, a), Q = A^-1, zero unless a is occupied in the excited configuration orbs:
+//     R(p, orbs(i)) = sum_q Q(p, q) (delta(i, e_q) - T[e_{nex+q}][i])
+//                                                          (get_compact_ph_R_matrices, 317-383)
+//   and nothing where det A == 0.
+// One thread per (determinant, walker); A is factorised in registers by Gauss-Jordan with
+// partial pivoting (first maximum of |re| + |im|).
+template <int NEX>
+__device__ inline bool small_inverse(cd (&a)[NEX][NEX], cd (&q)[NEX][NEX], cd& det) {
+#pragma unroll
+  for (int i = 0; i < NEX; ++i)
+#pragma unroll
+    for (int j = 0; j < NEX; ++j) q[i][j] = cmk(i == j ? 1.0 : 0.0, 0.0);
+  det = cmk(1.0, 0.0);
+#pragma unroll
+  for (int j = 0; j < NEX; ++j) {
+    int p = j;
+    double best = cabs1(a[j][j]);
+#pragma unroll
+    for (int r = j + 1; r < NEX; ++r)
+      if (cabs1(a[r][j]) > best) {
+        best = cabs1(a[r][j]);
+        p = r;
+      }
+    if (p != j) {
+      det = -det;
+#pragma unroll
+      for (int r = j + 1; r < NEX; ++r)
+        if (r == p)
+#pragma unroll
+          for (int c = 0; c < NEX; ++c) {
+            const cd t = a[j][c];
+            a[j][c] = a[r][c];
+            a[r][c] = t;
+            const cd u = q[j][c];
+            q[j][c] = q[r][c];
+            q[r][c] = u;
+          }
+    }
+    const cd piv = a[j][j];
+    det = det * piv;
+    if (cis0(piv)) return false;
+    const cd inv = cdiv(cmk(1.0, 0.0), piv);
+#pragma unroll
+    for (int c = 0; c < NEX; ++c) {
+      a[j][c] = a[j][c] * inv;
+      q[j][c] = q[j][c] * inv;
+    }
+#pragma unroll
+    for (int r = 0; r < NEX; ++r) {
+      if (r == j) continue;
+      const cd f = a[r][j];
+#pragma unroll
+      for (int c = 0; c < NEX; ++c) {
+        a[r][c] -= f * a[j][c];
+        q[r][c] -= f * q[j][c];
+      }
+    }
+  }
+  return true;
+}
+
+template <int NEX>
+__device__ inline bool load_excitation(int nact, int nelec, const int* e, const cd* Tw,
+                                       cd (&a)[NEX][NEX]) {
+  bool ok = true;
+#pragma unroll
+  for (int p = 0; p < NEX; ++p) ok &= e[p] >= 0 && e[p] < nelec && e[NEX + p] >= 0 &&
+                                      e[NEX + p] < nact;
+#pragma unroll
+  for (int p = 0; p < NEX; ++p)
+#pragma unroll
+    for (int q = 0; q < NEX; ++q) {
+      const int r = min(max(e[NEX + p], 0), nact - 1), c = min(max(e[q], 0), nelec - 1);
+      a[p][q] = Tw[(long)r * nelec + c];
+    }
+  return ok;
+}
+
+template <int NEX>
+__global__ __launch_bounds__(256) void k_ph_overlaps(int nw, int ndet, int nact, int nelec,
+                                                     const int* __restrict__ iexcit,
+                                                     const cd* __restrict__ T,
+                                                     cd* __restrict__ ov) {
+  const long id = (long)blockIdx.x * 256 + threadIdx.x;
+  if (id >= (long)nw * ndet) return;
+  const int w = (int)(id % nw), d = (int)(id / nw);
+  cd a[NEX][NEX], q[NEX][NEX], det;
+  const bool ok = load_excitation<NEX>(nact, nelec, iexcit + (long)d * 2 * NEX,
+                                       T + (long)w * nact * nelec, a);
+  if (!small_inverse<NEX>(a, q, det)) det = cmk(0.0, 0.0);
+  ov[(long)d * nw + w] = ok ? det : cmk(__builtin_nan(""), __builtin_nan(""));
+}
+
+template <int NEX>
+__global__ __launch_bounds__(256) void k_ph_compact_R(int nw, int ndet, int nact, int nelec,
+                                                      const int* __restrict__ iexcit,
+                                                      const int* __restrict__ refc,
+                                                      const cd* __restrict__ T,
+                                                      cd* __restrict__ R) {
+  const long id = (long)blockIdx.x * 256 + threadIdx.x;
+  if (id >= (long)nw * ndet) return;
+  const int w = (int)(id / ndet), d = (int)(id % ndet);
+  const int* e = iexcit + (long)d * 2 * NEX;
+  const cd* Tw = T + (long)w * nact * nelec;
+  cd* Rd = R + ((long)w * ndet + d) * NEX * nact;
+  for (int i = 0; i < NEX * nact; ++i) Rd[i] = cmk(0.0, 0.0);
+  cd a[NEX][NEX], q[NEX][NEX], det;
+  const bool ok = load_excitation<NEX>(nact, nelec, e, Tw, a);
+  if (!ok) {
+    for (int i = 0; i < NEX * nact; ++i) Rd[i] = cmk(__builtin_nan(""), __builtin_nan(""));
+    return;
+  }
+  if (!small_inverse<NEX>(a, q, det)) return;
+  for (int i = 0; i < nelec; ++i) {
+    int o = min(max(refc[i], 0), nact - 1);
+    int qi = -1;
+#pragma unroll
+    for (int p = 0; p < NEX; ++p)
+      if (e[p] == i) {
+        o = e[NEX + p];
+        qi = p;
+      }
+#pragma unroll
+    for (int p = 0; p < NEX; ++p) {
+      cd v = cmk(0.0, 0.0);
+#pragma unroll
+      for (int qq = 0; qq < NEX; ++qq) v -= q[p][qq] * Tw[(long)e[NEX + qq] * nelec + i];
+      if (qi >= 0) v += q[p][qi];
+      Rd[(long)p * nact + o] += v;
+    }
+  }
+}
+
+// calculate_R (phmsd_helpers.hpp:200-262) for one shell of rank nex, added to R [nw][nel][nact]:
+//   R[w][i][orbs_d(i)] += wgt[d][w] for every electron, and R[w][e_p][:] += wgt[d][w] Rc[w][d][p][:]
+// for every determinant with det A != 0 (ov != 0), in determinant order; nex = 0 adds the
+// reference determinant's term R[w][i][refc(i)] += wgt[0][w]. One workgroup per walker, threads
+// over the entries (i, a), so every entry is summed over the determinants in their order.
+__global__ __launch_bounds__(256) void k_ph_accumulate_R(int nw, int ndet, int nex, int nact,
+                                                         int nelec, const int* __restrict__ iexcit,
+                                                         const int* __restrict__ refc,
+                                                         const cd* __restrict__ Rc,
+                                                         const cd* __restrict__ ov,
+                                                         const cd* __restrict__ wgt,
+                                                         cd* __restrict__ R) {
+  const int w = blockIdx.x;
+  cd* Rw = R + (long)w * nelec * nact;
+  for (int ea = threadIdx.x; ea < nelec * nact; ea += 256) {
+    const int i = ea / nact, a = ea % nact;
+    cd acc = Rw[ea];
+    if (nex == 0) {
+      if (refc[i] == a) acc += wgt[w];
+    } else {
+      for (int d = 0; d < ndet; ++d) {
+        if (cis0(ov[(long)d * nw + w])) continue;
+        const int* e = iexcit + (long)d * 2 * nex;
+        const cd wd = wgt[(long)d * nw + w];
+        int o = refc[i], pe = -1;
+        for (int p = 0; p < nex; ++p)
+          if (e[p] == i) {
+            o = e[nex + p];
+            pe = p;
+          }
+        if (o == a) acc += wd;
+        if (pe >= 0) acc += wd * Rc[(((long)w * ndet + d) * nex + pe) * nact + a];
+      }
+    }
+    Rw[ea] = acc;
+  }
+}
+
 }  // namespace
 
 void launch_ph_2body(hipStream_t s, int nw, int ndet, int nex, int nact, int nelec, int nchol,
@@ -211,6 +388,62 @@ void launch_ph_1body(hipStream_t s, int nw, int ndet, int nex, int nact, int nel
   hipLaunchKernelGGL(k_ph_1body, dim3(nw), dim3(kPhThreads), 0, s, nw, ndet, nex, nact, nelec,
                      iexcit, refc, reinterpret_cast<const cd*>(S), reinterpret_cast<const cd*>(R),
                      reinterpret_cast<const cd*>(wgt), reinterpret_cast<cd*>(E));
+  AFQ_HIP(hipGetLastError());
+}
+
+void launch_ph_overlaps(hipStream_t s, int nw, int ndet, int nex, int nact, int nelec,
+                        const int* iexcit, const double* T, double* ov) {
+  if (nw <= 0 || ndet <= 0) return;
+  AFQ_CHECK(nex >= 1 && nex <= kPhMaxEx, AFQ_EUNSUPPORTED, "ph overlaps: nex must be 1..8");
+  const unsigned blocks = (unsigned)(((long)nw * ndet + 255) / 256);
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, s, nw, ndet, nact, nelec, iexcit,
+                       reinterpret_cast<const cd*>(T), reinterpret_cast<cd*>(ov));
+  };
+  switch (nex) {
+    case 1: go(k_ph_overlaps<1>); break;
+    case 2: go(k_ph_overlaps<2>); break;
+    case 3: go(k_ph_overlaps<3>); break;
+    case 4: go(k_ph_overlaps<4>); break;
+    case 5: go(k_ph_overlaps<5>); break;
+    case 6: go(k_ph_overlaps<6>); break;
+    case 7: go(k_ph_overlaps<7>); break;
+    default: go(k_ph_overlaps<8>); break;
+  }
+  AFQ_HIP(hipGetLastError());
+}
+
+void launch_ph_compact_R(hipStream_t s, int nw, int ndet, int nex, int nact, int nelec,
+                         const int* iexcit, const int* refc, const double* T, double* R) {
+  if (nw <= 0 || ndet <= 0) return;
+  AFQ_CHECK(nex >= 1 && nex <= kPhMaxEx, AFQ_EUNSUPPORTED, "ph R matrices: nex must be 1..8");
+  const unsigned blocks = (unsigned)(((long)nw * ndet + 255) / 256);
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, s, nw, ndet, nact, nelec, iexcit, refc,
+                       reinterpret_cast<const cd*>(T), reinterpret_cast<cd*>(R));
+  };
+  switch (nex) {
+    case 1: go(k_ph_compact_R<1>); break;
+    case 2: go(k_ph_compact_R<2>); break;
+    case 3: go(k_ph_compact_R<3>); break;
+    case 4: go(k_ph_compact_R<4>); break;
+    case 5: go(k_ph_compact_R<5>); break;
+    case 6: go(k_ph_compact_R<6>); break;
+    case 7: go(k_ph_compact_R<7>); break;
+    default: go(k_ph_compact_R<8>); break;
+  }
+  AFQ_HIP(hipGetLastError());
+}
+
+void launch_ph_accumulate_R(hipStream_t s, int nw, int ndet, int nex, int nact, int nelec,
+                            const int* iexcit, const int* refc, const double* Rc,
+                            const double* ov, const double* wgt, double* R) {
+  if (nw <= 0) return;
+  AFQ_CHECK(nex >= 0 && nex <= kPhMaxEx, AFQ_EUNSUPPORTED, "ph R accumulation: nex must be 0..8");
+  hipLaunchKernelGGL(k_ph_accumulate_R, dim3(nw), dim3(256), 0, s, nw, ndet, nex, nact, nelec,
+                     iexcit, refc, reinterpret_cast<const cd*>(Rc),
+                     reinterpret_cast<const cd*>(ov), reinterpret_cast<const cd*>(wgt),
+                     reinterpret_cast<cd*>(R));
   AFQ_HIP(hipGetLastError());
 }
 

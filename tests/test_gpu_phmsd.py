@@ -94,3 +94,82 @@ def test_ph_1body_matches_oracle():
                                                 A.stream_ptr()))
     torch.cuda.synchronize()
     np.testing.assert_allclose(E.cpu().numpy(), Eo, rtol=1e-12)
+
+
+# ---- excitation overlaps and R matrices (calculate_overlaps, compact R, calculate_R) ----
+from tests.test_oracle_phmsd_R import shells_for  # noqa: E402
+
+
+def i32(a):
+    return torch.from_numpy(np.ascontiguousarray(a, np.int32)).to("cuda")
+
+
+@pytest.mark.parametrize("nex", [1, 2, 3, 4, 5, 8])
+def test_ph_overlaps_match_oracle(nex):
+    rng = np.random.default_rng(40 + nex)
+    nw, nelec, nact, ndet = 33, 10, 24, 70
+    T = rand_c(rng, nw, nact, nelec)
+    ex = shells_for(rng, nelec, nact, ndet, nex)
+    # a target orbital with an all-zero row of T: every excitation into it is exactly singular
+    # (a zero pivot in any arithmetic)
+    T[:, ex[3, nex], :] = 0
+    ie, Td = i32(ex), dev(T)
+    ov = torch.zeros((ndet, nw), dtype=torch.complex128, device="cuda")
+    A.check(A.lib().afq_ph_overlaps(nw, ndet, nex, nact, nelec, A.ptr(ie), A.ptr(Td), A.ptr(ov),
+                                    A.stream_ptr()))
+    ref = O.ph_overlaps(ex, T)
+    got = ov.cpu().numpy()
+    assert np.all(ref[3] == 0) and np.all(got[3] == 0)
+    np.testing.assert_allclose(got, ref, rtol=1e-11, atol=1e-13 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("nex", [1, 2, 3, 4, 5])
+def test_ph_compact_R_matches_oracle(nex):
+    rng = np.random.default_rng(50 + nex)
+    nw, nelec, nact, ndet = 17, 10, 24, 40
+    T = rand_c(rng, nw, nact, nelec)
+    refc = rng.permutation(nact)[:nelec]
+    ex = shells_for(rng, nelec, nact, ndet, nex)
+    others = np.setdiff1d(np.arange(nact), refc)
+    ex[:, nex:] = np.array([rng.choice(others, nex, replace=False) for _ in range(ndet)])
+    ie, rc, Td = i32(ex), i32(refc), dev(T)
+    R = torch.empty((nw, ndet, nex, nact), dtype=torch.complex128, device="cuda")
+    A.check(A.lib().afq_ph_compact_R(nw, ndet, nex, nact, nelec, A.ptr(ie), A.ptr(rc), A.ptr(Td),
+                                     A.ptr(R), A.stream_ptr()))
+    ref = O.ph_compact_R(ex, refc, T)
+    np.testing.assert_allclose(R.cpu().numpy(), ref, rtol=1e-10, atol=1e-12 * np.abs(ref).max())
+
+
+def test_ph_calculate_R_matches_oracle():
+    rng = np.random.default_rng(7)
+    nw, nelec, nact = 19, 10, 30
+    T = rand_c(rng, nw, nact, nelec)
+    refc = np.arange(nelec)
+    shells = [shells_for(rng, nelec, nact, n, k + 1) for k, n in enumerate([12, 30, 20, 9])]
+    T[:, shells[1][4, 2], :] = 0                # determinants into this orbital are singular
+    nd = 1 + sum(len(s) for s in shells)
+    wgt = rand_c(rng, nd, nw)
+    Td, rc, Wd = dev(T), i32(refc), dev(wgt)
+    R = torch.zeros((nw, nelec, nact), dtype=torch.complex128, device="cuda")
+    A.check(A.lib().afq_ph_accumulate_R(nw, 1, 0, nact, nelec, None, A.ptr(rc), None, None,
+                                        A.ptr(Wd), A.ptr(R), A.stream_ptr()))
+    row = 1
+    keep = []
+    for sh in shells:
+        nex, ndet = sh.shape[1] // 2, sh.shape[0]
+        ie = i32(sh)
+        ov = torch.zeros((ndet, nw), dtype=torch.complex128, device="cuda")
+        Rc = torch.empty((nw, ndet, nex, nact), dtype=torch.complex128, device="cuda")
+        A.check(A.lib().afq_ph_overlaps(nw, ndet, nex, nact, nelec, A.ptr(ie), A.ptr(Td),
+                                        A.ptr(ov), A.stream_ptr()))
+        A.check(A.lib().afq_ph_compact_R(nw, ndet, nex, nact, nelec, A.ptr(ie), A.ptr(rc),
+                                         A.ptr(Td), A.ptr(Rc), A.stream_ptr()))
+        wsh = Wd[row:row + ndet]
+        A.check(A.lib().afq_ph_accumulate_R(nw, ndet, nex, nact, nelec, A.ptr(ie), A.ptr(rc),
+                                            A.ptr(Rc), A.ptr(ov), A.ptr(wsh), A.ptr(R),
+                                            A.stream_ptr()))
+        keep += [ie, ov, Rc, wsh]
+        row += ndet
+    torch.cuda.synchronize()
+    ref = O.ph_calculate_R(shells, refc, T, wgt)
+    np.testing.assert_allclose(R.cpu().numpy(), ref, rtol=1e-10, atol=1e-11 * np.abs(ref).max())
