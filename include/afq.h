@@ -260,6 +260,16 @@ int afq_rng_uniform(afq_rng rng, long n, double* out);
  * the N(0,1) fields of construct_X (construct_X_kernel.hpp:71). d_x may alias d_u. */
 int afq_probit(long n, const double* d_u, double* d_x, void* stream);
 
+/* MixedDensityMatrixForWoodbury (density_matrix.hpp:651-717), compact form: with TAB = A Phi_w
+ * (A [nact][nmo] c128, the trial's active orbitals as rows) and TNN = TAB[ref rows]:
+ * d_ov[w] += log det TNN (pivot signs, DBL_MIN, imaginary part wrapped to [-pi, pi]),
+ * d_G [nb][nel][nmo] = TNN^-T Phi_w^T (zero if TNN is singular), d_QQ0 [nb][nact][nel] =
+ * TAB TNN^-1, the T of afq_ph_overlaps / afq_ph_compact_R. Phi [nb] rows of ldw c128 holding
+ * [nmo][nel]; d_work: nel * nmo c128 of device workspace. */
+int afq_ph_mixed_density_matrix_woodbury(int nb, int nmo, int nel, int nact, const double* d_A,
+                                         const int* d_ref, const double* d_Phi, long ldw,
+                                         double* d_G, double* d_ov, double* d_QQ0,
+                                         double* d_work, void* stream);
 /* calculate_overlaps (phmsd_helpers.hpp:111-195) for the rank-nex excitations of one spin:
  * d_ov[d][w] = det A, A(p, q) = T[w][e_(nex+p)][e_q] with e = iexcit[d]; T [nw][nact][nelec]
  * c128 (the walker's active-space matrix); 0 for a singular A. nex <= 8. */

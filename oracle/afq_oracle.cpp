@@ -1017,6 +1017,40 @@ static bool ph_factor(int nex, int nact, int nelec, const int* e, const cd* Tw, 
   return std::abs(ov) != 0.0;
 }
 
+// MixedDensityMatrixForWoodbury (density_matrix.hpp:651-717), compact: TAB = A B_w,
+// TNN(i, :) = TAB(ref(i), :), getrf, ov[w] += log det (log_determinant_from_getrf),
+// getri_or_zero, QQ0 = TAB inv(TNN), G = inv(TNN)^T B_w^T
+void orc_ph_woodbury(int nb, int nmo, int nel, int nact, const cd* A, const int* ref,
+                     const cd* Phi, long ldw, cd* G, cd* ov, cd* QQ0) {
+  for (int w = 0; w < nb; ++w) {
+    const cd* B = Phi + (long)w * ldw;                    // [nmo][nel]
+    std::vector<cd> TAB((long)nact * nel, 0.0), TNN((long)nel * nel);
+    for (int a = 0; a < nact; ++a)
+      for (int k = 0; k < nmo; ++k) {
+        const cd x = A[(long)a * nmo + k];
+        for (int e = 0; e < nel; ++e) TAB[(long)a * nel + e] += x * B[(long)k * nel + e];
+      }
+    for (int i = 0; i < nel; ++i)
+      for (int e = 0; e < nel; ++e) TNN[(long)i * nel + e] = TAB[(long)ref[i] * nel + e];
+    std::vector<int> ipiv(nel);
+    getrf(nel, TNN.data(), ipiv.data());
+    log_det_from_getrf(nel, TNN.data(), ipiv.data(), ov[w]);
+    getri_or_zero(nel, TNN.data(), ipiv.data());
+    for (int a = 0; a < nact; ++a)
+      for (int e = 0; e < nel; ++e) {
+        cd acc = 0.0;
+        for (int j = 0; j < nel; ++j) acc += TAB[(long)a * nel + j] * TNN[(long)j * nel + e];
+        QQ0[((long)w * nact + a) * nel + e] = acc;
+      }
+    for (int e = 0; e < nel; ++e)
+      for (int k = 0; k < nmo; ++k) {
+        cd acc = 0.0;
+        for (int j = 0; j < nel; ++j) acc += TNN[(long)j * nel + e] * B[(long)k * nel + j];
+        G[((long)w * nel + e) * nmo + k] = acc;
+      }
+  }
+}
+
 // calculate_overlaps (phmsd_helpers.hpp:111-195), one shell: ov[d][w] = det A
 void orc_ph_overlaps(int nw, int ndet, int nex, int nact, int nelec, const int* iexcit,
                      const cd* T, cd* ov) {

@@ -73,6 +73,7 @@ def _declare(L):
                                         _vp, _vp]),
         "orc_ph_excited_1body": (None, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
         "orc_ph_overlaps": (None, [_i, _i, _i, _i, _i, _vp, _vp, _vp]),
+        "orc_ph_woodbury": (None, [_i, _i, _i, _i, _vp, _vp, _vp, _l, _vp, _vp, _vp]),
         "orc_ph_compact_R": (None, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _vp]),
         "orc_ph_calculate_R": (None, [_i, _i, _vp, _i, _i, _vp, _vp, _vp, _vp, _vp]),
         "orc_system_create": (_vp, [_i, _i, _i, _vp, _vp, _vp, _d, _d]),
@@ -286,6 +287,20 @@ def ph_excited_1body(iexcit, refc, S, R, wgt):
     lib().orc_ph_excited_1body(nw, ndet, nex, nact, nelec, _p(iex), _p(ref), _p(_c(S)),
                                _p(_c(R)), _p(_c(wgt)), _p(E))
     return E
+
+
+def ph_woodbury(A, ref, Phi, ov0=None):
+    """MixedDensityMatrixForWoodbury (density_matrix.hpp:651-717), compact: (G [nb][nel][nmo],
+    ov [nb] accumulated onto ov0, QQ0 [nb][nact][nel])."""
+    nb, nmo, nel = Phi.shape
+    nact = A.shape[0]
+    G = np.zeros((nb, nel, nmo), np.complex128)
+    QQ0 = np.zeros((nb, nact, nel), np.complex128)
+    ov = np.zeros(nb, np.complex128) if ov0 is None else np.array(ov0, np.complex128)
+    r = np.ascontiguousarray(ref, np.int32)
+    lib().orc_ph_woodbury(nb, nmo, nel, nact, _p(_c(A)), _p(r), _p(_c(Phi)), nmo * nel, _p(G),
+                          _p(ov), _p(QQ0))
+    return G, ov, QQ0
 
 
 def ph_overlaps(iexcit, T):

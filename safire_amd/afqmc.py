@@ -78,6 +78,8 @@ SIGNATURES = {
     "afq_ph_excited_2body_energy": (_i, [_i, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp,
                                          _vp, _vp, _vp]),
     "afq_ph_excited_1body_energy": (_i, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "afq_ph_mixed_density_matrix_woodbury": (_i, [_i, _i, _i, _i, _vp, _vp, _vp, _l, _vp, _vp,
+                                                  _vp, _vp, _vp]),
     "afq_ph_overlaps": (_i, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _vp]),
     "afq_ph_compact_R": (_i, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp]),
     "afq_ph_accumulate_R": (_i, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),

@@ -660,6 +660,19 @@ int afq_ph_excited_1body_energy(int nw, int ndet, int nex, int nact, int nelec,
   });
 }
 
+int afq_ph_mixed_density_matrix_woodbury(int nb, int nmo, int nel, int nact, const double* A,
+                                         const int* ref, const double* Phi, long ldw, double* G,
+                                         double* ov, double* QQ0, double* work, void* stream) {
+  return guard([&] {
+    AFQ_CHECK(nb >= 0 && nmo > 0 && nel > 0 && nel <= nact && nact <= nmo &&
+                  ldw >= (long)nmo * nel,
+              AFQ_EBADSHAPE, "Woodbury mixed density matrix: bad sizes");
+    AFQ_CHECK(nb == 0 || (A && ref && Phi && G && ov && QQ0 && work), AFQ_EINVAL,
+              "Woodbury mixed density matrix: null operand");
+    launch_ph_woodbury(S(stream), nb, nmo, nel, nact, A, ref, Phi, ldw, G, ov, QQ0, work);
+  });
+}
+
 int afq_ph_overlaps(int nw, int ndet, int nex, int nact, int nelec, const int* iexcit,
                     const double* T, double* ov, void* stream) {
   return guard([&] {

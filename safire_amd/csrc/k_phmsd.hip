@@ -352,6 +352,38 @@ __global__ __launch_bounds__(256) void k_ph_accumulate_R(int nw, int ndet, int n
   }
 }
 
+// MixedDensityMatrixForWoodbury (density_matrix.hpp:651-717) after the reference determinant's
+// G: QQ0[w][a][e] = sum_k A[a][k] G[w][e][k] (= TAB inv(TNN), since G^T = Phi inv(TNN)); one
+// thread per output entry.
+__global__ __launch_bounds__(256) void k_ph_qq0(int nb, int nmo, int nel, int nact,
+                                                const cd* __restrict__ A,
+                                                const cd* __restrict__ G,
+                                                cd* __restrict__ QQ0) {
+  const long id = (long)blockIdx.x * 256 + threadIdx.x;
+  if (id >= (long)nb * nact * nel) return;
+  const int e = (int)(id % nel);
+  const int a = (int)((id / nel) % nact);
+  const long w = id / ((long)nel * nact);
+  const cd* Ar = A + (long)a * nmo;
+  const cd* Gr = G + (w * nel + e) * nmo;
+  cd acc = cmk(0.0, 0.0);
+  for (int k = 0; k < nmo; ++k) acc += Ar[k] * Gr[k];
+  QQ0[id] = acc;
+}
+
+// rows ref[i] of A [nact][nmo] into Aref [nel][nmo]
+__global__ __launch_bounds__(256) void k_gather_rows(int nel, int nmo, int nact,
+                                                     const int* __restrict__ ref,
+                                                     const cd* __restrict__ A,
+                                                     cd* __restrict__ Aref) {
+  const long id = (long)blockIdx.x * 256 + threadIdx.x;
+  if (id >= (long)nel * nmo) return;
+  const int i = (int)(id / nmo), k = (int)(id % nmo);
+  const int r = ref[i];
+  Aref[id] = (r >= 0 && r < nact) ? A[(long)r * nmo + k]
+                                  : cmk(__builtin_nan(""), __builtin_nan(""));
+}
+
 }  // namespace
 
 void launch_ph_2body(hipStream_t s, int nw, int ndet, int nex, int nact, int nelec, int nchol,
@@ -388,6 +420,23 @@ void launch_ph_1body(hipStream_t s, int nw, int ndet, int nex, int nact, int nel
   hipLaunchKernelGGL(k_ph_1body, dim3(nw), dim3(kPhThreads), 0, s, nw, ndet, nex, nact, nelec,
                      iexcit, refc, reinterpret_cast<const cd*>(S), reinterpret_cast<const cd*>(R),
                      reinterpret_cast<const cd*>(wgt), reinterpret_cast<cd*>(E));
+  AFQ_HIP(hipGetLastError());
+}
+
+void launch_ph_woodbury(hipStream_t s, int nb, int nmo, int nel, int nact, const double* A,
+                        const int* ref, const double* Phi, long ldw, double* G, double* ov,
+                        double* QQ0, double* work) {
+  if (nb <= 0) return;
+  hipLaunchKernelGGL(k_gather_rows, dim3((unsigned)(((long)nel * nmo + 255) / 256)), dim3(256),
+                     0, s, nel, nmo, nact, ref, reinterpret_cast<const cd*>(A),
+                     reinterpret_cast<cd*>(work));
+  AFQ_HIP(hipGetLastError());
+  // T = A[ref] Phi, log det accumulated into ov (wrapped), G = T^-T Phi^T
+  launch_green(s, nb, nmo, nel, work, Phi, ldw, G, ov, 2, 1.0, cd{0.0, 0.0}, nullptr, 0.0, 0.0,
+               nullptr, 0);
+  hipLaunchKernelGGL(k_ph_qq0, dim3((unsigned)(((long)nb * nact * nel + 255) / 256)), dim3(256),
+                     0, s, nb, nmo, nel, nact, reinterpret_cast<const cd*>(A),
+                     reinterpret_cast<const cd*>(G), reinterpret_cast<cd*>(QQ0));
   AFQ_HIP(hipGetLastError());
 }
 

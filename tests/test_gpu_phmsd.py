@@ -173,3 +173,24 @@ def test_ph_calculate_R_matches_oracle():
     torch.cuda.synchronize()
     ref = O.ph_calculate_R(shells, refc, T, wgt)
     np.testing.assert_allclose(R.cpu().numpy(), ref, rtol=1e-10, atol=1e-11 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("nmo,nel,nact,nb", [(20, 5, 12, 7), (64, 10, 64, 33), (48, 16, 30, 5)])
+def test_ph_woodbury_matches_oracle(nmo, nel, nact, nb):
+    rng = np.random.default_rng(nmo + nel)
+    At = rand_c(rng, nact, nmo)
+    ref = np.sort(rng.choice(nact, nel, replace=False))
+    Phi = rand_c(rng, nb, nmo, nel)
+    ov0 = rand_c(rng, nb) * 0.1
+    Ad, rd, Pd = dev(At), i32(ref), dev(Phi)
+    G = torch.zeros((nb, nel, nmo), dtype=torch.complex128, device="cuda")
+    ov = dev(ov0)
+    QQ0 = torch.zeros((nb, nact, nel), dtype=torch.complex128, device="cuda")
+    work = torch.zeros(nel * nmo, dtype=torch.complex128, device="cuda")
+    A.check(A.lib().afq_ph_mixed_density_matrix_woodbury(
+        nb, nmo, nel, nact, A.ptr(Ad), A.ptr(rd), A.ptr(Pd), nmo * nel, A.ptr(G), A.ptr(ov),
+        A.ptr(QQ0), A.ptr(work), A.stream_ptr()))
+    Gr, ovr, Qr = O.ph_woodbury(At, ref, Phi, ov0)
+    np.testing.assert_allclose(G.cpu().numpy(), Gr, rtol=1e-10, atol=1e-12 * np.abs(Gr).max())
+    np.testing.assert_allclose(QQ0.cpu().numpy(), Qr, rtol=1e-10, atol=1e-12 * np.abs(Qr).max())
+    np.testing.assert_allclose(ov.cpu().numpy(), ovr, rtol=1e-10, atol=1e-12)

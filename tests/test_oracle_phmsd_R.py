@@ -95,3 +95,28 @@ def test_calculate_R_is_the_weighted_sum():
                     ref[w, e[p]] += wgt[row, w] * Rc[w, d, p]
             row += 1
     np.testing.assert_allclose(R, ref, rtol=1e-12, atol=1e-12)
+
+
+def test_woodbury_mixed_density_matrix_closed_forms():
+    """MixedDensityMatrixForWoodbury: TNN = (A Phi)[ref rows]; ov accumulates log det TNN
+    (wrapped imaginary part); QQ0 = (A Phi) TNN^-1; compact G = TNN^-T Phi^T. numpy's det, inv
+    and products give the same numbers independently."""
+    rng = np.random.default_rng(11)
+    nb, nmo, nel, nact = 4, 20, 5, 12
+    A = rand_c(rng, nact, nmo)
+    ref = np.array([0, 2, 3, 7, 9])
+    Phi = rand_c(rng, nb, nmo, nel)
+    ov0 = rand_c(rng, nb) * 0.1
+    G, ov, QQ0 = O.ph_woodbury(A, ref, Phi, ov0)
+    for w in range(nb):
+        TAB = A @ Phi[w]
+        TNN = TAB[ref]
+        inv = np.linalg.inv(TNN)
+        np.testing.assert_allclose(QQ0[w], TAB @ inv, rtol=1e-12, atol=1e-12)
+        np.testing.assert_allclose(G[w], inv.T @ Phi[w].T, rtol=1e-12, atol=1e-12)
+        d = ov[w] - ov0[w] - np.log(np.linalg.det(TNN))
+        d = d.real + 1j * ((d.imag + np.pi) % (2 * np.pi) - np.pi)
+        assert abs(d) < 1e-12
+        assert -np.pi <= ov[w].imag <= np.pi
+        # the reference rows of QQ0 are the identity (TNN TNN^-1)
+        np.testing.assert_allclose(QQ0[w][ref], np.eye(nel), atol=1e-12)
