@@ -270,6 +270,14 @@ int afq_ph_mixed_density_matrix_woodbury(int nb, int nmo, int nel, int nact, con
                                          const int* d_ref, const double* d_Phi, long ldw,
                                          double* d_G, double* d_ov, double* d_QQ0,
                                          double* d_work, void* stream);
+/* The spin coupling of the determinant weights (PHMSD.cpp:441-450): C [nu][nd] (CSR: d_cp row
+ * pointers [nu+1], d_ci columns, d_cv c128 CI coefficients) couples up excitation u with down
+ * excitation d; d_wgt[u][w] = ratio_up[u][w] sum_d C[u][d] ratio_dn[d][w] and
+ * d_ov[w] = sum_u d_wgt[u][w] (the multi-determinant overlap over the reference's).
+ * ratios [n][nw] c128 as afq_ph_overlaps writes them (row 0 = the reference, ratio 1). */
+int afq_ph_spin_weights(int nw, int nu, int nd, const int* d_cp, const int* d_ci,
+                        const double* d_cv, const double* d_rup, const double* d_rdn,
+                        double* d_wgt, double* d_ov, void* stream);
 /* calculate_overlaps (phmsd_helpers.hpp:111-195) for the rank-nex excitations of one spin:
  * d_ov[d][w] = det A, A(p, q) = T[w][e_(nex+p)][e_q] with e = iexcit[d]; T [nw][nact][nelec]
  * c128 (the walker's active-space matrix); 0 for a singular A. nex <= 8. */

@@ -1051,6 +1051,23 @@ void orc_ph_woodbury(int nb, int nmo, int nel, int nact, const cd* A, const int*
   }
 }
 
+// PHMSD.cpp:441-450: W = csrmm(C, ratio_dn); Ov = contract(ratio_up, W); wgt = W * ratio_up
+void orc_ph_spin_weights(int nw, int nu, const int* cp, const int* ci, const cd* cv,
+                         const cd* rup, const cd* rdn, cd* wgt, cd* ov) {
+  for (int u = 0; u < nu; ++u)
+    for (int w = 0; w < nw; ++w) {
+      cd acc = 0.0;
+      for (int j = cp[u]; j < cp[u + 1]; ++j) acc += cv[j] * rdn[(long)ci[j] * nw + w];
+      wgt[(long)u * nw + w] = acc;
+    }
+  for (int w = 0; w < nw; ++w) {
+    cd acc = 0.0;
+    for (int u = 0; u < nu; ++u) acc += rup[(long)u * nw + w] * wgt[(long)u * nw + w];
+    ov[w] = acc;
+  }
+  for (long i = 0; i < (long)nu * nw; ++i) wgt[i] *= rup[i];
+}
+
 // calculate_overlaps (phmsd_helpers.hpp:111-195), one shell: ov[d][w] = det A
 void orc_ph_overlaps(int nw, int ndet, int nex, int nact, int nelec, const int* iexcit,
                      const cd* T, cd* ov) {

@@ -73,6 +73,7 @@ def _declare(L):
                                         _vp, _vp]),
         "orc_ph_excited_1body": (None, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
         "orc_ph_overlaps": (None, [_i, _i, _i, _i, _i, _vp, _vp, _vp]),
+        "orc_ph_spin_weights": (None, [_i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
         "orc_ph_woodbury": (None, [_i, _i, _i, _i, _vp, _vp, _vp, _l, _vp, _vp, _vp]),
         "orc_ph_compact_R": (None, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _vp]),
         "orc_ph_calculate_R": (None, [_i, _i, _vp, _i, _i, _vp, _vp, _vp, _vp, _vp]),
@@ -301,6 +302,17 @@ def ph_woodbury(A, ref, Phi, ov0=None):
     lib().orc_ph_woodbury(nb, nmo, nel, nact, _p(_c(A)), _p(r), _p(_c(Phi)), nmo * nel, _p(G),
                           _p(ov), _p(QQ0))
     return G, ov, QQ0
+
+
+def ph_spin_weights(cp, ci, cv, rup, rdn):
+    """PHMSD.cpp:441-450: (wgt [nu][nw], Ov [nw]) from the CSR CI couplings."""
+    nu, nw = rup.shape
+    cp_, ci_ = np.ascontiguousarray(cp, np.int32), np.ascontiguousarray(ci, np.int32)
+    wgt = np.zeros((nu, nw), np.complex128)
+    ov = np.zeros(nw, np.complex128)
+    lib().orc_ph_spin_weights(nw, nu, _p(cp_), _p(ci_), _p(_c(cv)), _p(_c(rup)), _p(_c(rdn)),
+                              _p(wgt), _p(ov))
+    return wgt, ov
 
 
 def ph_overlaps(iexcit, T):

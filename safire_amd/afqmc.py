@@ -81,6 +81,7 @@ SIGNATURES = {
     "afq_ph_mixed_density_matrix_woodbury": (_i, [_i, _i, _i, _i, _vp, _vp, _vp, _l, _vp, _vp,
                                                   _vp, _vp, _vp]),
     "afq_ph_overlaps": (_i, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _vp]),
+    "afq_ph_spin_weights": (_i, [_i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "afq_ph_compact_R": (_i, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp]),
     "afq_ph_accumulate_R": (_i, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "afq_split_seed": (C.c_ulonglong, [_i, _i, C.c_uint]),

@@ -673,6 +673,15 @@ int afq_ph_mixed_density_matrix_woodbury(int nb, int nmo, int nel, int nact, con
   });
 }
 
+int afq_ph_spin_weights(int nw, int nu, int nd, const int* cp, const int* ci, const double* cv,
+                        const double* rup, const double* rdn, double* wgt, double* ov,
+                        void* stream) {
+  return guard([&] {
+    AFQ_CHECK(nw >= 0 && nu >= 0 && nd >= 0, AFQ_EBADSHAPE, "ph spin weights: bad sizes");
+    launch_ph_spin_weights(S(stream), nw, nu, cp, ci, cv, rup, rdn, wgt, ov);
+  });
+}
+
 int afq_ph_overlaps(int nw, int ndet, int nex, int nact, int nelec, const int* iexcit,
                     const double* T, double* ov, void* stream) {
   return guard([&] {

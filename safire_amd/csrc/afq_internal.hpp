@@ -258,6 +258,9 @@ void launch_ph_1body(hipStream_t s, int nw, int ndet, int nex, int nact, int nel
 void launch_ph_woodbury(hipStream_t s, int nb, int nmo, int nel, int nact, const double* A,
                         const int* ref, const double* Phi, long ldw, double* G, double* ov,
                         double* QQ0, double* work);
+void launch_ph_spin_weights(hipStream_t s, int nw, int nu, const int* cp, const int* ci,
+                            const double* cv, const double* rup, const double* rdn, double* wgt,
+                            double* ov);
 void launch_ph_overlaps(hipStream_t s, int nw, int ndet, int nex, int nact, int nelec,
                         const int* iexcit, const double* T, double* ov);
 void launch_ph_compact_R(hipStream_t s, int nw, int ndet, int nex, int nact, int nelec,

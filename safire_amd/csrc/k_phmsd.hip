@@ -384,6 +384,33 @@ __global__ __launch_bounds__(256) void k_gather_rows(int nel, int nmo, int nact,
                                   : cmk(__builtin_nan(""), __builtin_nan(""));
 }
 
+// The spin coupling of the determinant weights (PHMSD.cpp:441-450): with the CI couplings
+// C [nu][nd] in CSR form (row pointers cp, columns ci, values cv),
+//   wgt[u][w] = ratio_up[u][w] sum_{d in row u} C[u][d] ratio_dn[d][w]   (csrmm, then the product)
+//   Ov[w]     = sum_u wgt[u][w]                                          (contract "uw,uw->w")
+// One workgroup per walker; each thread takes a fixed set of rows u and the workgroup sums the
+// partial overlaps in a fixed order.
+__global__ __launch_bounds__(256) void k_ph_spin_weights(int nw, int nu, const int* __restrict__ cp,
+                                                         const int* __restrict__ ci,
+                                                         const cd* __restrict__ cv,
+                                                         const cd* __restrict__ rup,
+                                                         const cd* __restrict__ rdn,
+                                                         cd* __restrict__ wgt,
+                                                         cd* __restrict__ ov) {
+  __shared__ cd red[kPhThreads / 64];
+  const int w = blockIdx.x;
+  cd part = cmk(0.0, 0.0);
+  for (int u = threadIdx.x; u < nu; u += kPhThreads) {
+    cd acc = cmk(0.0, 0.0);
+    for (int j = cp[u]; j < cp[u + 1]; ++j) acc += cv[j] * rdn[(long)ci[j] * nw + w];
+    const cd x = acc * rup[(long)u * nw + w];
+    wgt[(long)u * nw + w] = x;
+    part += x;
+  }
+  const cd sum = block_sum(part, red);
+  if (threadIdx.x == 0) ov[w] = sum;
+}
+
 }  // namespace
 
 void launch_ph_2body(hipStream_t s, int nw, int ndet, int nex, int nact, int nelec, int nchol,
@@ -437,6 +464,17 @@ void launch_ph_woodbury(hipStream_t s, int nb, int nmo, int nel, int nact, const
   hipLaunchKernelGGL(k_ph_qq0, dim3((unsigned)(((long)nb * nact * nel + 255) / 256)), dim3(256),
                      0, s, nb, nmo, nel, nact, reinterpret_cast<const cd*>(A),
                      reinterpret_cast<const cd*>(G), reinterpret_cast<cd*>(QQ0));
+  AFQ_HIP(hipGetLastError());
+}
+
+void launch_ph_spin_weights(hipStream_t s, int nw, int nu, const int* cp, const int* ci,
+                            const double* cv, const double* rup, const double* rdn, double* wgt,
+                            double* ov) {
+  if (nw <= 0) return;
+  hipLaunchKernelGGL(k_ph_spin_weights, dim3(nw), dim3(kPhThreads), 0, s, nw, nu, cp, ci,
+                     reinterpret_cast<const cd*>(cv), reinterpret_cast<const cd*>(rup),
+                     reinterpret_cast<const cd*>(rdn), reinterpret_cast<cd*>(wgt),
+                     reinterpret_cast<cd*>(ov));
   AFQ_HIP(hipGetLastError());
 }
 

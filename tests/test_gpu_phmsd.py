@@ -194,3 +194,20 @@ def test_ph_woodbury_matches_oracle(nmo, nel, nact, nb):
     np.testing.assert_allclose(G.cpu().numpy(), Gr, rtol=1e-10, atol=1e-12 * np.abs(Gr).max())
     np.testing.assert_allclose(QQ0.cpu().numpy(), Qr, rtol=1e-10, atol=1e-12 * np.abs(Qr).max())
     np.testing.assert_allclose(ov.cpu().numpy(), ovr, rtol=1e-10, atol=1e-12)
+
+
+@pytest.mark.parametrize("nu,nd,nw", [(1, 1, 1), (23, 17, 9), (300, 260, 70)])
+def test_ph_spin_weights_match_oracle(nu, nd, nw):
+    from tests.test_oracle_phmsd_R import random_csr
+    rng = np.random.default_rng(nu + nw)
+    C, cp, ci, cv = random_csr(rng, nu, nd)
+    rup, rdn = rand_c(rng, nu, nw), rand_c(rng, nd, nw)
+    cpd, cid, cvd, rud, rdd = i32(cp), i32(ci), dev(cv), dev(rup), dev(rdn)
+    wgt = torch.empty((nu, nw), dtype=torch.complex128, device="cuda")
+    ov = torch.empty(nw, dtype=torch.complex128, device="cuda")
+    A.check(A.lib().afq_ph_spin_weights(nw, nu, nd, A.ptr(cpd), A.ptr(cid), A.ptr(cvd),
+                                        A.ptr(rud), A.ptr(rdd), A.ptr(wgt), A.ptr(ov),
+                                        A.stream_ptr()))
+    wr, ovr = O.ph_spin_weights(cp, ci, cv, rup, rdn)
+    np.testing.assert_allclose(wgt.cpu().numpy(), wr, rtol=1e-12, atol=1e-14 * np.abs(wr).max())
+    np.testing.assert_allclose(ov.cpu().numpy(), ovr, rtol=1e-11, atol=1e-13 * np.abs(ovr).max())

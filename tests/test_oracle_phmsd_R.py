@@ -120,3 +120,39 @@ def test_woodbury_mixed_density_matrix_closed_forms():
         assert -np.pi <= ov[w].imag <= np.pi
         # the reference rows of QQ0 are the identity (TNN TNN^-1)
         np.testing.assert_allclose(QQ0[w][ref], np.eye(nel), atol=1e-12)
+
+
+def random_csr(rng, nu, nd, density=0.3):
+    """A CSR matrix [nu][nd] with random complex values, some empty rows, and a dense twin."""
+    C = np.zeros((nu, nd), complex)
+    for u in range(nu):
+        if u % 5 == 4:
+            continue                                   # an up excitation with no partner
+        cols = rng.choice(nd, max(1, int(density * nd)), replace=False)
+        C[u, cols] = rng.standard_normal(len(cols)) + 1j * rng.standard_normal(len(cols))
+    cp = np.zeros(nu + 1, np.int32)
+    ci, cv = [], []
+    for u in range(nu):
+        nz = np.nonzero(C[u])[0]
+        ci += list(nz)
+        cv += list(C[u, nz])
+        cp[u + 1] = len(ci)
+    return C, cp, np.array(ci, np.int32), np.array(cv, complex)
+
+
+def test_spin_weights_is_the_dense_product():
+    """PHMSD.cpp:441-450: wgt = ratio_up * (C ratio_dn), Ov = sum over up excitations of wgt;
+    pinned by numpy's dense product and, for a single coupling, by the closed form c r_u r_d."""
+    rng = np.random.default_rng(11)
+    nu, nd, nw = 23, 17, 9
+    C, cp, ci, cv = random_csr(rng, nu, nd)
+    rup, rdn = rand_c(rng, nu, nw), rand_c(rng, nd, nw)
+    wgt, ov = O.ph_spin_weights(cp, ci, cv, rup, rdn)
+    W = rup * (C @ rdn)
+    np.testing.assert_allclose(wgt, W, rtol=1e-13, atol=1e-14)
+    np.testing.assert_allclose(ov, W.sum(axis=0), rtol=1e-12)
+    assert np.all(wgt[4::5] == 0)
+    # one determinant pair: the multi-determinant overlap is c * ratio_up * ratio_dn
+    wgt1, ov1 = O.ph_spin_weights(np.array([0, 1], np.int32), np.array([0], np.int32),
+                                  np.array([0.5 - 2j]), rup[:1], rdn[:1])
+    np.testing.assert_allclose(ov1, (0.5 - 2j) * rup[0] * rdn[0], rtol=1e-15)
