@@ -28,6 +28,7 @@ extern "C" {
 
 typedef struct afq_ham_s* afq_ham;
 typedef struct afq_rng_t* afq_rng;
+typedef struct afq_ph_t* afq_ph;
 
 enum afq_status {
   AFQ_OK = 0,
@@ -270,6 +271,36 @@ int afq_ph_mixed_density_matrix_woodbury(int nb, int nmo, int nel, int nact, con
                                          const int* d_ref, const double* d_Phi, long ldw,
                                          double* d_G, double* d_ov, double* d_QQ0,
                                          double* d_work, void* stream);
+/* ph_reference_energy / ph_ref_energy_impl (Real3IndexFactorization.hpp:212-233, 598-696) for
+ * one spin: with G [nw][nel][nmo] c128 (the reference determinant's compact G), Lna
+ * [nchol][nact][nmo] c128 and haj [nact][nmo] c128 (NULL: no one-body term),
+ * d_Twina [nw][nel][nchol][nact] = G Lnaᵀ and d_Swia [nw][nel][nact] = G hajᵀ are written, and
+ * d_E [nw][3] c128 = (E0 + sum_i Swia[i][i], -scl/2 sum_{ijn} T[i][n][j] T[j][n][i],
+ * 1/2 sum_n Kl[n]^2) with d_Kl [nw][nchol] = sum_i T[i][n][i]. The reference determinant
+ * occupies active orbitals 0..nel-1, as the reference assumes. */
+int afq_ph_reference_energy(int nw, int nel, int nmo, int nchol, int nact, double scl, double E0,
+                            const double* d_G, const double* d_Lna, const double* d_haj,
+                            double* d_E, double* d_Kl, double* d_Twina, double* d_Swia,
+                            void* stream);
+/* PHMSD trial over COLLINEAR walkers (Wavefunction.cpp:341-497 phmsd_from_params with
+ * readWfn.cpp:218-340 build_ph_struct; PHMSD.hpp). Host inputs: occs [ndet][nup + ndn] int, the
+ * sorted occupied orbitals of each determinant (up in [0, nmo), down in [nmo, 2 nmo)); determinant
+ * 0 is the reference and must occupy orbitals 0..nup-1 and nmo..nmo+ndn-1. ci [ndet] c128.
+ * orbmats [nref][nmo][nmo] c128: the trial orbitals as rows (PsiT_MO; nref 1 = one set for both
+ * spins, 2 = one per spin); every orbital is active. h1 [nmo][nmo] f64, likn [nmo*nmo][nchol] f64
+ * (the Cholesky vectors, as afq_ham_create takes them), E0 the constant energy. walker_block:
+ * walkers per energy batch (the reference's nwalk_block_size; bounds the workspace). */
+int afq_ph_create(int nmo, int nup, int ndn, int nchol, int ndet, const int* occs,
+                  const double* ci, int nref, const double* orbmats, const double* h1,
+                  const double* likn, double E0, int walker_block, afq_ph* out);
+void afq_ph_destroy(afq_ph ph);
+/* Unique excitations per spin (the reference counted as index 0) and non-empty ranks. */
+int afq_ph_info(afq_ph ph, int* nunique, int* nshells);
+/* PHMSD::energy / energy_alg1 (PHMSD.cpp:377-544): d_E [nw][3] c128 (E1, EXX, EJ of the
+ * multi-determinant local energy) and d_ov [nw] c128 = log sum_D conj(c_D) <D|phi_w>. Walkers:
+ * rows of ldw c128 holding Phi_up [nmo][nup] then Phi_dn [nmo][ndn] (the COLLINEAR WalkerSet). */
+int afq_ph_energy(afq_ph ph, int nw, const double* d_wbuf, long ldw, double* d_E, double* d_ov,
+                  void* stream);
 /* The spin coupling of the determinant weights (PHMSD.cpp:441-450): C [nu][nd] (CSR: d_cp row
  * pointers [nu+1], d_ci columns, d_cv c128 CI coefficients) couples up excitation u with down
  * excitation d; d_wgt[u][w] = ratio_up[u][w] sum_d C[u][d] ratio_dn[d][w] and

@@ -98,7 +98,9 @@ def _declare(L):
 
 
 def _p(a):
-    return None if a is None else a.ctypes.data
+    # the returned pointer object holds a reference to the array, so a temporary made by _c()
+    # inside the call's argument list lives until the call returns
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
 
 
 def _c(a, dt=np.complex128):

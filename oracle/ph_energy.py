@@ -1,0 +1,194 @@
+"""PHMSD local energy, restated on the CPU -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/ import this module, as the checker of the GPU path; the product library never loads
+it. It composes the oracle's PHMSD pieces (liborc.so through oracle.py) the way the reference
+composes them, and restates in numpy the parts that are plain array algebra:
+
+* build_ph_struct: readWfn.cpp:218-340 (unique excitations per spin and rank, the CI sign of the
+  in-place excitation order, configuration indices) and Wavefunction.cpp:371-470 (active-space
+  indices, the CI couplings conj(ci) in CSR form, both directions).
+* ph_reference_energy: Real3IndexFactorization.hpp:212-233 and ph_ref_energy_impl 598-696.
+* energy_alg1: PHMSD.cpp:377-544 (COLLINEAR walkers).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import oracle as O
+
+
+def excitation(ref, confg):
+    """get_excitation_number (Excitations.hpp:148-192): (holes as positions in ref, particles as
+    orbitals in confg order, sign of sorting ref with the holes replaced in place)."""
+    ref = list(ref)
+    confg = list(confg)
+    holes = [i for i, o in enumerate(ref) if o not in confg]
+    parts = [o for o in confg if o not in ref]
+    work = list(ref)
+    for h, p in zip(holes, parts):
+        work[h] = p
+    sign = 1
+    n = len(work)
+    for i in range(n):                       # the reference's exchange count (bubble order)
+        for j in range(i + 1, n):
+            if work[j] < work[i]:
+                sign = -sign
+                work[i], work[j] = work[j], work[i]
+    return holes, parts, sign
+
+
+def build_ph_struct(ci, occs, nmo, nup, ndn):
+    """The PHMSD structures of a CI expansion. occs [ndet][nup + ndn]: sorted occupied orbitals,
+    up in [0, nmo), down in [nmo, 2 nmo); determinant 0 is the reference.
+
+    Returns dict with
+      refc:   [refc_up, refc_dn] active indices of the reference orbitals
+      shells: [spin][n - 1] = int32 [ndet_n][2 n] unique rank-n excitations (positions, then
+              active orbitals), in order of first appearance
+      nunique: [nu, nd] (the reference is index 0 of each spin)
+      C:      [C_ud (nu x nd), C_du (nd x nu)] as dense complex arrays (conj of the signed CI
+              coefficients), and csr: [(cp, ci, cv), (cp, ci, cv)] of the same.
+    """
+    ci = np.asarray(ci, np.complex128)
+    occs = np.asarray(occs)
+    ndet = len(ci)
+    refs = [list(occs[0, :nup]), list(occs[0, nup:nup + ndn])]
+    nel = [nup, ndn]
+    unique = [[[] for _ in range(nel[s] + 1)] for s in range(2)]
+    configs = []
+    for i in range(ndet):
+        c = ci[i]
+        idx = []
+        for s in range(2):
+            conf = list(occs[i, :nup]) if s == 0 else list(occs[i, nup:nup + ndn])
+            holes, parts, sign = excitation(refs[s], conf)
+            c = c * sign
+            n = len(holes)
+            if n == 0:
+                idx.append((0, 0))
+                continue
+            key = tuple(holes) + tuple(parts)
+            if key not in unique[s][n]:
+                unique[s][n].append(key)
+            idx.append((n, unique[s][n].index(key)))
+        configs.append((idx, c))
+    # active-space indices: every orbital of the basis is active (Wavefunction.cpp:371-408 maps
+    # orbital i and i + nmo to the same index i)
+    shells, offsets, nunique = [], [], []
+    for s in range(2):
+        sh, off, tot = [], [0] * (nel[s] + 1), 1
+        for n in range(1, nel[s] + 1):
+            off[n] = tot
+            lst = [list(k[:n]) + [o - s * nmo for o in k[n:]] for k in unique[s][n]]
+            sh.append(np.array(lst, np.int32).reshape(len(lst), 2 * n))
+            tot += len(lst)
+        shells.append(sh)
+        offsets.append(off)
+        nunique.append(tot)
+    C_ud = np.zeros((nunique[0], nunique[1]), np.complex128)
+    entries = [[], []]
+    for (iu, idn), c in configs:
+        u = 0 if iu[0] == 0 else offsets[0][iu[0]] + iu[1]
+        d = 0 if idn[0] == 0 else offsets[1][idn[0]] + idn[1]
+        entries[0].append((u, d, np.conj(c)))
+        entries[1].append((d, u, np.conj(c)))
+        C_ud[u, d] += np.conj(c)
+    csr = []
+    for s, rows in enumerate(nunique):
+        ent = sorted(entries[s], key=lambda e: (e[0], e[1]))
+        cp = np.zeros(rows + 1, np.int32)
+        for r, _, _ in ent:
+            cp[r + 1] += 1
+        cp = np.cumsum(cp).astype(np.int32)
+        csr.append((cp, np.array([e[1] for e in ent], np.int32),
+                    np.array([e[2] for e in ent], np.complex128)))
+    refc = [np.array(refs[0], np.int32), np.array([o - nmo for o in refs[1]], np.int32)]
+    return {"refc": refc, "shells": shells, "nunique": nunique, "C": [C_ud, C_ud.T.copy()],
+            "csr": csr}
+
+
+def ph_reference_energy(G, Lna, haj, E0=0.0, scl=1.0):
+    """ph_reference_energy / ph_ref_energy_impl (Real3IndexFactorization.hpp:212-233, 598-696):
+    G [nw][nel][nmo] (the reference determinant's compact G), Lna [nchol][nact][nmo],
+    haj [nact][nmo] or None (no one-body term). Returns (E [nw][3], Kl [nw][nchol],
+    Twina [nw][nel][nchol][nact], Swia [nw][nel][nact] or None). Assumes refc = 0..nel-1, as the
+    reference does (E1 takes the diagonal of Swia, the exchange the first nel columns)."""
+    nw, nel, _ = G.shape
+    E = np.zeros((nw, 3), np.complex128)
+    Swia = None
+    if haj is not None:
+        E[:, 0] = E0
+        Swia = np.einsum("wik,ak->wia", G, haj, optimize=True)
+        E[:, 0] += np.einsum("wii->w", Swia[:, :, :nel])
+    Twina = np.einsum("wik,nak->wina", G, Lna, optimize=True)
+    Tn = Twina[:, :, :, :nel]
+    E[:, 1] = -0.5 * scl * np.einsum("winj,wjni->w", Tn, Tn, optimize=True)
+    Kl = np.einsum("wini->wn", Tn)
+    E[:, 2] = 0.5 * np.einsum("wn,wn->w", Kl, Kl)
+    return E, Kl, Twina, Swia
+
+
+def _shell_overlaps(struct, s, QQ0, nw):
+    ratios = np.zeros((struct["nunique"][s], nw), np.complex128)
+    ratios[0] = 1.0
+    row = 1
+    for sh in struct["shells"][s]:
+        if len(sh):
+            ratios[row:row + len(sh)] = O.ph_overlaps(sh, QQ0)
+        row += len(sh)
+    return ratios
+
+
+def _excited(struct, s, wgt, QQ0, Twina, Swia, KE):
+    """ph_excited_energies_{first,second}_step (phmsd_helpers.hpp:408-523) without the
+    determinant blocks (numerically irrelevant): E planar [3][nw] and KE rows per determinant."""
+    nw = QQ0.shape[0]
+    E = np.zeros((3, nw), np.complex128)
+    refc = struct["refc"][s]
+    row = 1
+    for sh in struct["shells"][s]:
+        nd = len(sh)
+        if nd:
+            R = O.ph_compact_R(sh, refc, QQ0)
+            w = np.ascontiguousarray(wgt[row:row + nd])
+            E[0] += O.ph_excited_1body(sh, refc, Swia, R, w)
+            EX, EJ, K = O.ph_excited_2body(sh, refc, Twina, R, w)
+            E[1] += EX
+            E[2] += EJ
+            KE[row:row + nd] = K
+        row += nd
+    return E
+
+
+def energy_alg1(struct, A, Lna, haj, E0, Phi_up, Phi_dn):
+    """PHMSD::energy_alg1 (PHMSD.cpp:377-544) for COLLINEAR walkers. A[s] [nact][nmo] (the
+    trial orbitals as rows), Lna[s] [nchol][nact][nmo], haj[s] [nact][nmo], Phi_s [nw][nmo][nel_s].
+    Returns (E [nw][3], log overlap [nw])."""
+    nw = Phi_up.shape[0]
+    nchol = Lna[0].shape[0]
+    nu, nd = struct["nunique"]
+    refc = struct["refc"]
+    G0, log_ov, Q0 = O.ph_woodbury(A[0], refc[0], Phi_up)
+    G1, log_ov, Q1 = O.ph_woodbury(A[1], refc[1], Phi_dn, log_ov)
+    r_up = _shell_overlaps(struct, 0, Q0, nw)
+    r_dn = _shell_overlaps(struct, 1, Q1, nw)
+    Eacc = np.zeros((nw, 3), np.complex128)
+    # spin up
+    wgt, Ov = O.ph_spin_weights(*struct["csr"][0], r_up, r_dn)
+    Eref, Kl, Twina, Swia = ph_reference_energy(G0, Lna[0], haj[0], E0)
+    Eacc += Eref * Ov[:, None]
+    KE = np.zeros((max(nu, nd), nw, nchol), np.complex128)
+    KE[0] = Kl
+    Eacc += _excited(struct, 0, wgt, Q0, Twina, Swia, KE).T
+    Tdn = KE[:nu] * r_up[:, :, None]
+    KEr = np.einsum("du,uwn->dwn", struct["C"][1], Tdn, optimize=True) * r_dn[:, :, None]
+    # spin down
+    wgt1, _ = O.ph_spin_weights(*struct["csr"][1], r_dn, r_up)
+    Eref, KEl, Twina, Swia = ph_reference_energy(G1, Lna[1], haj[1], 0.0)
+    Eacc += Eref * Ov[:, None]
+    Eacc[:, 2] += np.einsum("wn,wn->w", KEr[0], KEl)
+    KEd = np.zeros((nd, nw, nchol), np.complex128)
+    Ed = _excited(struct, 1, wgt1, Q1, Twina, Swia, KEd)
+    Ed[2] += np.einsum("dwn,dwn->w", KEr[1:nd], KEd[1:nd])
+    Eacc += Ed.T
+    return Eacc / Ov[:, None], np.log(Ov) + log_ov

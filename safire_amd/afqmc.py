@@ -82,6 +82,12 @@ SIGNATURES = {
                                                   _vp, _vp, _vp]),
     "afq_ph_overlaps": (_i, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _vp]),
     "afq_ph_spin_weights": (_i, [_i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "afq_ph_reference_energy": (_i, [_i, _i, _i, _i, _i, _d, _d, _vp, _vp, _vp, _vp, _vp, _vp,
+                                     _vp, _vp]),
+    "afq_ph_create": (_i, [_i, _i, _i, _i, _i, _vp, _vp, _i, _vp, _vp, _vp, _d, _i, _vp]),
+    "afq_ph_destroy": (None, [_vp]),
+    "afq_ph_info": (_i, [_vp, _vp, _vp]),
+    "afq_ph_energy": (_i, [_vp, _i, _vp, _l, _vp, _vp, _vp]),
     "afq_ph_compact_R": (_i, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp]),
     "afq_ph_accumulate_R": (_i, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "afq_split_seed": (C.c_ulonglong, [_i, _i, C.c_uint]),
@@ -236,6 +242,59 @@ class HamiltonianOperations:
 
     def energy(self, G, E, stream=None):
         check(lib().afq_energy(self.handle, G.shape[0], ptr(G), ptr(E), stream_ptr(stream)))
+
+
+class PHMSD:
+    """Particle-hole multi-determinant trial over COLLINEAR walkers (PHMSD.hpp; built as
+    phmsd_from_params does, Wavefunction.cpp:341-497). occs [ndet][nup + ndn]: sorted occupied
+    orbitals (down spin offset by nmo), determinant 0 the reference; ci [ndet]; orbmats
+    [nref][nmo][nmo]: trial orbitals as rows; h [nmo][nmo]; Likn [nmo][nmo][nchol]."""
+
+    def __init__(self, occs, ci, orbmats, h, Likn, E0=0.0, walker_block=32):
+        _torch()
+        self.occs = np.ascontiguousarray(occs, np.int32)
+        self.ci = np.ascontiguousarray(ci, np.complex128)
+        self.orbmats = np.ascontiguousarray(orbmats, np.complex128)
+        if self.orbmats.ndim == 2:
+            self.orbmats = self.orbmats[None]
+        self.h = np.ascontiguousarray(h, np.float64)
+        self.Likn = np.ascontiguousarray(Likn, np.float64)
+        self.nmo = self.h.shape[0]
+        self.nchol = self.Likn.shape[-1]
+        self.nup = int(np.sum(self.occs[0] < self.nmo))
+        self.ndn = self.occs.shape[1] - self.nup
+        handle = C.c_void_p()
+        check(lib().afq_ph_create(self.nmo, self.nup, self.ndn, self.nchol, len(self.ci),
+                                  self.occs.ctypes.data, self.ci.ctypes.data,
+                                  self.orbmats.shape[0], self.orbmats.ctypes.data,
+                                  self.h.ctypes.data, self.Likn.ctypes.data, float(E0),
+                                  int(walker_block), C.byref(handle)))
+        self.handle = handle
+        nu, ns = (C.c_int * 2)(), (C.c_int * 2)()
+        check(lib().afq_ph_info(self.handle, nu, ns))
+        self.nunique = (nu[0], nu[1])
+
+    def close(self):
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            lib().afq_ph_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def energy(self, wbuf, E=None, ov=None, stream=None):
+        """PHMSD::energy (energy_alg1, PHMSD.cpp:377-544) of the walkers in wbuf [W][ldw]
+        (c128, Phi_up [nmo][nup] then Phi_dn [nmo][ndn] per row): (E [W][3], log overlap [W])."""
+        torch = _torch()
+        W = wbuf.shape[0]
+        E = torch.empty((W, 3), dtype=torch.complex128, device=wbuf.device) if E is None else E
+        ov = torch.empty(W, dtype=torch.complex128, device=wbuf.device) if ov is None else ov
+        check(lib().afq_ph_energy(self.handle, W, ptr(wbuf), wbuf.shape[1], ptr(E), ptr(ov),
+                                  stream_ptr(stream)))
+        return E, ov
 
 
 class WalkerSet:

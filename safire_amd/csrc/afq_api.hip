@@ -673,6 +673,19 @@ int afq_ph_mixed_density_matrix_woodbury(int nb, int nmo, int nel, int nact, con
   });
 }
 
+int afq_ph_reference_energy(int nw, int nel, int nmo, int nchol, int nact, double scl, double E0,
+                            const double* G, const double* Lna, const double* haj, double* E,
+                            double* Kl, double* Twina, double* Swia, void* stream) {
+  return guard([&] {
+    AFQ_CHECK(nw >= 0 && nel > 0 && nmo > 0 && nchol > 0 && nact >= nel, AFQ_EBADSHAPE,
+              "ph reference energy: bad sizes (needs nact >= nel)");
+    AFQ_CHECK(G && Lna && E && Kl && Twina && (!haj || Swia), AFQ_EINVAL,
+              "ph reference energy: null operand");
+    launch_ph_reference_energy(S(stream), nw, nel, nmo, nchol, nact, scl, E0, G, Lna, haj, E, Kl,
+                               Twina, Swia);
+  });
+}
+
 int afq_ph_spin_weights(int nw, int nu, int nd, const int* cp, const int* ci, const double* cv,
                         const double* rup, const double* rdn, double* wgt, double* ov,
                         void* stream) {

@@ -265,6 +265,20 @@ void launch_ph_overlaps(hipStream_t s, int nw, int ndet, int nex, int nact, int 
                         const int* iexcit, const double* T, double* ov);
 void launch_ph_compact_R(hipStream_t s, int nw, int ndet, int nex, int nact, int nelec,
                          const int* iexcit, const int* refc, const double* T, double* R);
+void launch_ph_reference_energy(hipStream_t s, int nw, int nel, int nmo, int nchol, int nact,
+                                double scl, double E0, const double* G, const double* Lna,
+                                const double* haj, double* E, double* Kl, double* Twina,
+                                double* Swia);
+void launch_ph_couple_ke(hipStream_t s, int nw, int nchol, int nout, const int* cp, const int* ci,
+                         const double* cv, const double* rin, const double* KEin,
+                         const double* rout, double* KEout);
+void launch_ph_ke_dot(hipStream_t s, int nw, int nchol, int nd, const double* a, const double* b,
+                      double* out, int ostride);
+void launch_ph_accumulate(hipStream_t s, int nw, const double* Eref, const double* Ov,
+                          const double* Ex, double* acc);
+void launch_ph_finalize(hipStream_t s, int nw, const double* acc, const double* Ov,
+                        const double* log_ov, double* E, double* ov_out);
+void launch_ph_fill(hipStream_t s, long n, double re, double im, double* x);
 void launch_ph_accumulate_R(hipStream_t s, int nw, int ndet, int nex, int nact, int nelec,
                             const int* iexcit, const int* refc, const double* Rc,
                             const double* ov, const double* wgt, double* R);

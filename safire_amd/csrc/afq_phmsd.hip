@@ -1,0 +1,408 @@
+// The particle-hole multi-determinant (PHMSD) trial over COLLINEAR walkers: its construction from
+// a CI expansion and its local energy, on the C ABI.
+//
+//   afq_ph_create  = phmsd_from_params (Wavefunction.cpp:341-497) with build_ph_struct
+//                    (readWfn.cpp:218-340): the reference is determinant 0; every other
+//                    determinant is written per spin as an excitation of it (positions of the
+//                    excited reference orbitals, then the orbitals they go to, in the order the
+//                    determinant lists them), unique per spin and rank in order of first
+//                    appearance, with the CI coefficient's sign fixed for the in-place order;
+//                    every orbital of the basis is active (the counts of find_active_space,
+//                    Excitations.hpp:71-102, are never negative), so orbital i and i + nmo map
+//                    to active index i; the couplings conj(ci) go into CSR matrices [nu][nd] and
+//                    [nd][nu] (Wavefunction.cpp:440-466). The half-rotated tensors of each spin
+//                    (haj = A h, Lna_n = A L_n, A the trial orbitals as rows) are built on the GPU.
+//   afq_ph_energy  = PHMSD::energy_alg1 (PHMSD.cpp:377-544), in walker blocks (the reference's
+//                    nwalk_block_size) and determinant blocks (ndet_block_size), both
+//                    numerically irrelevant: every kernel is the GPU form of one reference call
+//                    (k_phmsd.hip), in the reference's order.
+#include "afq_internal.hpp"
+
+#include <algorithm>
+#include <memory>
+#include <new>
+#include <vector>
+
+namespace afq {
+
+void launch_half_rotate(hipStream_t s, int M, int N, int K, const double* PsiT, const double* h,
+                        const double* Likn, double* haj, double* Lnak);
+
+namespace {
+
+constexpr int kPhDetBlock = 256;   // determinants per R block
+
+struct Shell {
+  int nex = 0, ndet = 0, row = 0;  // rank, unique excitations, first row in the ratio arrays
+  int* iexcit = nullptr;           // device [ndet][2 nex]
+};
+
+struct Spin {
+  int nel = 0, nunique = 1;
+  double* A = nullptr;             // c128 [nact][nmo]
+  int* refc = nullptr;             // [nel]
+  double* Lna = nullptr;           // c128 [nchol][nact][nmo]
+  double* haj = nullptr;           // c128 [nact][nmo]
+  std::vector<Shell> shells;
+  int *cp = nullptr, *ci = nullptr;  // couplings of this spin's excitations with the other's
+  double* cv = nullptr;
+};
+
+template <typename T>
+T* dev_alloc(std::vector<void*>& owned, size_t n) {
+  void* p = nullptr;
+  AFQ_HIP(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+  owned.push_back(p);
+  return static_cast<T*>(p);
+}
+
+template <typename T>
+T* dev_copy(std::vector<void*>& owned, const T* src, size_t n) {
+  T* p = dev_alloc<T>(owned, n);
+  if (n) AFQ_HIP(hipMemcpy(p, src, n * sizeof(T), hipMemcpyHostToDevice));
+  return p;
+}
+
+// get_excitation_number (Excitations.hpp:148-192) for sorted ref and conf: positions of the
+// reference orbitals missing from conf, the orbitals of conf missing from ref, and the sign of
+// sorting ref with the first replaced in place by the second (exchange count of a bubble pass).
+int excitation_of(const std::vector<int>& ref, const std::vector<int>& conf, std::vector<int>& ex) {
+  ex.clear();
+  std::vector<int> parts, work(ref);
+  for (size_t i = 0; i < ref.size(); ++i)
+    if (!std::binary_search(conf.begin(), conf.end(), ref[i])) ex.push_back((int)i);
+  for (int o : conf)
+    if (!std::binary_search(ref.begin(), ref.end(), o)) parts.push_back(o);
+  AFQ_CHECK(parts.size() == ex.size(), AFQ_EINVAL,
+            "afq_ph_create: a determinant has a different electron count than the reference");
+  for (size_t c = 0; c < parts.size(); ++c) work[ex[c]] = parts[c];
+  int sign = 1;
+  for (size_t i = 0; i < work.size(); ++i)
+    for (size_t j = i + 1; j < work.size(); ++j)
+      if (work[j] < work[i]) {
+        sign = -sign;
+        std::swap(work[i], work[j]);
+      }
+  ex.insert(ex.end(), parts.begin(), parts.end());
+  return sign;
+}
+
+}  // namespace
+
+}  // namespace afq
+
+struct afq_ph_t {
+  int nmo = 0, nchol = 0, nact = 0, wblock = 0, device = 0;
+  double E0 = 0.0;
+  afq::Spin spin[2];
+  std::vector<void*> owned;
+  // workspace for one walker block
+  double *G[2] = {nullptr, nullptr}, *QQ0[2] = {nullptr, nullptr}, *work = nullptr;
+  double *ratios[2] = {nullptr, nullptr}, *wgt[2] = {nullptr, nullptr};
+  double *Ov = nullptr, *Ov2 = nullptr, *log_ov = nullptr, *Eref = nullptr, *Ex = nullptr,
+         *acc = nullptr;
+  double *Twina = nullptr, *Swia = nullptr, *KEa = nullptr, *KEb = nullptr, *KEl = nullptr,
+         *R = nullptr;
+  ~afq_ph_t() {
+    for (void* p : owned) (void)hipFree(p);
+  }
+};
+
+using namespace afq;
+
+namespace {
+
+template <typename F>
+int ph_guard(F&& f) {
+  try {
+    f();
+    return AFQ_OK;
+  } catch (const Error& e) {
+    set_error(e.msg);
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    set_error("out of host memory");
+    return AFQ_ENOMEM;
+  } catch (...) {
+    set_error("unknown error");
+    return AFQ_EINVAL;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int afq_ph_create(int nmo, int nup, int ndn, int nchol, int ndet, const int* occs,
+                  const double* ci, int nref, const double* orbmats, const double* h1,
+                  const double* likn, double E0, int walker_block, afq_ph* out) {
+  return ph_guard([&] {
+    AFQ_CHECK(out != nullptr, AFQ_EINVAL, "afq_ph_create: null output handle");
+    *out = nullptr;
+    AFQ_CHECK(nmo > 0 && nup > 0 && ndn > 0 && nup <= nmo && ndn <= nmo && nchol > 0 &&
+                  ndet > 0 && walker_block > 0,
+              AFQ_EBADSHAPE, "afq_ph_create: bad sizes");
+    AFQ_CHECK(nmo <= 256 && nup <= 64 && ndn <= 64, AFQ_EUNSUPPORTED,
+              "afq_ph_create: needs nmo <= 256 and at most 64 electrons per spin");
+    AFQ_CHECK(nref == 1 || nref == 2, AFQ_EBADSHAPE, "afq_ph_create: nref must be 1 or 2");
+    AFQ_CHECK(occs && ci && orbmats && h1 && likn, AFQ_EINVAL, "afq_ph_create: null input");
+    const int nel[2] = {nup, ndn};
+    // --- build_ph_struct on the host
+    std::vector<int> ref[2];
+    for (int s = 0; s < 2; ++s) {
+      for (int k = 0; k < nel[s]; ++k) {
+        const int q = occs[(s ? nup : 0) + k];
+        AFQ_CHECK(q >= s * nmo && q < (s + 1) * nmo, AFQ_EINVAL,
+                  "afq_ph_create: bad occupation in the reference determinant");
+        ref[s].push_back(q);
+      }
+      AFQ_CHECK(std::is_sorted(ref[s].begin(), ref[s].end()), AFQ_EINVAL,
+                "afq_ph_create: occupations must be sorted");
+    }
+    for (int s = 0; s < 2; ++s)
+      for (int k = 0; k < nel[s]; ++k)
+        AFQ_CHECK(ref[s][k] - s * nmo == k, AFQ_EUNSUPPORTED,
+                  "afq_ph_create: the reference must occupy orbitals 0..nel-1 of each spin (as "
+                  "the reference's energy assumes)");
+    std::vector<std::vector<std::vector<int>>> unique[2];   // [spin][rank] list of excitations
+    for (int s = 0; s < 2; ++s) unique[s].resize(nel[s] + 1);
+    struct Conf {
+      int rank[2], loc[2];
+      double re, im;
+    };
+    std::vector<Conf> confs((size_t)ndet);
+    std::vector<int> conf, ex;
+    for (int d = 0; d < ndet; ++d) {
+      Conf c{};
+      c.re = ci[2L * d];
+      c.im = ci[2L * d + 1];
+      for (int s = 0; s < 2; ++s) {
+        conf.assign(occs + (long)d * (nup + ndn) + (s ? nup : 0),
+                    occs + (long)d * (nup + ndn) + (s ? nup : 0) + nel[s]);
+        for (int q : conf)
+          AFQ_CHECK(q >= s * nmo && q < (s + 1) * nmo, AFQ_EINVAL,
+                    "afq_ph_create: bad occupation number in determinant " + std::to_string(d));
+        AFQ_CHECK(std::is_sorted(conf.begin(), conf.end()), AFQ_EINVAL,
+                  "afq_ph_create: occupations must be sorted");
+        const int sign = excitation_of(ref[s], conf, ex);
+        if (sign < 0) {
+          c.re = -c.re;
+          c.im = -c.im;
+        }
+        const int n = (int)ex.size() / 2;
+        for (int k = n; k < 2 * n; ++k) ex[k] -= s * nmo;      // active indices
+        c.rank[s] = n;
+        c.loc[s] = 0;
+        if (n > 0) {
+          auto& lst = unique[s][n];
+          auto it = std::find(lst.begin(), lst.end(), ex);
+          c.loc[s] = (int)(it - lst.begin());
+          if (it == lst.end()) lst.push_back(ex);
+        }
+      }
+      confs[d] = c;
+    }
+    std::unique_ptr<afq_ph_t> P(new afq_ph_t);
+    AFQ_HIP(hipGetDevice(&P->device));
+    P->nmo = nmo;
+    P->nact = nmo;
+    P->nchol = nchol;
+    P->E0 = E0;
+    P->wblock = walker_block;
+    auto& own = P->owned;
+    std::vector<int> first[2];
+    int max_nex = 1;
+    for (int s = 0; s < 2; ++s) {
+      Spin& S = P->spin[s];
+      S.nel = nel[s];
+      first[s].assign(nel[s] + 1, 0);
+      int row = 1;
+      for (int n = 1; n <= nel[s]; ++n) {
+        first[s][n] = row;
+        const auto& lst = unique[s][n];
+        if (lst.empty()) continue;
+        std::vector<int> flat;
+        for (const auto& e : lst) flat.insert(flat.end(), e.begin(), e.end());
+        Shell sh;
+        sh.nex = n;
+        sh.ndet = (int)lst.size();
+        sh.row = row;
+        sh.iexcit = dev_copy(own, flat.data(), flat.size());
+        S.shells.push_back(sh);
+        row += sh.ndet;
+        max_nex = std::max(max_nex, n);
+      }
+      S.nunique = row;
+      std::vector<int> rc(nel[s]);
+      for (int k = 0; k < nel[s]; ++k) rc[k] = ref[s][k] - s * nmo;
+      S.refc = dev_copy(own, rc.data(), rc.size());
+    }
+    AFQ_CHECK(max_nex <= 8, AFQ_EUNSUPPORTED, "afq_ph_create: excitations of rank > 8");
+    // couplings: spin 0 rows u, columns d; spin 1 rows d, columns u; entries sorted by column
+    for (int s = 0; s < 2; ++s) {
+      std::vector<std::vector<std::pair<int, std::pair<double, double>>>> rows(
+          (size_t)P->spin[s].nunique);
+      for (const Conf& c : confs) {
+        const int u = c.rank[0] ? first[0][c.rank[0]] + c.loc[0] : 0;
+        const int d = c.rank[1] ? first[1][c.rank[1]] + c.loc[1] : 0;
+        const int r = s == 0 ? u : d, col = s == 0 ? d : u;
+        rows[r].push_back({col, {c.re, -c.im}});
+      }
+      std::vector<int> cp(1, 0), cix;
+      std::vector<double> cv;
+      for (auto& r : rows) {
+        std::stable_sort(r.begin(), r.end(),
+                         [](const auto& a, const auto& b) { return a.first < b.first; });
+        for (const auto& e : r) {
+          cix.push_back(e.first);
+          cv.push_back(e.second.first);
+          cv.push_back(e.second.second);
+        }
+        cp.push_back((int)cix.size());
+      }
+      P->spin[s].cp = dev_copy(own, cp.data(), cp.size());
+      P->spin[s].ci = dev_copy(own, cix.data(), cix.size());
+      P->spin[s].cv = dev_copy(own, cv.data(), cv.size());
+    }
+    // --- half-rotated Hamiltonian per spin on the GPU
+    hipStream_t st = nullptr;
+    const size_t nmm = (size_t)nmo * nmo;
+    double* dh = dev_copy(own, h1, nmm);
+    double* dL = dev_copy(own, likn, nmm * nchol);
+    for (int s = 0; s < 2; ++s) {
+      Spin& S = P->spin[s];
+      S.A = dev_copy(own, orbmats + (nref == 2 ? s : 0) * 2 * nmm, 2 * nmm);
+      S.Lna = dev_alloc<double>(own, 2 * nmm * nchol);
+      S.haj = dev_alloc<double>(own, 2 * nmm);
+      launch_half_rotate(st, nmo, nmo, nchol, S.A, dh, dL, S.haj, S.Lna);
+    }
+    AFQ_HIP(hipStreamSynchronize(st));
+    // --- workspace for one walker block
+    const long nb = walker_block, K = nchol, na = nmo;
+    const int maxu = std::max(P->spin[0].nunique, P->spin[1].nunique);
+    int maxdet = 1;
+    for (int s = 0; s < 2; ++s)
+      for (const Shell& sh : P->spin[s].shells) maxdet = std::max(maxdet, sh.ndet);
+    const long dblk = std::min(maxdet, kPhDetBlock);
+    const int maxel = std::max(nup, ndn);
+    for (int s = 0; s < 2; ++s) {
+      P->G[s] = dev_alloc<double>(own, 2 * nb * nel[s] * nmo);
+      P->QQ0[s] = dev_alloc<double>(own, 2 * nb * na * nel[s]);
+      P->ratios[s] = dev_alloc<double>(own, 2 * nb * P->spin[s].nunique);
+      P->wgt[s] = dev_alloc<double>(own, 2 * nb * P->spin[s].nunique);
+    }
+    P->work = dev_alloc<double>(own, 2L * maxel * nmo);
+    P->Ov = dev_alloc<double>(own, 2 * nb);
+    P->Ov2 = dev_alloc<double>(own, 2 * nb);
+    P->log_ov = dev_alloc<double>(own, 2 * nb);
+    P->Eref = dev_alloc<double>(own, 6 * nb);
+    P->Ex = dev_alloc<double>(own, 6 * nb);
+    P->acc = dev_alloc<double>(own, 6 * nb);
+    P->Twina = dev_alloc<double>(own, 2 * nb * maxel * K * na);
+    P->Swia = dev_alloc<double>(own, 2 * nb * maxel * na);
+    P->KEa = dev_alloc<double>(own, 2L * maxu * nb * K);
+    P->KEb = dev_alloc<double>(own, 2L * maxu * nb * K);
+    P->KEl = dev_alloc<double>(own, 2 * std::max(dblk, 1L) * nb * K);
+    P->R = dev_alloc<double>(own, 2 * nb * dblk * max_nex * na);
+    *out = P.release();
+  });
+}
+
+void afq_ph_destroy(afq_ph ph) { delete ph; }
+
+int afq_ph_info(afq_ph ph, int* nunique, int* nshells) {
+  return ph_guard([&] {
+    AFQ_CHECK(ph != nullptr, AFQ_EINVAL, "afq_ph_info: null handle");
+    for (int s = 0; s < 2; ++s) {
+      if (nunique) nunique[s] = ph->spin[s].nunique;
+      if (nshells) nshells[s] = (int)ph->spin[s].shells.size();
+    }
+  });
+}
+
+int afq_ph_energy(afq_ph ph, int nw, const double* d_wbuf, long ldw, double* d_E, double* d_ov,
+                  void* stream) {
+  return ph_guard([&] {
+    AFQ_CHECK(ph != nullptr, AFQ_EINVAL, "afq_ph_energy: null handle");
+    AFQ_CHECK(nw >= 0, AFQ_EBADSHAPE, "afq_ph_energy: negative walker count");
+    const long need = (long)ph->nmo * (ph->spin[0].nel + ph->spin[1].nel);
+    AFQ_CHECK(ldw >= need, AFQ_EBADSHAPE, "afq_ph_energy: ldw smaller than the Slater matrices");
+    if (nw == 0) return;
+    AFQ_CHECK(d_wbuf && d_E && d_ov, AFQ_EINVAL, "afq_ph_energy: null operand");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const int nmo = ph->nmo, na = ph->nact, K = ph->nchol;
+    Spin* S = ph->spin;
+    const int nu = S[0].nunique, nd = S[1].nunique;
+    for (int w0 = 0; w0 < nw; w0 += ph->wblock) {
+      const int nb = std::min(ph->wblock, nw - w0);
+      const double* Phi[2] = {d_wbuf + 2 * (long)w0 * ldw,
+                              d_wbuf + 2 * ((long)w0 * ldw + (long)nmo * S[0].nel)};
+      launch_ph_fill(s, nb, 0.0, 0.0, ph->log_ov);
+      launch_ph_fill(s, 3L * nb, 0.0, 0.0, ph->acc);
+      // 1. reference G, log overlap, QQ0 and the excitation overlap ratios of both spins
+      for (int sp = 0; sp < 2; ++sp) {
+        launch_ph_woodbury(s, nb, nmo, S[sp].nel, na, S[sp].A, S[sp].refc, Phi[sp], ldw,
+                           ph->G[sp], ph->log_ov, ph->QQ0[sp], ph->work);
+        launch_ph_fill(s, nb, 1.0, 0.0, ph->ratios[sp]);
+        for (const Shell& sh : S[sp].shells)
+          launch_ph_overlaps(s, nb, sh.ndet, sh.nex, na, S[sp].nel, sh.iexcit, ph->QQ0[sp],
+                             ph->ratios[sp] + 2L * sh.row * nb);
+      }
+      // 2a. weights and the multi-determinant overlap
+      launch_ph_spin_weights(s, nb, nu, S[0].cp, S[0].ci, S[0].cv, ph->ratios[0], ph->ratios[1],
+                             ph->wgt[0], ph->Ov);
+      // 3a. reference energy of the up spin (KE row 0 = its Cholesky trace), times Ov
+      launch_ph_reference_energy(s, nb, S[0].nel, nmo, K, na, 1.0, ph->E0, ph->G[0], S[0].Lna,
+                                 S[0].haj, ph->Eref, ph->KEa, ph->Twina, ph->Swia);
+      launch_ph_accumulate(s, nb, ph->Eref, ph->Ov, nullptr, ph->acc);
+      // 4a. excited up determinants: E corrections and their Cholesky vectors KE (rows 1..)
+      launch_ph_fill(s, 3L * nb, 0.0, 0.0, ph->Ex);
+      for (const Shell& sh : S[0].shells)
+        for (int d0 = 0; d0 < sh.ndet; d0 += kPhDetBlock) {
+          const int nbk = std::min(kPhDetBlock, sh.ndet - d0);
+          const int* iex = sh.iexcit + 2L * sh.nex * d0;
+          const double* w = ph->wgt[0] + 2L * (sh.row + d0) * nb;
+          launch_ph_compact_R(s, nb, nbk, sh.nex, na, S[0].nel, iex, S[0].refc, ph->QQ0[0],
+                              ph->R);
+          launch_ph_1body(s, nb, nbk, sh.nex, na, S[0].nel, iex, S[0].refc, ph->Swia, ph->R, w,
+                          ph->Ex);
+          launch_ph_2body(s, nb, nbk, sh.nex, na, S[0].nel, K, iex, S[0].refc, ph->Twina, ph->R,
+                          w, ph->Ex + 2L * nb, ph->Ex + 4L * nb,
+                          ph->KEa + 2L * (sh.row + d0) * nb * K);
+        }
+      launch_ph_accumulate(s, nb, nullptr, nullptr, ph->Ex, ph->acc);
+      // KEb[d] = ratio_dn[d] sum_u C[d][u] ratio_up[u] KEa[u]
+      launch_ph_couple_ke(s, nb, K, nd, S[1].cp, S[1].ci, S[1].cv, ph->ratios[0], ph->KEa,
+                          ph->ratios[1], ph->KEb);
+      // 2b. weights of the down determinants
+      launch_ph_spin_weights(s, nb, nd, S[1].cp, S[1].ci, S[1].cv, ph->ratios[1], ph->ratios[0],
+                             ph->wgt[1], ph->Ov2);
+      // 3b. reference energy of the down spin (E0 only with the up spin), the cross term
+      launch_ph_reference_energy(s, nb, S[1].nel, nmo, K, na, 1.0, 0.0, ph->G[1], S[1].Lna,
+                                 S[1].haj, ph->Eref, ph->KEa, ph->Twina, ph->Swia);
+      launch_ph_accumulate(s, nb, ph->Eref, ph->Ov, nullptr, ph->acc);
+      launch_ph_ke_dot(s, nb, K, 1, ph->KEb, ph->KEa, ph->acc + 4, 3);
+      // 4b. excited down determinants, contracted with the coupled up vectors
+      launch_ph_fill(s, 3L * nb, 0.0, 0.0, ph->Ex);
+      for (const Shell& sh : S[1].shells)
+        for (int d0 = 0; d0 < sh.ndet; d0 += kPhDetBlock) {
+          const int nbk = std::min(kPhDetBlock, sh.ndet - d0);
+          const int* iex = sh.iexcit + 2L * sh.nex * d0;
+          const double* w = ph->wgt[1] + 2L * (sh.row + d0) * nb;
+          launch_ph_compact_R(s, nb, nbk, sh.nex, na, S[1].nel, iex, S[1].refc, ph->QQ0[1],
+                              ph->R);
+          launch_ph_1body(s, nb, nbk, sh.nex, na, S[1].nel, iex, S[1].refc, ph->Swia, ph->R, w,
+                          ph->Ex);
+          launch_ph_2body(s, nb, nbk, sh.nex, na, S[1].nel, K, iex, S[1].refc, ph->Twina, ph->R,
+                          w, ph->Ex + 2L * nb, ph->Ex + 4L * nb, ph->KEl);
+          launch_ph_ke_dot(s, nb, K, nbk, ph->KEb + 2L * (sh.row + d0) * nb * K, ph->KEl,
+                           ph->Ex + 4L * nb, 1);
+        }
+      launch_ph_accumulate(s, nb, nullptr, nullptr, ph->Ex, ph->acc);
+      // E /= Ov, Ov -> log Ov + log of the reference overlaps
+      launch_ph_finalize(s, nb, ph->acc, ph->Ov, ph->log_ov, d_E + 6L * w0, d_ov + 2L * w0);
+    }
+  });
+}
+
+}  // extern "C"

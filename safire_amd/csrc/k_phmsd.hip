@@ -411,6 +411,115 @@ __global__ __launch_bounds__(256) void k_ph_spin_weights(int nw, int nu, const i
   if (threadIdx.x == 0) ov[w] = sum;
 }
 
+// ph_ref_energy_impl's reductions (Real3IndexFactorization.hpp:646-694) over one walker's
+// Twina [nel][nchol][nact] and Swia [nel][nact], with the reference at orbitals 0..nel-1 as the
+// reference assumes:
+//   E[w][0] = E0 + sum_i Swia[i][i]            (only with the one-body term)
+//   E[w][1] = -scl/2 sum_{i,j,n} T[i][n][j] T[j][n][i]
+//   Kl[w][n] = sum_i T[i][n][i],   E[w][2] = 1/2 sum_n Kl[w][n]^2
+// One workgroup per walker, its threads over n; fixed-order workgroup sums.
+__global__ __launch_bounds__(256) void k_ph_ref_reduce(int nel, int nchol, int nact, double scl,
+                                                       cd E0, int has_h1,
+                                                       const cd* __restrict__ Twina,
+                                                       const cd* __restrict__ Swia,
+                                                       cd* __restrict__ E, cd* __restrict__ Kl) {
+  __shared__ cd red[kPhThreads / 64];
+  const int w = blockIdx.x;
+  const cd* T = Twina + (long)w * nel * nchol * nact;
+  cd ex = cmk(0.0, 0.0), ej = cmk(0.0, 0.0), e1 = cmk(0.0, 0.0);
+  for (int n = threadIdx.x; n < nchol; n += kPhThreads) {
+    cd kl = cmk(0.0, 0.0), x = cmk(0.0, 0.0);
+    for (int i = 0; i < nel; ++i) {
+      const cd* Ti = T + ((long)i * nchol + n) * nact;
+      kl += Ti[i];
+      for (int j = 0; j < nel; ++j) x += Ti[j] * T[((long)j * nchol + n) * nact + i];
+    }
+    Kl[(long)w * nchol + n] = kl;
+    ex += x;
+    ej += kl * kl;
+  }
+  if (has_h1)
+    for (int i = threadIdx.x; i < nel; i += kPhThreads) e1 += Swia[((long)w * nel + i) * nact + i];
+  const cd sx = block_sum(ex, red);
+  const cd sj = block_sum(ej, red);
+  const cd s1 = block_sum(e1, red);
+  if (threadIdx.x == 0) {
+    E[3L * w] = has_h1 ? E0 + s1 : cmk(0.0, 0.0);
+    E[3L * w + 1] = (-0.5 * scl) * sx;
+    E[3L * w + 2] = 0.5 * sj;
+  }
+}
+
+// The cross-spin coupling of the Cholesky vectors of the excited determinants
+// (PHMSD.cpp:470-492): with the couplings C [nout][nin] in CSR form,
+//   KEout[d][w][n] = rout[d][w] sum_{u in row d} C[d][u] rin[u][w] KEin[u][w][n]
+// (the scaling by the first spin's ratios, csrmm, then the scaling by the second spin's).
+// One workgroup per (d, w); threads over n.
+__global__ __launch_bounds__(256) void k_ph_couple_ke(int nw, int nchol, const int* __restrict__ cp,
+                                                      const int* __restrict__ ci,
+                                                      const cd* __restrict__ cv,
+                                                      const cd* __restrict__ rin,
+                                                      const cd* __restrict__ KEin,
+                                                      const cd* __restrict__ rout,
+                                                      cd* __restrict__ KEout) {
+  const int d = blockIdx.y, w = blockIdx.x;
+  const int j0 = cp[d], j1 = cp[d + 1];
+  const cd ro = rout[(long)d * nw + w];
+  for (int n = threadIdx.x; n < nchol; n += kPhThreads) {
+    cd acc = cmk(0.0, 0.0);
+    for (int j = j0; j < j1; ++j) {
+      const long u = ci[j];
+      acc += (cv[j] * rin[u * nw + w]) * KEin[(u * nw + w) * nchol + n];
+    }
+    KEout[((long)d * nw + w) * nchol + n] = ro * acc;
+  }
+}
+
+// out[w * ostride] += sum_{d < nd} sum_n a[d][w][n] b[d][w][n]  (PHMSD.cpp:518-519,
+// phmsd_helpers.hpp:520-521); one workgroup per walker, fixed-order sums.
+__global__ __launch_bounds__(256) void k_ph_ke_dot(int nw, int nchol, int nd,
+                                                   const cd* __restrict__ a,
+                                                   const cd* __restrict__ b, cd* __restrict__ out,
+                                                   int ostride) {
+  __shared__ cd red[kPhThreads / 64];
+  const int w = blockIdx.x;
+  cd acc = cmk(0.0, 0.0);
+  for (int d = 0; d < nd; ++d) {
+    const long base = ((long)d * nw + w) * nchol;
+    for (int n = threadIdx.x; n < nchol; n += kPhThreads) acc += a[base + n] * b[base + n];
+  }
+  const cd s = block_sum(acc, red);
+  if (threadIdx.x == 0) out[(long)w * ostride] += s;
+}
+
+// acc[w][c] += Eref[w][c] Ov[w] + Ex[c][w]   (PHMSD.cpp:453-466, 505-526; either term optional)
+__global__ void k_ph_accumulate(int nw, const cd* __restrict__ Eref, const cd* __restrict__ Ov,
+                                const cd* __restrict__ Ex, cd* __restrict__ acc) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= 3 * nw) return;
+  const int w = e / 3, c = e % 3;
+  cd v = acc[e];
+  if (Eref) v += Eref[e] * Ov[w];
+  if (Ex) v += Ex[(long)c * nw + w];
+  acc[e] = v;
+}
+
+// E[w][c] = acc[w][c] / Ov[w], Ov -> log Ov + log_ov (PHMSD.cpp:531-543)
+__global__ void k_ph_finalize(int nw, const cd* __restrict__ acc, const cd* __restrict__ Ov,
+                              const cd* __restrict__ log_ov, cd* __restrict__ E,
+                              cd* __restrict__ ov_out) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= nw) return;
+  const cd o = Ov[w];
+  for (int c = 0; c < 3; ++c) E[3L * w + c] = cdiv(acc[3L * w + c], o);
+  ov_out[w] = clog_(o) + log_ov[w];
+}
+
+__global__ void k_ph_fill(long n, cd v, cd* __restrict__ x) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) x[e] = v;
+}
+
 }  // namespace
 
 void launch_ph_2body(hipStream_t s, int nw, int ndet, int nex, int nact, int nelec, int nchol,
@@ -531,6 +640,75 @@ void launch_ph_accumulate_R(hipStream_t s, int nw, int ndet, int nex, int nact, 
                      iexcit, refc, reinterpret_cast<const cd*>(Rc),
                      reinterpret_cast<const cd*>(ov), reinterpret_cast<const cd*>(wgt),
                      reinterpret_cast<cd*>(R));
+  AFQ_HIP(hipGetLastError());
+}
+
+}  // namespace afq
+
+namespace afq {
+
+static dim3 ph_blocks(long n) { return dim3((unsigned)((n + 255) / 256)); }
+
+void launch_ph_reference_energy(hipStream_t s, int nw, int nel, int nmo, int nchol, int nact,
+                                double scl, double E0, const double* G, const double* Lna,
+                                const double* haj, double* E, double* Kl, double* Twina,
+                                double* Swia) {
+  if (nw <= 0) return;
+  // Twina[w][i][n][a] = sum_k G[w][i][k] Lna[n][a][k]; Swia[w][i][a] = sum_k G[w][i][k] haj[a][k]
+  launch_gemm_nt_z(s, AFQ_FP64, nw * nel, nchol * nact, nmo, 1.0, G, nmo, Lna, nmo, true, Twina,
+                   (long)nchol * nact);
+  if (haj)
+    launch_gemm_nt_z(s, AFQ_FP64, nw * nel, nact, nmo, 1.0, G, nmo, haj, nmo, true, Swia, nact);
+  hipLaunchKernelGGL(k_ph_ref_reduce, dim3(nw), dim3(kPhThreads), 0, s, nel, nchol, nact, scl,
+                     cmk(E0, 0.0), haj ? 1 : 0, reinterpret_cast<const cd*>(Twina),
+                     reinterpret_cast<const cd*>(Swia), reinterpret_cast<cd*>(E),
+                     reinterpret_cast<cd*>(Kl));
+  AFQ_HIP(hipGetLastError());
+}
+
+void launch_ph_couple_ke(hipStream_t s, int nw, int nchol, int nout, const int* cp, const int* ci,
+                         const double* cv, const double* rin, const double* KEin,
+                         const double* rout, double* KEout) {
+  if (nw <= 0 || nout <= 0) return;
+  hipLaunchKernelGGL(k_ph_couple_ke, dim3(nw, nout), dim3(kPhThreads), 0, s, nw, nchol, cp, ci,
+                     reinterpret_cast<const cd*>(cv), reinterpret_cast<const cd*>(rin),
+                     reinterpret_cast<const cd*>(KEin), reinterpret_cast<const cd*>(rout),
+                     reinterpret_cast<cd*>(KEout));
+  AFQ_HIP(hipGetLastError());
+}
+
+void launch_ph_ke_dot(hipStream_t s, int nw, int nchol, int nd, const double* a, const double* b,
+                      double* out, int ostride) {
+  if (nw <= 0 || nd <= 0) return;
+  hipLaunchKernelGGL(k_ph_ke_dot, dim3(nw), dim3(kPhThreads), 0, s, nw, nchol, nd,
+                     reinterpret_cast<const cd*>(a), reinterpret_cast<const cd*>(b),
+                     reinterpret_cast<cd*>(out), ostride);
+  AFQ_HIP(hipGetLastError());
+}
+
+void launch_ph_accumulate(hipStream_t s, int nw, const double* Eref, const double* Ov,
+                          const double* Ex, double* acc) {
+  if (nw <= 0) return;
+  hipLaunchKernelGGL(k_ph_accumulate, ph_blocks(3L * nw), dim3(256), 0, s, nw,
+                     reinterpret_cast<const cd*>(Eref), reinterpret_cast<const cd*>(Ov),
+                     reinterpret_cast<const cd*>(Ex), reinterpret_cast<cd*>(acc));
+  AFQ_HIP(hipGetLastError());
+}
+
+void launch_ph_finalize(hipStream_t s, int nw, const double* acc, const double* Ov,
+                        const double* log_ov, double* E, double* ov_out) {
+  if (nw <= 0) return;
+  hipLaunchKernelGGL(k_ph_finalize, ph_blocks(nw), dim3(256), 0, s, nw,
+                     reinterpret_cast<const cd*>(acc), reinterpret_cast<const cd*>(Ov),
+                     reinterpret_cast<const cd*>(log_ov), reinterpret_cast<cd*>(E),
+                     reinterpret_cast<cd*>(ov_out));
+  AFQ_HIP(hipGetLastError());
+}
+
+void launch_ph_fill(hipStream_t s, long n, double re, double im, double* x) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_ph_fill, ph_blocks(n), dim3(256), 0, s, n, cmk(re, im),
+                     reinterpret_cast<cd*>(x));
   AFQ_HIP(hipGetLastError());
 }
 

@@ -109,6 +109,14 @@ inline dim3 blocks(long n, int t = 256) { return dim3((unsigned)((n + t - 1) / t
 
 }  // namespace
 
+void launch_half_rotate(hipStream_t s, int M, int N, int K, const double* PsiT, const double* h,
+                        const double* Likn, double* haj, double* Lnak) {
+  const long nL = (long)K * N * M;
+  hipLaunchKernelGGL(k_half_rotate, blocks(nL + (long)N * M), dim3(256), 0, s, M, N, K,
+                     (const cd*)PsiT, h, Likn, (cd*)haj, (cd*)Lnak);
+  AFQ_HIP(hipGetLastError());
+}
+
 void setup_hamiltonian(afq_ham_s* H, hipStream_t s) {
   const int M = H->M, N = H->N, K = H->K;
   const long nL = (long)K * N * M;

@@ -1,0 +1,86 @@
+"""PHMSD local energy on the GPU (afq_ph_energy, PHMSD::energy_alg1) and its reference-determinant
+energy (afq_ph_reference_energy) against the oracle's restatement, which the CPU tests pin by a
+brute-force multi-determinant evaluation."""
+import numpy as np
+import pytest
+
+from oracle import ph_energy as P
+from tests._helpers import rand_c
+from tests.test_oracle_ph_energy import ph_system
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+if not torch.cuda.is_available():  # pragma: no cover
+    pytest.skip("no GPU", allow_module_level=True)
+
+from safire_amd import afqmc as A  # noqa: E402
+
+
+@pytest.mark.parametrize("with_h1", [True, False])
+def test_ph_reference_energy_matches_oracle(with_h1):
+    rng = np.random.default_rng(4 + with_h1)
+    nw, nel, nmo, nchol, nact = 7, 6, 20, 33, 20
+    G, Lna, haj = rand_c(rng, nw, nel, nmo), rand_c(rng, nchol, nact, nmo), rand_c(rng, nact, nmo)
+    Eo, Klo, To, So = P.ph_reference_energy(G, Lna, haj if with_h1 else None, 0.75, 1.0)
+    Gd, Ld, hd = A.to_device(G), A.to_device(Lna), A.to_device(haj)
+    E = torch.zeros((nw, 3), dtype=torch.complex128, device="cuda")
+    Kl = torch.zeros((nw, nchol), dtype=torch.complex128, device="cuda")
+    T = torch.zeros((nw, nel, nchol, nact), dtype=torch.complex128, device="cuda")
+    S = torch.zeros((nw, nel, nact), dtype=torch.complex128, device="cuda")
+    A.check(A.lib().afq_ph_reference_energy(nw, nel, nmo, nchol, nact, 1.0, 0.75, A.ptr(Gd),
+                                            A.ptr(Ld), A.ptr(hd) if with_h1 else None, A.ptr(E),
+                                            A.ptr(Kl), A.ptr(T), A.ptr(S), A.stream_ptr()))
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(T.cpu().numpy(), To, rtol=1e-12, atol=1e-12 * np.abs(To).max())
+    np.testing.assert_allclose(Kl.cpu().numpy(), Klo, rtol=1e-12, atol=1e-12 * np.abs(Klo).max())
+    np.testing.assert_allclose(E.cpu().numpy(), Eo, rtol=1e-11, atol=1e-11 * np.abs(Eo).max())
+    if with_h1:
+        np.testing.assert_allclose(S.cpu().numpy(), So, rtol=1e-12, atol=1e-12 * np.abs(So).max())
+
+
+def walkers(rng, nw, nmo, nup, ndn, pad=3):
+    Pu, Pd = rand_c(rng, nw, nmo, nup), rand_c(rng, nw, nmo, ndn)
+    buf = np.zeros((nw, nmo * (nup + ndn) + pad), complex)
+    buf[:, :nmo * nup] = Pu.reshape(nw, -1)
+    buf[:, nmo * nup:nmo * (nup + ndn)] = Pd.reshape(nw, -1)
+    return Pu, Pd, buf
+
+
+@pytest.mark.parametrize("uhf,nw,block", [(False, 5, 32), (True, 9, 4)])
+def test_ph_energy_matches_oracle(uhf, nw, block):
+    rng = np.random.default_rng(30 + uhf)
+    s = ph_system(rng, 9, 4, 3, 7, 40, 3, uhf=uhf)
+    Pu, Pd, buf = walkers(rng, nw, 9, 4, 3)
+    E0 = 1.25
+    orb = np.stack(s["A"][:2]) if uhf else s["A"][0][None]
+    ph = A.PHMSD(s["occs"], s["ci"], orb, s["h"], s["L"].transpose(1, 2, 0), E0, walker_block=block)
+    assert ph.nunique == tuple(s["struct"]["nunique"])
+    E, ov = ph.energy(A.to_device(buf))
+    Eo, ovo = P.energy_alg1(s["struct"], s["A"], s["Lna"], s["haj"], E0, Pu, Pd)
+    np.testing.assert_allclose(E.cpu().numpy(), Eo, rtol=1e-10, atol=1e-11 * np.abs(Eo).max())
+    d = ov.cpu().numpy() - ovo
+    d.imag = (d.imag + np.pi) % (2 * np.pi) - np.pi
+    assert np.abs(d).max() < 1e-10
+    ph.close()
+
+
+@pytest.mark.parametrize("ndet,max_rank", [(1000, 4), (800, 2)])
+def test_ph_energy_c5_shape(ndet, max_rank):
+    """NMO 64, 10 + 10 electrons, 1000 determinants of rank 1-4 (and 800 of rank 1-2, whose
+    rank-2 shells exceed one R block of 256 determinants), 24 walkers in blocks of 16; a reduced
+    Cholesky rank keeps the oracle fast."""
+    rng = np.random.default_rng(5)
+    s = ph_system(rng, 64, 10, 10, 48, ndet, max_rank)
+    if max_rank == 2:
+        assert max(len(x) for x in s["struct"]["shells"][0]) > 256
+    Pu, Pd, buf = walkers(rng, 24, 64, 10, 10)
+    ph = A.PHMSD(s["occs"], s["ci"], s["A"][0][None], s["h"], s["L"].transpose(1, 2, 0), 0.0,
+                 walker_block=16)
+    E, ov = ph.energy(A.to_device(buf))
+    Eo, ovo = P.energy_alg1(s["struct"], s["A"], s["Lna"], s["haj"], 0.0, Pu, Pd)
+    np.testing.assert_allclose(E.cpu().numpy(), Eo, rtol=1e-9, atol=1e-10 * np.abs(Eo).max())
+    d = ov.cpu().numpy() - ovo
+    d.imag = (d.imag + np.pi) % (2 * np.pi) - np.pi
+    assert np.abs(d).max() < 1e-9
+    ph.close()

@@ -1,0 +1,131 @@
+"""The oracle's PHMSD local energy (PHMSD::energy_alg1 restated) pinned by a brute-force
+multi-determinant evaluation: every determinant of the CI expansion is built explicitly, its
+overlap and mixed-estimator energy are evaluated by Wick's theorem from the full one-body density
+matrix, and the CI-weighted average is compared component by component."""
+import numpy as np
+import pytest
+
+from oracle import ph_energy as P
+from tests._helpers import rand_c
+
+
+def ph_system(rng, nmo, nup, ndn, nchol, ndet, max_rank, uhf=False):
+    """A random COLLINEAR PHMSD system. Returns a dict with the Hamiltonian (h [nmo][nmo],
+    L [nchol][nmo][nmo]), orbital rows A[s] [nmo][nmo], the half-rotated tensors, the CI
+    expansion (occs, ci) and the derived structure."""
+    X = rng.standard_normal((nmo, nmo))
+    h = 0.1 * (X + X.T) + np.diag(np.linspace(-2, 0, nmo))
+    B = rng.standard_normal((nchol, nmo, nmo))
+    L = 0.5 * (B + B.transpose(0, 2, 1)) / np.sqrt(2) * np.exp(-np.arange(nchol) / 6)[:, None, None]
+    A = []
+    for s in range(2 if uhf else 1):
+        Q, _ = np.linalg.qr(rand_c(rng, nmo, nmo))
+        A.append(np.ascontiguousarray(Q.conj().T))           # orbitals as rows (PsiT)
+    if not uhf:
+        A.append(A[0])
+    occs, dets = [], set()
+    ref = (tuple(range(nup)), tuple(range(nmo, nmo + ndn)))
+    occs.append(list(ref[0]) + list(ref[1]))
+    dets.add(ref)
+    while len(occs) < ndet:
+        conf = []
+        for s, (ne, base) in enumerate([(nup, 0), (ndn, nmo)]):
+            r = rng.integers(0, min(max_rank, ne) + 1)
+            occ = list(range(base, base + ne))
+            holes = rng.choice(ne, r, replace=False)
+            virt = rng.choice(np.arange(base + ne, base + nmo), r, replace=False)
+            for hh, v in zip(holes, virt):
+                occ[hh] = v
+            conf.append(tuple(sorted(occ)))
+        if tuple(conf) in dets:
+            continue
+        dets.add(tuple(conf))
+        occs.append(list(conf[0]) + list(conf[1]))
+    rank = np.array([sum(o not in ref[0] + ref[1] for o in oc) for oc in occs])
+    ci = (rng.standard_normal(ndet) + 0.3j * rng.standard_normal(ndet)) * np.exp(-rank)
+    ci[0] = 0.9
+    occs = np.array(occs, np.int32)
+    Lna = [np.ascontiguousarray(np.einsum("al,nlk->nak", A[s], L)) for s in range(2)]
+    haj = [np.ascontiguousarray(A[s] @ h) for s in range(2)]
+    return {"h": h, "L": L, "A": A, "Lna": Lna, "haj": haj, "occs": occs, "ci": ci,
+            "nmo": nmo, "nup": nup, "ndn": ndn,
+            "struct": P.build_ph_struct(ci, occs, nmo, nup, ndn)}
+
+
+def brute_force(sysd, Phi_up, Phi_dn, E0):
+    """sum_D conj(c_D) <D|phi> E_D / sum_D conj(c_D) <D|phi>, per energy component, with
+    rho_s = Phi (A_D Phi)^-1 A_D: E1 = E0 + sum_s tr(h rho_s), EXX = -1/2 sum_ns tr(L rho L rho),
+    EJ = 1/2 sum_n (sum_s tr(L_n rho_s))^2."""
+    h, L, A, nmo, nup = sysd["h"], sysd["L"], sysd["A"], sysd["nmo"], sysd["nup"]
+    nw = Phi_up.shape[0]
+    num = np.zeros((nw, 3), complex)
+    den = np.zeros(nw, complex)
+    for occ, c in zip(sysd["occs"], sysd["ci"]):
+        rows = [list(occ[:nup]), [o - nmo for o in occ[nup:]]]
+        for w in range(nw):
+            ov, E = 1.0, np.array([E0, 0.0, 0.0], complex)
+            trL = np.zeros(L.shape[0], complex)
+            for s, Phi in enumerate([Phi_up[w], Phi_dn[w]]):
+                Ad = A[s][rows[s]]
+                M = Ad @ Phi
+                ov *= np.linalg.det(M)
+                rho = Phi @ np.linalg.solve(M, Ad)
+                E[0] += np.trace(h @ rho)
+                LR = L @ rho
+                E[1] += -0.5 * np.einsum("nij,nji->", LR, LR)
+                trL += np.einsum("nii->n", LR)
+            E[2] = 0.5 * np.sum(trL * trL)
+            num[w] += np.conj(c) * ov * E
+            den[w] += np.conj(c) * ov
+    return num / den[:, None], den
+
+
+def test_excitation_sign_and_positions():
+    # orbital 1 (position 1) -> 5: in-place order [0, 5, 2] needs one exchange to sort
+    holes, parts, sign = P.excitation([0, 1, 2], [0, 2, 5])
+    assert holes == [1] and parts == [5] and sign == -1
+    # 0 -> 3 and 2 -> 4: in place [3, 1, 4] -> sorted [1, 3, 4], one exchange
+    holes, parts, sign = P.excitation([0, 1, 2], [1, 3, 4])
+    assert holes == [0, 2] and parts == [3, 4] and sign == -1
+    assert P.excitation([0, 1, 2], [0, 1, 7]) == ([2], [7], 1)
+
+
+def test_struct_couplings_reproduce_the_expansion():
+    rng = np.random.default_rng(3)
+    s = ph_system(rng, 9, 3, 2, 4, 25, 3)
+    st = s["struct"]
+    nu, nd = st["nunique"]
+    assert st["C"][0].shape == (nu, nd) and np.allclose(st["C"][1], st["C"][0].T)
+    # sum over the couplings of |c| equals the expansion's (signs aside)
+    assert np.isclose(np.abs(st["C"][0]).sum(), np.abs(s["ci"]).sum())
+    assert st["C"][0][0, 0] == 0.9
+    for spin in range(2):
+        cp, ci, cv = st["csr"][spin]
+        dense = np.zeros_like(st["C"][spin])
+        for r in range(len(cp) - 1):
+            dense[r, ci[cp[r]:cp[r + 1]]] += cv[cp[r]:cp[r + 1]]
+        np.testing.assert_array_equal(dense, st["C"][spin])
+
+
+@pytest.mark.parametrize("uhf", [False, True])
+def test_energy_alg1_matches_brute_force(uhf):
+    rng = np.random.default_rng(17 + uhf)
+    s = ph_system(rng, 9, 4, 3, 7, 40, 3, uhf=uhf)
+    nw = 5
+    Phi_up, Phi_dn = rand_c(rng, nw, 9, 4), rand_c(rng, nw, 9, 3)
+    E0 = 1.25
+    E, logov = P.energy_alg1(s["struct"], s["A"], s["Lna"], s["haj"], E0, Phi_up, Phi_dn)
+    Eb, ovb = brute_force(s, Phi_up, Phi_dn, E0)
+    scale = np.abs(Eb).max()
+    np.testing.assert_allclose(E, Eb, rtol=1e-10, atol=1e-11 * scale)
+    np.testing.assert_allclose(np.exp(logov), ovb, rtol=1e-10)
+
+
+def test_single_determinant_reduces_to_the_reference_energy():
+    rng = np.random.default_rng(5)
+    s = ph_system(rng, 7, 3, 3, 5, 1, 1)
+    Phi_up, Phi_dn = rand_c(rng, 3, 7, 3), rand_c(rng, 3, 7, 3)
+    E, logov = P.energy_alg1(s["struct"], s["A"], s["Lna"], s["haj"], 0.0, Phi_up, Phi_dn)
+    Eb, ovb = brute_force(s, Phi_up, Phi_dn, 0.0)
+    np.testing.assert_allclose(E, Eb, rtol=1e-11, atol=1e-12 * np.abs(Eb).max())
+    np.testing.assert_allclose(np.exp(logov), 0.9 * ovb / 0.9, rtol=1e-11)
