@@ -301,6 +301,15 @@ int afq_ph_info(afq_ph ph, int* nunique, int* nshells);
  * rows of ldw c128 holding Phi_up [nmo][nup] then Phi_dn [nmo][ndn] (the COLLINEAR WalkerSet). */
 int afq_ph_energy(afq_ph ph, int nw, const double* d_wbuf, long ldw, double* d_E, double* d_ov,
                   void* stream);
+/* PHMSD::MixedDensityMatrix (PHMSD.cpp:549-795): d_ov [nw] c128 = log sum_D conj(c_D) <D|phi_w>;
+ * full = 0: d_G [nw][2 nmo][nmo] c128, the compact G of the up spin's active orbitals then the down
+ * spin's (G_s = R_s^T G_ref,s / Ov); full = 1: d_G [nw][2][nmo][nmo] = A_s^T G_s. */
+int afq_ph_mixed_density_matrix(afq_ph ph, int nw, const double* d_wbuf, long ldw, int full,
+                                double* d_G, double* d_ov, void* stream);
+/* PHMSD::vbias (PHMSD.cpp:53-72) with Real3IndexFactorization::vbias (hpp:328-361), COLLINEAR:
+ * d_v [nw][nchol] c128 = sqrt(dt) sum_s sum_{a,k} Lna_s[n][a][k] G_s[w][a][k]. */
+int afq_ph_vbias(afq_ph ph, int nw, const double* d_wbuf, long ldw, double dt, double* d_v,
+                 void* stream);
 /* The spin coupling of the determinant weights (PHMSD.cpp:441-450): C [nu][nd] (CSR: d_cp row
  * pointers [nu+1], d_ci columns, d_cv c128 CI coefficients) couples up excitation u with down
  * excitation d; d_wgt[u][w] = ratio_up[u][w] sum_d C[u][d] ratio_dn[d][w] and

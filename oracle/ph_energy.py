@@ -192,3 +192,40 @@ def energy_alg1(struct, A, Lna, haj, E0, Phi_up, Phi_dn):
     Ed[2] += np.einsum("dwn,dwn->w", KEr[1:nd], KEd[1:nd])
     Eacc += Ed.T
     return Eacc / Ov[:, None], np.log(Ov) + log_ov
+
+
+def mixed_density_matrix(struct, A, Phi_up, Phi_dn, compact=True):
+    """PHMSD::MixedDensityMatrix (PHMSD.cpp:549-795), COLLINEAR walkers. Returns (G, log overlap):
+    compact G [nw][nact_up + nact_dn][nmo] (rows: the active orbitals of each spin,
+    G_s = R_s^T G_ref,s / Ov with R_s = calculate_R of the spin's determinant weights), or the full
+    G [nw][2][nmo][nmo] = A_s^T G_s (csrmm 'T' with the orbital rows)."""
+    refc = struct["refc"]
+    nw = Phi_up.shape[0]
+    G0, log_ov, Q0 = O.ph_woodbury(A[0], refc[0], Phi_up)
+    G1, log_ov, Q1 = O.ph_woodbury(A[1], refc[1], Phi_dn, log_ov)
+    r_up = _shell_overlaps(struct, 0, Q0, nw)
+    r_dn = _shell_overlaps(struct, 1, Q1, nw)
+    wgt0, Ov = O.ph_spin_weights(*struct["csr"][0], r_up, r_dn)
+    Ra = O.ph_calculate_R(struct["shells"][0], refc[0], Q0, wgt0)
+    wgt1, _ = O.ph_spin_weights(*struct["csr"][1], r_dn, r_up)
+    Rb = O.ph_calculate_R(struct["shells"][1], refc[1], Q1, wgt1)
+    Gc = np.concatenate([np.einsum("wia,wik->wak", Ra, G0), np.einsum("wia,wik->wak", Rb, G1)],
+                        axis=1)
+    Gc = Gc * (1.0 / Ov)[:, None, None]
+    logov = np.log(Ov) + log_ov
+    if compact:
+        return Gc, logov
+    na = A[0].shape[0]
+    Gf = np.stack([np.einsum("al,wak->wlk", A[0], Gc[:, :na]),
+                   np.einsum("al,wak->wlk", A[1], Gc[:, na:])], axis=1)
+    return Gf, logov
+
+
+def vbias(struct, A, Lna, dt, Phi_up, Phi_dn):
+    """PHMSD::vbias (PHMSD.cpp:53-72) with Real3IndexFactorization::vbias (hpp:328-361) for
+    COLLINEAR walkers: v[w][n] = sqrt(dt) sum_s sum_{a,k} Lna_s[n][a][k] G_s[w][a][k] over the
+    compact multi-determinant G."""
+    Gc, _ = mixed_density_matrix(struct, A, Phi_up, Phi_dn)
+    na = A[0].shape[0]
+    return np.sqrt(dt) * (np.einsum("wak,nak->wn", Gc[:, :na], Lna[0], optimize=True) +
+                          np.einsum("wak,nak->wn", Gc[:, na:], Lna[1], optimize=True))

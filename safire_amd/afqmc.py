@@ -88,6 +88,8 @@ SIGNATURES = {
     "afq_ph_destroy": (None, [_vp]),
     "afq_ph_info": (_i, [_vp, _vp, _vp]),
     "afq_ph_energy": (_i, [_vp, _i, _vp, _l, _vp, _vp, _vp]),
+    "afq_ph_mixed_density_matrix": (_i, [_vp, _i, _vp, _l, _i, _vp, _vp, _vp]),
+    "afq_ph_vbias": (_i, [_vp, _i, _vp, _l, _d, _vp, _vp]),
     "afq_ph_compact_R": (_i, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp]),
     "afq_ph_accumulate_R": (_i, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "afq_split_seed": (C.c_ulonglong, [_i, _i, C.c_uint]),
@@ -295,6 +297,28 @@ class PHMSD:
         check(lib().afq_ph_energy(self.handle, W, ptr(wbuf), wbuf.shape[1], ptr(E), ptr(ov),
                                   stream_ptr(stream)))
         return E, ov
+
+    def MixedDensityMatrix(self, wbuf, full=False, stream=None):
+        """PHMSD::MixedDensityMatrix (PHMSD.cpp:549-795): (G, log overlap); compact G
+        [W][2 nmo][nmo] or full G [W][2][nmo][nmo]."""
+        torch = _torch()
+        W, n = wbuf.shape[0], self.nmo
+        G = torch.empty((W, 2, n, n) if full else (W, 2 * n, n), dtype=torch.complex128,
+                        device=wbuf.device)
+        ov = torch.empty(W, dtype=torch.complex128, device=wbuf.device)
+        check(lib().afq_ph_mixed_density_matrix(self.handle, W, ptr(wbuf), wbuf.shape[1],
+                                                int(bool(full)), ptr(G), ptr(ov),
+                                                stream_ptr(stream)))
+        return G, ov
+
+    def vbias(self, wbuf, dt, v=None, stream=None):
+        """PHMSD::vbias (PHMSD.cpp:53-72): v [W][nchol]."""
+        torch = _torch()
+        W = wbuf.shape[0]
+        v = torch.empty((W, self.nchol), dtype=torch.complex128, device=wbuf.device) if v is None else v
+        check(lib().afq_ph_vbias(self.handle, W, ptr(wbuf), wbuf.shape[1], float(dt), ptr(v),
+                                 stream_ptr(stream)))
+        return v
 
 
 class WalkerSet:

@@ -279,6 +279,12 @@ void launch_ph_accumulate(hipStream_t s, int nw, const double* Eref, const doubl
 void launch_ph_finalize(hipStream_t s, int nw, const double* acc, const double* Ov,
                         const double* log_ov, double* E, double* ov_out);
 void launch_ph_fill(hipStream_t s, long n, double re, double im, double* x);
+void launch_ph_rg(hipStream_t s, int nw, int nel, int nact, int nmo, int ldrows, int row0,
+                  const double* R, const double* GA, const double* Ov, double* G);
+void launch_ph_full_g(hipStream_t s, int nw, int nact, int nmo, const double* A0,
+                      const double* A1, const double* Gc, double* Gf);
+void launch_ph_log_overlap(hipStream_t s, int nw, const double* Ov, const double* log_ov,
+                           double* out);
 void launch_ph_accumulate_R(hipStream_t s, int nw, int ndet, int nex, int nact, int nelec,
                             const int* iexcit, const int* refc, const double* Rc,
                             const double* ov, const double* wgt, double* R);

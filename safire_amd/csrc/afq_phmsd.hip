@@ -16,6 +16,10 @@
 //                    nwalk_block_size) and determinant blocks (ndet_block_size), both
 //                    numerically irrelevant: every kernel is the GPU form of one reference call
 //                    (k_phmsd.hip), in the reference's order.
+//   afq_ph_mixed_density_matrix = PHMSD::MixedDensityMatrix (PHMSD.cpp:549-795), compact or full.
+//   afq_ph_vbias   = PHMSD::vbias (PHMSD.cpp:53-72): the compact G of both spins contracted with
+//                    the half-rotated Cholesky vectors of both spins stacked per vector
+//                    (Real3IndexFactorization.hpp:328-361, sqrt(dt) for COLLINEAR walkers).
 #include "afq_internal.hpp"
 
 #include <algorithm>
@@ -95,6 +99,7 @@ struct afq_ph_t {
   int nmo = 0, nchol = 0, nact = 0, wblock = 0, device = 0;
   double E0 = 0.0;
   afq::Spin spin[2];
+  double* Lstack = nullptr;        // c128 [nchol][2 nact][nmo]: both spins' Lna per vector
   std::vector<void*> owned;
   // workspace for one walker block
   double *G[2] = {nullptr, nullptr}, *QQ0[2] = {nullptr, nullptr}, *work = nullptr;
@@ -103,6 +108,8 @@ struct afq_ph_t {
          *acc = nullptr;
   double *Twina = nullptr, *Swia = nullptr, *KEa = nullptr, *KEb = nullptr, *KEl = nullptr,
          *R = nullptr;
+  double *Racc = nullptr, *Gblk = nullptr, *split = nullptr;
+  long split_elems = 0;
   ~afq_ph_t() {
     for (void* p : owned) (void)hipFree(p);
   }
@@ -127,6 +134,52 @@ int ph_guard(F&& f) {
     set_error("unknown error");
     return AFQ_EINVAL;
   }
+}
+
+// Step 1 of energy_alg1 / MixedDensityMatrix: the reference G, QQ0 and log overlap of both spins
+// and the excitation overlap ratios (row 0 = the reference).
+void ph_overlaps_block(afq_ph_t* ph, hipStream_t s, int nb, const double* const Phi[2], long ldw) {
+  Spin* S = ph->spin;
+  launch_ph_fill(s, nb, 0.0, 0.0, ph->log_ov);
+  for (int sp = 0; sp < 2; ++sp) {
+    launch_ph_woodbury(s, nb, ph->nmo, S[sp].nel, ph->nact, S[sp].A, S[sp].refc, Phi[sp], ldw,
+                       ph->G[sp], ph->log_ov, ph->QQ0[sp], ph->work);
+    launch_ph_fill(s, nb, 1.0, 0.0, ph->ratios[sp]);
+    for (const Shell& sh : S[sp].shells)
+      launch_ph_overlaps(s, nb, sh.ndet, sh.nex, ph->nact, S[sp].nel, sh.iexcit, ph->QQ0[sp],
+                         ph->ratios[sp] + 2L * sh.row * nb);
+  }
+}
+
+// PHMSD::MixedDensityMatrix (PHMSD.cpp:692-795), compact, for one walker block: G rows
+// [0, nact) and [nact, 2 nact) of d_Gc [nb][2 nact][nmo]; d_logov [nb] = log Ov + log_ov.
+void ph_density_block(afq_ph_t* ph, hipStream_t s, int nb, const double* const Phi[2], long ldw,
+                      double* d_Gc, double* d_logov) {
+  Spin* S = ph->spin;
+  const int na = ph->nact;
+  ph_overlaps_block(ph, s, nb, Phi, ldw);
+  for (int sp = 0; sp < 2; ++sp) {
+    // weights W[u] = ratio_s[u] sum_d C[u][d] ratio_other[d] (the up spin's also give Ov)
+    launch_ph_spin_weights(s, nb, S[sp].nunique, S[sp].cp, S[sp].ci, S[sp].cv, ph->ratios[sp],
+                           ph->ratios[1 - sp], ph->wgt[sp], sp == 0 ? ph->Ov : ph->Ov2);
+    // calculate_R: the reference, then every shell in determinant blocks
+    launch_ph_fill(s, (long)nb * S[sp].nel * na, 0.0, 0.0, ph->Racc);
+    launch_ph_accumulate_R(s, nb, 1, 0, na, S[sp].nel, nullptr, S[sp].refc, nullptr, nullptr,
+                           ph->wgt[sp], ph->Racc);
+    for (const Shell& sh : S[sp].shells)
+      for (int d0 = 0; d0 < sh.ndet; d0 += kPhDetBlock) {
+        const int nbk = std::min(kPhDetBlock, sh.ndet - d0);
+        const int* iex = sh.iexcit + 2L * sh.nex * d0;
+        launch_ph_compact_R(s, nb, nbk, sh.nex, na, S[sp].nel, iex, S[sp].refc, ph->QQ0[sp],
+                            ph->R);
+        launch_ph_accumulate_R(s, nb, nbk, sh.nex, na, S[sp].nel, iex, S[sp].refc, ph->R,
+                               ph->ratios[sp] + 2L * (sh.row + d0) * nb,
+                               ph->wgt[sp] + 2L * (sh.row + d0) * nb, ph->Racc);
+      }
+    launch_ph_rg(s, nb, S[sp].nel, na, ph->nmo, 2 * na, sp * na, ph->Racc, ph->G[sp], ph->Ov,
+                 d_Gc);
+  }
+  launch_ph_log_overlap(s, nb, ph->Ov, ph->log_ov, d_logov);
 }
 
 }  // namespace
@@ -276,6 +329,15 @@ int afq_ph_create(int nmo, int nup, int ndn, int nchol, int ndet, const int* occ
       S.haj = dev_alloc<double>(own, 2 * nmm);
       launch_half_rotate(st, nmo, nmo, nchol, S.A, dh, dL, S.haj, S.Lna);
     }
+    {
+      double* stackA = dev_alloc<double>(own, 4 * nmm);
+      AFQ_HIP(hipMemcpy(stackA, P->spin[0].A, 2 * nmm * sizeof(double), hipMemcpyDeviceToDevice));
+      AFQ_HIP(hipMemcpy(stackA + 2 * nmm, P->spin[1].A, 2 * nmm * sizeof(double),
+                        hipMemcpyDeviceToDevice));
+      P->Lstack = dev_alloc<double>(own, 4 * nmm * nchol);
+      double* hstack = dev_alloc<double>(own, 4 * nmm);
+      launch_half_rotate(st, nmo, 2 * nmo, nchol, stackA, dh, dL, hstack, P->Lstack);
+    }
     AFQ_HIP(hipStreamSynchronize(st));
     // --- workspace for one walker block
     const long nb = walker_block, K = nchol, na = nmo;
@@ -304,6 +366,10 @@ int afq_ph_create(int nmo, int nup, int ndn, int nchol, int ndet, const int* occ
     P->KEb = dev_alloc<double>(own, 2L * maxu * nb * K);
     P->KEl = dev_alloc<double>(own, 2 * std::max(dblk, 1L) * nb * K);
     P->R = dev_alloc<double>(own, 2 * nb * dblk * max_nex * na);
+    P->Racc = dev_alloc<double>(own, 2 * nb * maxel * na);
+    P->Gblk = dev_alloc<double>(own, 2 * nb * 2 * na * nmo);
+    P->split_elems = (long)kMaxSplits * nb * K;
+    P->split = dev_alloc<double>(own, 2 * P->split_elems);
     *out = P.release();
   });
 }
@@ -337,17 +403,9 @@ int afq_ph_energy(afq_ph ph, int nw, const double* d_wbuf, long ldw, double* d_E
       const int nb = std::min(ph->wblock, nw - w0);
       const double* Phi[2] = {d_wbuf + 2 * (long)w0 * ldw,
                               d_wbuf + 2 * ((long)w0 * ldw + (long)nmo * S[0].nel)};
-      launch_ph_fill(s, nb, 0.0, 0.0, ph->log_ov);
       launch_ph_fill(s, 3L * nb, 0.0, 0.0, ph->acc);
       // 1. reference G, log overlap, QQ0 and the excitation overlap ratios of both spins
-      for (int sp = 0; sp < 2; ++sp) {
-        launch_ph_woodbury(s, nb, nmo, S[sp].nel, na, S[sp].A, S[sp].refc, Phi[sp], ldw,
-                           ph->G[sp], ph->log_ov, ph->QQ0[sp], ph->work);
-        launch_ph_fill(s, nb, 1.0, 0.0, ph->ratios[sp]);
-        for (const Shell& sh : S[sp].shells)
-          launch_ph_overlaps(s, nb, sh.ndet, sh.nex, na, S[sp].nel, sh.iexcit, ph->QQ0[sp],
-                             ph->ratios[sp] + 2L * sh.row * nb);
-      }
+      ph_overlaps_block(ph, s, nb, Phi, ldw);
       // 2a. weights and the multi-determinant overlap
       launch_ph_spin_weights(s, nb, nu, S[0].cp, S[0].ci, S[0].cv, ph->ratios[0], ph->ratios[1],
                              ph->wgt[0], ph->Ov);
@@ -401,6 +459,56 @@ int afq_ph_energy(afq_ph ph, int nw, const double* d_wbuf, long ldw, double* d_E
       launch_ph_accumulate(s, nb, nullptr, nullptr, ph->Ex, ph->acc);
       // E /= Ov, Ov -> log Ov + log of the reference overlaps
       launch_ph_finalize(s, nb, ph->acc, ph->Ov, ph->log_ov, d_E + 6L * w0, d_ov + 2L * w0);
+    }
+  });
+}
+
+int afq_ph_mixed_density_matrix(afq_ph ph, int nw, const double* d_wbuf, long ldw, int full,
+                                double* d_G, double* d_ov, void* stream) {
+  return ph_guard([&] {
+    AFQ_CHECK(ph != nullptr, AFQ_EINVAL, "afq_ph_mixed_density_matrix: null handle");
+    AFQ_CHECK(nw >= 0, AFQ_EBADSHAPE, "afq_ph_mixed_density_matrix: negative walker count");
+    const long need = (long)ph->nmo * (ph->spin[0].nel + ph->spin[1].nel);
+    AFQ_CHECK(ldw >= need, AFQ_EBADSHAPE,
+              "afq_ph_mixed_density_matrix: ldw smaller than the Slater matrices");
+    if (nw == 0) return;
+    AFQ_CHECK(d_wbuf && d_G && d_ov, AFQ_EINVAL, "afq_ph_mixed_density_matrix: null operand");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const int nmo = ph->nmo, na = ph->nact;
+    for (int w0 = 0; w0 < nw; w0 += ph->wblock) {
+      const int nb = std::min(ph->wblock, nw - w0);
+      const double* Phi[2] = {d_wbuf + 2 * (long)w0 * ldw,
+                              d_wbuf + 2 * ((long)w0 * ldw + (long)nmo * ph->spin[0].nel)};
+      if (full) {
+        ph_density_block(ph, s, nb, Phi, ldw, ph->Gblk, d_ov + 2L * w0);
+        launch_ph_full_g(s, nb, na, nmo, ph->spin[0].A, ph->spin[1].A, ph->Gblk,
+                         d_G + 2L * w0 * 2 * nmo * nmo);
+      } else {
+        ph_density_block(ph, s, nb, Phi, ldw, d_G + 2L * w0 * 2 * na * nmo, d_ov + 2L * w0);
+      }
+    }
+  });
+}
+
+int afq_ph_vbias(afq_ph ph, int nw, const double* d_wbuf, long ldw, double dt, double* d_v,
+                 void* stream) {
+  return ph_guard([&] {
+    AFQ_CHECK(ph != nullptr, AFQ_EINVAL, "afq_ph_vbias: null handle");
+    AFQ_CHECK(nw >= 0 && dt > 0.0, AFQ_EBADSHAPE, "afq_ph_vbias: bad walker count or dt");
+    const long need = (long)ph->nmo * (ph->spin[0].nel + ph->spin[1].nel);
+    AFQ_CHECK(ldw >= need, AFQ_EBADSHAPE, "afq_ph_vbias: ldw smaller than the Slater matrices");
+    if (nw == 0) return;
+    AFQ_CHECK(d_wbuf && d_v, AFQ_EINVAL, "afq_ph_vbias: null operand");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const int nmo = ph->nmo, na = ph->nact, K = ph->nchol;
+    const int kx = 2 * na * nmo;
+    for (int w0 = 0; w0 < nw; w0 += ph->wblock) {
+      const int nb = std::min(ph->wblock, nw - w0);
+      const double* Phi[2] = {d_wbuf + 2 * (long)w0 * ldw,
+                              d_wbuf + 2 * ((long)w0 * ldw + (long)nmo * ph->spin[0].nel)};
+      ph_density_block(ph, s, nb, Phi, ldw, ph->Gblk, ph->Eref);   // log overlap unused
+      launch_gemm_nt_z(s, AFQ_FP64, nb, K, kx, std::sqrt(dt), ph->Gblk, kx, ph->Lstack, kx, true,
+                       d_v + 2L * w0 * K, K, ph->split, ph->split_elems);
     }
   });
 }

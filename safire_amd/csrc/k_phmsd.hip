@@ -515,6 +515,46 @@ __global__ void k_ph_finalize(int nw, const cd* __restrict__ acc, const cd* __re
   ov_out[w] = clog_(o) + log_ov[w];
 }
 
+// The multi-determinant G of one spin from its R matrix and the reference G
+// (PHMSD.cpp:731-742, 768-771, 775-784): G[w][row0 + a][k] = sum_i R[w][i][a] GA[w][i][k] / Ov[w]
+// (the reciprocal of Ov taken once per walker, as the reference's device path does).
+__global__ void k_ph_rg(int nel, int nact, int nmo, int ldrows, int row0, const cd* __restrict__ R,
+                        const cd* __restrict__ GA, const cd* __restrict__ Ov,
+                        cd* __restrict__ G) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int w = blockIdx.y;
+  if (e >= (long)nact * nmo) return;
+  const int a = (int)(e / nmo), k = (int)(e % nmo);
+  const cd* Rw = R + (long)w * nel * nact;
+  const cd* Gw = GA + (long)w * nel * nmo;
+  cd acc = cmk(0.0, 0.0);
+  for (int i = 0; i < nel; ++i) acc += Rw[(long)i * nact + a] * Gw[(long)i * nmo + k];
+  G[((long)w * ldrows + row0 + a) * nmo + k] = acc * cdiv(cmk(1.0, 0.0), Ov[w]);
+}
+
+// The full G of both spins from the compact one (csrmm 'T' with the orbital rows,
+// PHMSD.cpp:744-749, 786-791): Gf[w][s][l][k] = sum_a A_s[a][l] Gc[w][s nact + a][k].
+__global__ void k_ph_full_g(int nact, int nmo, const cd* __restrict__ A0,
+                            const cd* __restrict__ A1, const cd* __restrict__ Gc,
+                            cd* __restrict__ Gf) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int w = blockIdx.y, sp = blockIdx.z;
+  if (e >= (long)nmo * nmo) return;
+  const int l = (int)(e / nmo), k = (int)(e % nmo);
+  const cd* As = sp ? A1 : A0;
+  const cd* G = Gc + ((long)w * 2 * nact + (long)sp * nact) * nmo;
+  cd acc = cmk(0.0, 0.0);
+  for (int a = 0; a < nact; ++a) acc += As[(long)a * nmo + l] * G[(long)a * nmo + k];
+  Gf[(((long)w * 2 + sp) * nmo + l) * nmo + k] = acc;
+}
+
+// out[w] = log Ov[w] + log_ov[w]
+__global__ void k_ph_log_overlap(int nw, const cd* __restrict__ Ov, const cd* __restrict__ log_ov,
+                                 cd* __restrict__ out) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w < nw) out[w] = clog_(Ov[w]) + log_ov[w];
+}
+
 __global__ void k_ph_fill(long n, cd v, cd* __restrict__ x) {
   const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e < n) x[e] = v;
@@ -709,6 +749,35 @@ void launch_ph_fill(hipStream_t s, long n, double re, double im, double* x) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_ph_fill, ph_blocks(n), dim3(256), 0, s, n, cmk(re, im),
                      reinterpret_cast<cd*>(x));
+  AFQ_HIP(hipGetLastError());
+}
+
+void launch_ph_rg(hipStream_t s, int nw, int nel, int nact, int nmo, int ldrows, int row0,
+                  const double* R, const double* GA, const double* Ov, double* G) {
+  if (nw <= 0) return;
+  hipLaunchKernelGGL(k_ph_rg, dim3((unsigned)(((long)nact * nmo + 255) / 256), nw), dim3(256), 0,
+                     s, nel, nact, nmo, ldrows, row0, reinterpret_cast<const cd*>(R),
+                     reinterpret_cast<const cd*>(GA), reinterpret_cast<const cd*>(Ov),
+                     reinterpret_cast<cd*>(G));
+  AFQ_HIP(hipGetLastError());
+}
+
+void launch_ph_full_g(hipStream_t s, int nw, int nact, int nmo, const double* A0,
+                      const double* A1, const double* Gc, double* Gf) {
+  if (nw <= 0) return;
+  hipLaunchKernelGGL(k_ph_full_g, dim3((unsigned)(((long)nmo * nmo + 255) / 256), nw, 2),
+                     dim3(256), 0, s, nact, nmo, reinterpret_cast<const cd*>(A0),
+                     reinterpret_cast<const cd*>(A1), reinterpret_cast<const cd*>(Gc),
+                     reinterpret_cast<cd*>(Gf));
+  AFQ_HIP(hipGetLastError());
+}
+
+void launch_ph_log_overlap(hipStream_t s, int nw, const double* Ov, const double* log_ov,
+                           double* out) {
+  if (nw <= 0) return;
+  hipLaunchKernelGGL(k_ph_log_overlap, ph_blocks(nw), dim3(256), 0, s, nw,
+                     reinterpret_cast<const cd*>(Ov), reinterpret_cast<const cd*>(log_ov),
+                     reinterpret_cast<cd*>(out));
   AFQ_HIP(hipGetLastError());
 }
 

@@ -84,3 +84,36 @@ def test_ph_energy_c5_shape(ndet, max_rank):
     d.imag = (d.imag + np.pi) % (2 * np.pi) - np.pi
     assert np.abs(d).max() < 1e-9
     ph.close()
+
+
+@pytest.mark.parametrize("uhf,nw,block", [(False, 6, 32), (True, 11, 4)])
+def test_ph_density_matrix_and_vbias_match_oracle(uhf, nw, block):
+    rng = np.random.default_rng(80 + uhf)
+    s = ph_system(rng, 10, 4, 3, 9, 60, 3, uhf=uhf)
+    Pu, Pd, buf = walkers(rng, nw, 10, 4, 3)
+    orb = np.stack(s["A"][:2]) if uhf else s["A"][0][None]
+    ph = A.PHMSD(s["occs"], s["ci"], orb, s["h"], s["L"].transpose(1, 2, 0), 0.0, walker_block=block)
+    bd = A.to_device(buf)
+    for full in (False, True):
+        G, ov = ph.MixedDensityMatrix(bd, full=full)
+        Go, ovo = P.mixed_density_matrix(s["struct"], s["A"], Pu, Pd, compact=not full)
+        np.testing.assert_allclose(G.cpu().numpy(), Go, rtol=1e-10, atol=1e-11 * np.abs(Go).max())
+        d = ov.cpu().numpy() - ovo
+        d.imag = (d.imag + np.pi) % (2 * np.pi) - np.pi
+        assert np.abs(d).max() < 1e-10
+    v = ph.vbias(bd, 0.01)
+    vo = P.vbias(s["struct"], s["A"], s["Lna"], 0.01, Pu, Pd)
+    np.testing.assert_allclose(v.cpu().numpy(), vo, rtol=1e-10, atol=1e-11 * np.abs(vo).max())
+    ph.close()
+
+
+def test_ph_vbias_c5_shape():
+    rng = np.random.default_rng(9)
+    s = ph_system(rng, 64, 10, 10, 64, 1000, 4)
+    Pu, Pd, buf = walkers(rng, 40, 64, 10, 10)
+    ph = A.PHMSD(s["occs"], s["ci"], s["A"][0][None], s["h"], s["L"].transpose(1, 2, 0), 0.0,
+                 walker_block=32)
+    v = ph.vbias(A.to_device(buf), 0.005)
+    vo = P.vbias(s["struct"], s["A"], s["Lna"], 0.005, Pu, Pd)
+    np.testing.assert_allclose(v.cpu().numpy(), vo, rtol=1e-9, atol=1e-10 * np.abs(vo).max())
+    ph.close()

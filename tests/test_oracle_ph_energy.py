@@ -129,3 +129,46 @@ def test_single_determinant_reduces_to_the_reference_energy():
     Eb, ovb = brute_force(s, Phi_up, Phi_dn, 0.0)
     np.testing.assert_allclose(E, Eb, rtol=1e-11, atol=1e-12 * np.abs(Eb).max())
     np.testing.assert_allclose(np.exp(logov), 0.9 * ovb / 0.9, rtol=1e-11)
+
+
+def brute_force_density(sysd, Phi_up, Phi_dn):
+    """The full multi-determinant mixed density matrix of each spin,
+    sum_D conj(c_D) <D|phi> rho_D^T / sum_D conj(c_D) <D|phi> with rho_D = Phi (A_D Phi)^-1 A_D."""
+    A, nmo, nup = sysd["A"], sysd["nmo"], sysd["nup"]
+    nw = Phi_up.shape[0]
+    num = np.zeros((nw, 2, nmo, nmo), complex)
+    den = np.zeros(nw, complex)
+    for occ, c in zip(sysd["occs"], sysd["ci"]):
+        rows = [list(occ[:nup]), [o - nmo for o in occ[nup:]]]
+        for w in range(nw):
+            ov, rho = 1.0, []
+            for s, Phi in enumerate([Phi_up[w], Phi_dn[w]]):
+                Ad = A[s][rows[s]]
+                M = Ad @ Phi
+                ov *= np.linalg.det(M)
+                rho.append(Phi @ np.linalg.solve(M, Ad))
+            for s in range(2):
+                num[w, s] += np.conj(c) * ov * rho[s].T
+            den[w] += np.conj(c) * ov
+    return num / den[:, None, None, None], den
+
+
+@pytest.mark.parametrize("uhf", [False, True])
+def test_mixed_density_matrix_and_vbias_match_brute_force(uhf):
+    rng = np.random.default_rng(70 + uhf)
+    s = ph_system(rng, 8, 3, 2, 6, 30, 2, uhf=uhf)
+    nw = 4
+    Phi_up, Phi_dn = rand_c(rng, nw, 8, 3), rand_c(rng, nw, 8, 2)
+    Gf, logov = P.mixed_density_matrix(s["struct"], s["A"], Phi_up, Phi_dn, compact=False)
+    Gb, ovb = brute_force_density(s, Phi_up, Phi_dn)
+    np.testing.assert_allclose(Gf, Gb, rtol=1e-10, atol=1e-11 * np.abs(Gb).max())
+    np.testing.assert_allclose(np.exp(logov), ovb, rtol=1e-10)
+    # the compact form carries the same G through the orbital rows (A is square and unitary)
+    Gc, _ = P.mixed_density_matrix(s["struct"], s["A"], Phi_up, Phi_dn)
+    np.testing.assert_allclose(np.einsum("al,wak->wlk", s["A"][0], Gc[:, :8]), Gb[:, 0],
+                               rtol=1e-10, atol=1e-11 * np.abs(Gb).max())
+    # vbias: sqrt(dt) sum_s tr(L_n G_s^T) over the brute-force density
+    dt = 0.01
+    v = P.vbias(s["struct"], s["A"], s["Lna"], dt, Phi_up, Phi_dn)
+    vb = np.sqrt(dt) * np.einsum("nlk,wslk->wn", s["L"], Gb)
+    np.testing.assert_allclose(v, vb, rtol=1e-10, atol=1e-11 * np.abs(vb).max())
