@@ -3,8 +3,8 @@
 //   vbias  (Real3IndexFactorization.hpp:350-361): A = G [W][N*M], B = Lnak [K][N*M]  (complex)
 //   vHS    (Real3IndexFactorization.hpp:297-313): A = X [W][K],   B = Likn [M*M][K]   (real)
 //
-// Tiling for CDNA4: a 256-thread workgroup owns a 64 x TN tile of C (TN = 64, or 128 for the
-// FP32 form with real B), each 64-wide wavefront a 32 x TN/2 sub-tile made of 2 x TN/32 MFMA
+// Tiling for CDNA4: a 256-thread workgroup owns a 64 x TN tile of C (TN = 64 in every launch; the
+// template also takes 128), each 64-wide wavefront a 32 x TN/2 sub-tile made of 2 x TN/32 MFMA
 // 16x16x4 tiles with separate real/imag accumulators (4 real MFMAs per complex tile and k-step,
 // 2 when B is real). The workgroup stages each 64 x KS slice of A and TN x KS slice of B in LDS
 // once (every thread loads 1/256 of it; rows padded to KS + 1
@@ -206,9 +206,11 @@ void launch_gemm_nt_z(hipStream_t s, int precision, int m, int n, int k, double 
   kchunk = (kchunk + 31) & ~31;   // k chunks on whole staged slices (16 or 32 wide)
   splits = (k + kchunk - 1) / kchunk;
   if (splits < 1) splits = 1;
-  // FP32 with real B (vHS) takes 64 x 128 tiles: half the re-reads of A per output column (the
-  // FP64 form of that tile needs more than 128 VGPRs + 128 AGPRs and would run one wave per SIMD)
-  const int tn = (b_complex || precision == AFQ_FP64) ? 64 : 128;
+  // 64 x 64 tiles and 16-wide k slices everywhere: the vHS grid (512 x 4096 at c2) then has 512
+  // workgroups, two per CU; in the FP32 form 64 x 128 tiles with 32-wide slices (one workgroup
+  // per CU) measured 63 us against 56 us, and the FP32 vbias 43 us with 32-wide slices against
+  // 36 us
+  const int tn = 64;
   // real B (vHS, the large Cholesky tensor): column tiles fastest (XCD-local B)
   const int nmajor = b_complex ? 0 : 1;
   const unsigned mt = (m + kTile - 1) / kTile, nt = (n + tn - 1) / tn;
@@ -226,10 +228,10 @@ void launch_gemm_nt_z(hipStream_t s, int precision, int m, int n, int k, double 
                          kchunk, a, A, lda, (const double*)B, ldb, out, ldo, sstride, nmajor);
   } else {
     if (b_complex)
-      hipLaunchKernelGGL((k_gemm_nt<float, true, 32, 64>), grid, dim3(256), 0, s, m, n, k, kchunk,
+      hipLaunchKernelGGL((k_gemm_nt<float, true, 16, 64>), grid, dim3(256), 0, s, m, n, k, kchunk,
                          a, A, lda, (const float*)B, ldb, out, ldo, sstride, nmajor);
     else
-      hipLaunchKernelGGL((k_gemm_nt<float, false, 32, 128>), grid, dim3(256), 0, s, m, n, k,
+      hipLaunchKernelGGL((k_gemm_nt<float, false, 16, 64>), grid, dim3(256), 0, s, m, n, k,
                          kchunk, a, A, lda, (const float*)B, ldb, out, ldo, sstride, nmajor);
   }
   AFQ_HIP(hipGetLastError());
