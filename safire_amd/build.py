@@ -11,6 +11,10 @@ LIB = os.path.join(HERE, "libsafire_afq.so")
 SOURCES = ["afq_api.hip", "afq_phmsd.hip", "afq_rng.hip", "k_green.hip", "k_gemm.hip", "k_energy.hip", "k_exchange_bf16.hip",
            "k_exchange_i8.hip", "k_phmsd.hip", "k_propagate.hip", "k_qr.hip", "k_walker.hip",
            "k_setup.hip"]
+# k_green.hip unrolls the 32 Gauss-Jordan steps of its N <= 32 kernel in full (each step indexes
+# the register-resident rows with the step number); the default budget of a pragma-driven unroll
+# would leave the loop rolled and the rows in scratch memory.
+FILE_FLAGS = {"k_green.hip": ["-mllvm", "-pragma-unroll-threshold=1000000"]}
 # Packed FP32 VALU ops are switched off for every HIP translation unit.
 HIPFLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -24,7 +28,8 @@ def build(verbose: bool = False, jobs: int = 8) -> str:
     for src in SOURCES:
         obj = os.path.join(objdir, src.replace(".hip", ".o"))
         objs.append(obj)
-        cmd = ["hipcc", *HIPFLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+        cmd = ["hipcc", *HIPFLAGS, *FILE_FLAGS.get(src, []), "-c", os.path.join(CSRC, src), "-o",
+               obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE,

@@ -58,6 +58,43 @@ __device__ inline void stage_batch(cd* dst, const cd* __restrict__ src, int n, i
   }
 }
 
+template <int N_>
+__device__ inline double ror_d(double v) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = __builtin_amdgcn_update_dpp((unsigned)u, (unsigned)u, 0x120 + N_, 0xf, 0xf,
+                                                  false);
+  const unsigned hi = __builtin_amdgcn_update_dpp((unsigned)(u >> 32), (unsigned)(u >> 32),
+                                                  0x120 + N_, 0xf, 0xf, false);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+template <int N_>
+__device__ inline int ror_i(int v) {
+  return __builtin_amdgcn_update_dpp(v, v, 0x120 + N_, 0xf, 0xf, false);
+}
+
+// maximum over the 16 lanes of each DPP row (every lane ends with its row's maximum; NaN is
+// ignored unless a whole row is NaN)
+__device__ inline double row_max16(double v) {
+  v = fmax(v, ror_d<1>(v));
+  v = fmax(v, ror_d<2>(v));
+  v = fmax(v, ror_d<4>(v));
+  v = fmax(v, ror_d<8>(v));
+  return v;
+}
+
+__device__ inline int row_min16(int v) {
+  v = min(v, ror_i<1>(v));
+  v = min(v, ror_i<2>(v));
+  v = min(v, ror_i<4>(v));
+  v = min(v, ror_i<8>(v));
+  return v;
+}
+
+__device__ inline int lowest_lane(unsigned long long mask) {
+  return __builtin_ctzll(mask);
+}
+
 template <int NMAX, bool STAGE>
 __global__ __launch_bounds__(64) void k_green(int M, int N, const cd* __restrict__ PsiT,
                                               const cd* __restrict__ Phi, long ldw,
@@ -153,6 +190,7 @@ __global__ __launch_bounds__(64) void k_green(int M, int N, const cd* __restrict
   }
   cd upiv = cmk(1.0, 0.0);     // lane j keeps +-U_jj; the logs are taken in parallel afterwards
   bool singular = false;
+  int rowpos = 0;              // N > 16: the getrf position of this lane's row
   if constexpr (NMAX <= 16) {
     // Step j is unrolled, so column j is a register of lane j: lane j finds the pivot in its own
     // column (izamax: first maximum of |re|+|im| from row j down) and readlane broadcasts it; the
@@ -210,64 +248,83 @@ __global__ __launch_bounds__(64) void k_green(int M, int N, const cd* __restrict
       }
     }
   } else {
-    // N > 16: the unrolled form exceeds the unroll budget; step j stays a loop variable and
-    // the pivot search is a wavefront reduction over the column broadcast through LDS
-    for (int j = 0; j < N; ++j) {
-      wave_sync();
-      if (lane == j)
+    // 16 < N <= 32: a row layout that keeps x[] in registers through fully unrolled steps. Lane
+    // (r, h) = (lane % 32, lane / 32) holds columns 32h..32h+31 of row r of [T | I] (T in
+    // columns 0..31, I in 32..63). Rows are never moved: each row carries its getrf position,
+    // the pivot is chosen with izamax's rule over the rows at positions >= j (first maximum of
+    // |re|+|im| by position; a NaN at position j is kept, a later NaN is skipped), and positions
+    // are swapped as getrf swaps rows, so the eliminations, pivots and signs are getrf's. Each
+    // step takes two 16-lane DPP maxima, a few readlanes and one LDS round trip for the pivot
+    // row and the multipliers.
+    __shared__ cd sRow[2][64];
+    __shared__ cd sF[2][32];
+    const int r = lane & 31, h = lane >> 5;
 #pragma unroll
-        for (int r = 0; r < NMAX; ++r)
-          if (r < N) sCol[0][r] = x[r];
-      wave_sync();
-      double v = -1.0;
-      int idx = 0x7fffffff;
-      if (lane >= j && lane < N) {
-        v = cabs1(sCol[0][lane]);
-        idx = lane;
-      }
-      for (int off = 32; off > 0; off >>= 1) {
-        const double ov_ = __shfl_xor(v, off, 64);
-        const int oi = __shfl_xor(idx, off, 64);
-        if (ov_ > v || (ov_ == v && oi < idx)) {
-          v = ov_;
-          idx = oi;
+    for (int e = 0; e < NMAX; ++e) {
+      cd v = cmk(0.0, 0.0);
+      if (r < N) {
+        if (h == 0) {
+          if (e < N) v = sT[r * NB + e];
+        } else if (e == r) {
+          v = cmk(1.0, 0.0);
         }
       }
-      const int p = idx;
-      const cd piv = sCol[0][p];
-      if (lane == j) upiv = (p == j) ? piv : -piv;   // the sign flips where the pivot row moved
-      if (cis0(piv)) {
-        singular = true;   // getrf records the zero pivot and goes on; getri_or_zero zeroes T^-1
-        continue;
-      }
-      if (p != j) {
-        cd xj = cmk(0, 0), xp = cmk(0, 0);
+      x[e] = v;
+    }
+    int pos = r;
 #pragma unroll
-        for (int r = 0; r < NMAX; ++r) {
-          if (r == j) xj = x[r];
-          if (r == p) xp = x[r];
+    for (int j = 0; j < NMAX; ++j) {
+      if (j < N) {
+        const bool cand = h == 0 && r < N && pos >= j;
+        const double v = cand ? cabs1(x[j]) : -1.0;
+        const int q = lowest_lane(__ballot(h == 0 && pos == j));
+        const double vq = readlane_d(v, q);
+        int p;
+        if (isnan(vq)) {
+          p = q;
+        } else {
+          const double m16 = row_max16(v);
+          const double m = fmax(readlane_d(m16, 0), readlane_d(m16, 16));
+          const unsigned long long mask = __ballot(cand && v == m);
+          if (__builtin_popcountll(mask) == 1) {
+            p = lowest_lane(mask);
+          } else {
+            const int k16 = row_min16(((mask >> lane) & 1ull) ? pos : 0x7fffffff);
+            const int km = min(__builtin_amdgcn_readlane(k16, 0), __builtin_amdgcn_readlane(k16, 16));
+            p = lowest_lane(__ballot(h == 0 && pos == km));
+          }
         }
+        p = __builtin_amdgcn_readfirstlane(p);
+        const cd piv = cmk(readlane_d(x[j].re, p), readlane_d(x[j].im, p));
+        const int pp = __builtin_amdgcn_readlane(pos, p);
+        if (lane == j) upiv = (pp == j) ? piv : -piv;   // the sign flips where getrf swaps rows
+        if (cis0(piv)) {
+          singular = true;   // getrf records the zero pivot and goes on; getri_or_zero zeroes it
+        } else {
+          if (r == q) pos = pp;
+          if (r == p) pos = j;
+          cd* row = sRow[j & 1];
+          cd* fv = sF[j & 1];
+          if (r == p)
 #pragma unroll
-        for (int r = 0; r < NMAX; ++r) {
-          if (r == j) x[r] = xp;
-          if (r == p) x[r] = xj;
-        }
-      }
-      const cd rinv = cdiv(cmk(1.0, 0.0), piv);
-      cd xjn = cmk(0, 0);
+            for (int e = 0; e < NMAX; ++e) row[32 * h + e] = x[e];
+          if (h == 0) fv[r] = x[j];
+          wave_sync();
+          const cd inv = cdiv(cmk(1.0, 0.0), piv);
+          const cd f = fv[r];
+          if (r == p) {
 #pragma unroll
-      for (int r = 0; r < NMAX; ++r)
-        if (r == j) xjn = x[r] * rinv;
+            for (int e = 0; e < NMAX; ++e) x[e] = row[32 * h + e] * inv;
+          } else if (!cis0(f)) {
+            const cd mlt = f * inv;
 #pragma unroll
-      for (int r = 0; r < NMAX; ++r) {
-        if (r < N) {
-          const int rr = (r == j) ? p : ((r == p) ? j : r);   // row r's column-j entry, pre-swap
-          const cd f = sCol[0][rr];
-          if (r == j) x[r] = xjn;
-          else if (!cis0(f)) x[r] -= f * xjn;
+            for (int e = 0; e < NMAX; ++e) x[e] -= mlt * row[32 * h + e];
+          }
         }
       }
     }
+    // row r is row pos of [I | T^-1]; lanes (r, 1) publish T^-1 after the zero fill below
+    rowpos = pos;
   }
 
   // sum_j log(+-U_jj + DBL_MIN): one log per lane, then a wavefront sum
@@ -295,11 +352,18 @@ __global__ __launch_bounds__(64) void k_green(int M, int N, const cd* __restrict
   wave_sync();
   for (int e = lane; e < NB * NB; e += 64) sT[e] = cmk(0, 0);
   wave_sync();
-  if (lane >= N && lane < 2 * N && !singular) {
-    const int a = lane - N;
+  if constexpr (NMAX <= 16) {
+    if (lane >= N && lane < 2 * N && !singular) {
+      const int a = lane - N;
 #pragma unroll
-    for (int r = 0; r < NMAX; ++r)
-      if (r < N) sT[r * NB + a] = x[r];
+      for (int r = 0; r < NMAX; ++r)
+        if (r < N) sT[r * NB + a] = x[r];
+    }
+  } else {
+    if (lane >= 32 && (lane & 31) < N && !singular)
+#pragma unroll
+      for (int e = 0; e < NMAX; ++e)
+        if (e < N) sT[rowpos * NB + e] = x[e];
   }
   wave_sync();
 
@@ -382,43 +446,6 @@ __global__ __launch_bounds__(64) void k_green(int M, int N, const cd* __restrict
 //   * wavefronts 1-3 compute the 16 x 16 tiles of G (and the E1 partial sums) while wavefront 0
 //     takes the logarithms of the pivots.
 constexpr int kGWaves = 4;
-
-template <int N_>
-__device__ inline double ror_d(double v) {
-  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
-  const unsigned lo = __builtin_amdgcn_update_dpp((unsigned)u, (unsigned)u, 0x120 + N_, 0xf, 0xf,
-                                                  false);
-  const unsigned hi = __builtin_amdgcn_update_dpp((unsigned)(u >> 32), (unsigned)(u >> 32),
-                                                  0x120 + N_, 0xf, 0xf, false);
-  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
-}
-
-template <int N_>
-__device__ inline int ror_i(int v) {
-  return __builtin_amdgcn_update_dpp(v, v, 0x120 + N_, 0xf, 0xf, false);
-}
-
-// maximum over the 16 lanes of each DPP row (every lane ends with its row's maximum; NaN is
-// ignored unless a whole row is NaN)
-__device__ inline double row_max16(double v) {
-  v = fmax(v, ror_d<1>(v));
-  v = fmax(v, ror_d<2>(v));
-  v = fmax(v, ror_d<4>(v));
-  v = fmax(v, ror_d<8>(v));
-  return v;
-}
-
-__device__ inline int row_min16(int v) {
-  v = min(v, ror_i<1>(v));
-  v = min(v, ror_i<2>(v));
-  v = min(v, ror_i<4>(v));
-  v = min(v, ror_i<8>(v));
-  return v;
-}
-
-__device__ inline int lowest_lane(unsigned long long mask) {
-  return __builtin_ctzll(mask);
-}
 
 // e = q d + rem for 0 <= e < 2^24 and 0 < d, from a float estimate corrected by one step
 __device__ inline void divmod(int e, int d, float rcp, int& q, int& rem) {

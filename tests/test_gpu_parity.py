@@ -106,7 +106,7 @@ def test_singular_overlap_gives_zero_G():
     np.testing.assert_allclose(host(ov), ovref, rtol=1e-14)
 
 
-@pytest.mark.parametrize("M,N", [(24, 5), (64, 16), (40, 12)])
+@pytest.mark.parametrize("M,N", [(24, 5), (64, 16), (40, 12), (40, 17), (48, 24), (128, 32)])
 def test_mixed_density_matrix_pivoting(M, N):
     """Overlap matrices that force row exchanges and ties in the pivot search: permutation
     matrices (one candidate per column, a swap at most steps) and small-integer matrices, whose
