@@ -312,13 +312,22 @@ __global__ __launch_bounds__(64) void k_green(int M, int N, const cd* __restrict
           wave_sync();
           const cd inv = cdiv(cmk(1.0, 0.0), piv);
           const cd f = fv[r];
-          if (r == p) {
+          const bool upd = r != p && !cis0(f);
+          const cd mlt = f * inv;
+          // the pivot row in chunks of 8 columns: each chunk's LDS reads are all in flight before
+          // the first use (one wait per chunk rather than one per column)
 #pragma unroll
-            for (int e = 0; e < NMAX; ++e) x[e] = row[32 * h + e] * inv;
-          } else if (!cis0(f)) {
-            const cd mlt = f * inv;
+          for (int e0 = 0; e0 < NMAX; e0 += 8) {
+            cd pr[8];
 #pragma unroll
-            for (int e = 0; e < NMAX; ++e) x[e] -= mlt * row[32 * h + e];
+            for (int e = 0; e < 8; ++e) pr[e] = row[32 * h + e0 + e];
+            if (r == p) {
+#pragma unroll
+              for (int e = 0; e < 8; ++e) x[e0 + e] = pr[e] * inv;
+            } else if (upd) {
+#pragma unroll
+              for (int e = 0; e < 8; ++e) x[e0 + e] -= mlt * pr[e];
+            }
           }
         }
       }
