@@ -136,6 +136,31 @@ def eloc(A, torch, ham, wset):
     return E.cpu().numpy()
 
 
+def contraction_errors(A, torch, ham64, hamf, wset):
+    """Relative error (max |x - x64| / max |x64|) of a form's vbias and vHS against FP64 on the
+    same walker state: vbias from the state's G, vHS from that vbias as X."""
+    W = wset.W
+    G = torch.zeros((W, ham64.N * ham64.M), dtype=torch.complex128, device="cuda")
+    ov = torch.zeros(W, dtype=torch.complex128, device="cuda")
+    A.check(A.lib().afq_mixed_density_matrix(ham64.handle, W, A.ptr(wset.buf), wset.ldw,
+                                             A.ptr(G), A.ptr(ov), A.stream_ptr()))
+    out = {}
+    v = {}
+    for key, h in (("fp64", ham64), ("form", hamf)):
+        h.reserve(W)
+        v[key] = torch.zeros((W, h.K), dtype=torch.complex128, device="cuda")
+        h.vbias(G, v[key])
+    V = {}
+    for key, h in (("fp64", ham64), ("form", hamf)):
+        V[key] = torch.zeros((W, h.M * h.M), dtype=torch.complex128, device="cuda")
+        h.vHS(v["fp64"], V[key])
+    torch.cuda.synchronize()
+    for name, d in (("vbias", v), ("vHS", V)):
+        ref = d["fp64"]
+        out[name] = float((d["form"] - ref).abs().max() / ref.abs().max())
+    return out
+
+
 def cpu_baseline(s, W, ortho, seconds):
     """The FP64 oracle (C++, OpenMP over walkers) on a bounded sample of the same workload."""
     import numpy as np
@@ -294,6 +319,7 @@ def main(argv=None):
     # projection keeps weight 0 and enters no estimator)
     ref_form = forms.get("fp64")
     gate = {}
+    cerr = {}
     if ref_form is not None:
         P = M * N
         ham64 = ref_form["ham"]
@@ -318,8 +344,14 @@ def main(argv=None):
                 worst = max(worst, float(np.max(np.abs(E64 - Ef)[live])))
                 nlive += int(live.sum())
             gate[f] = worst
+            cerr[f] = {}
+            for st in states:
+                for k, e in contraction_errors(A, torch, ham64, forms[f]["ham"], st).items():
+                    cerr[f][k] = max(cerr[f].get(k, 0.0), e)
             print(f"[bench] {f}: max |E_loc({f}) - E_loc(fp64)| = {worst:.3e} Ha over {nlive} "
-                  f"live walker states", file=sys.stderr, flush=True)
+                  f"live walker states; relative error of vbias {cerr[f]['vbias']:.2e}, "
+                  f"vHS {cerr[f]['vHS']:.2e} (G is FP64 in every form)", file=sys.stderr,
+                  flush=True)
     passing = [f for f in forms if f == "fp64" or gate.get(f, 1.0) < GATE_HA]
     # the headline: the fastest form that meets the accuracy gate
     mixed = min(passing, key=lambda f: forms[f]["ms"])
@@ -370,6 +402,8 @@ def main(argv=None):
                        "exchange_frac_fp64_peak": round(exchange_flops(W, M, N, K) / (ex64_avg / 1e3) / 1e12 / PEAK_TFLOPS["fp64"], 4)}
         out["mixed_over_fp64"] = round(value / v64, 3)
     out["max_abs_dEloc_Ha"] = {f: gate[f] for f in gate}
+    out["contraction_rel_err_vs_fp64"] = {f: {k: float(f"{e:.3g}") for k, e in c.items()}
+                                          for f, c in cerr.items()}
     out["headline_form"] = mixed
     out["forms_walker_steps_per_s"] = {f: round(W * args.steps / (r["ms"] / 1e3), 1)
                                        for f, r in forms.items()}

@@ -6,7 +6,9 @@ equilibration steps, QR and population control every 10 steps, energy every 10 s
 propagation, energies recomputed by the estimator). The inputs and the reference's means and
 errors are fixtures converted from its HDF5 files (tests/golden/make_bh_fixtures.py).
 
-The same run here: 2048 walkers on one GPU, FP64, Gaussian fields from a seeded device
+The same run here: 2048 walkers on one GPU, in the FP64 form and in the int8-split form (FP32
+vbias/vHS, int8-split exchange in the estimator's energies, FP64 elsewhere), Gaussian fields
+from a seeded device
 generator (the run is deterministic for a given seed; field-for-field parity with the
 reference's generator is a separate mode). The mean of every energy observable must lie within
 4 combined standard errors of the reference's; this run's error comes from 30 blocks of
@@ -29,7 +31,8 @@ from safire_amd import afqmc as A  # noqa: E402
 FIX = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "bh_rhf_closed.npz")
 
 
-def test_bh_rhf_closed_energy_matches_reference_statistics():
+@pytest.mark.parametrize("precision", [A.FP64, A.INT8_SPLIT])
+def test_bh_rhf_closed_energy_matches_reference_statistics(precision):
     d = np.load(FIX)
     inp = json.loads(str(d["run"]))
     run = inp["execute"]
@@ -38,11 +41,14 @@ def test_bh_rhf_closed_energy_matches_reference_statistics():
     W = 2048
     ham = A.HamiltonianOperations(d["hcore"], d["Likn"], d["PsiT"], float(d["E0"]), dt)
     ham.set_params(hybrid=1)
+    ham.set_precision(precision)
     prop = A.Propagator(ham)
     wset = A.WalkerSet(ham, W)
     prop.seed(wset)
     E_trial = ES.average_energy(prop.estimator_sums(wset).cpu().numpy())
-    assert E_trial == pytest.approx(-25.06369204663835, abs=1e-10)      # RHF energy (closed form)
+    # RHF energy (closed form); the int8 split's exchange is exact to ~2^-39 of the row maxima
+    tol = 1e-10 if precision == A.FP64 else 1e-7
+    assert E_trial == pytest.approx(-25.06369204663835, abs=tol)
     gen = torch.Generator(device="cuda")
     gen.manual_seed(inp["seed"])
     xi = torch.empty((W, ham.K), dtype=torch.float64, device="cuda")
