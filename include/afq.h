@@ -294,6 +294,12 @@ int afq_ph_create(int nmo, int nup, int ndn, int nchol, int ndet, const int* occ
                   const double* ci, int nref, const double* orbmats, const double* h1,
                   const double* likn, double E0, int walker_block, afq_ph* out);
 void afq_ph_destroy(afq_ph ph);
+/* PHMSD::vMF (PHMSD.cpp:1329-1565): d_vmf [nchol] c128 = sqrt(dt) sum_s sum_{a,k} Lna_s[n][a][k]
+ * G_MF,s[a][k], with G_MF the trial's mean-field density formed on the host as the reference's
+ * host path forms it (configuration overlaps log det(As^H As), diagonal terms, one-orbital
+ * pair terms within rows of the other spin's couplings, division by sum |c|^2 ov_up ov_dn).
+ * Synchronises the stream (the host part needs no device data; the upload is ordered on it). */
+int afq_ph_vmf(afq_ph ph, double dt, double* d_vmf, void* stream);
 /* Unique excitations per spin (the reference counted as index 0) and non-empty ranks. */
 int afq_ph_info(afq_ph ph, int* nunique, int* nshells);
 /* PHMSD::energy / energy_alg1 (PHMSD.cpp:377-544): d_E [nw][3] c128 (E1, EXX, EJ of the

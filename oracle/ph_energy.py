@@ -229,3 +229,74 @@ def vbias(struct, A, Lna, dt, Phi_up, Phi_dn):
     na = A[0].shape[0]
     return np.sqrt(dt) * (np.einsum("wak,nak->wn", Gc[:, :na], Lna[0], optimize=True) +
                           np.einsum("wak,nak->wn", Gc[:, na:], Lna[1], optimize=True))
+
+
+def configurations(struct, s):
+    """get_configuration (Excitations.hpp:486-492): the reference occupation with each unique
+    excitation of spin s applied in place (index 0 = the reference)."""
+    ref = [int(o) for o in struct["refc"][s]]
+    out = [list(ref)]
+    for sh in struct["shells"][s]:
+        n = sh.shape[1] // 2
+        for e in sh:
+            c = list(ref)
+            for p in range(n):
+                c[int(e[p])] = int(e[n + p])
+            out.append(c)
+    return out
+
+
+def vmf(struct, A, Lna, dt):
+    """PHMSD::vMF (PHMSD.cpp:1329-1565), COLLINEAR, as the reference computes it: overlaps of
+    every unique configuration relative to the reference (log det(As^H As)); the diagonal term of
+    each configuration (its own compact G, weighted by its overlap times sum_n |coupling|^2 x the
+    other spin's overlap); the off-diagonal term of every pair of one spin's configurations that
+    share a row of the other spin's couplings and differ by one orbital (weight c1 conj(c2), the
+    orbital rows added as the reference adds them, no fermionic sign); division by
+    sum |c|^2 ov_up ov_dn; then vbias with sqrt(dt)."""
+    nact, nmo = A[0].shape
+    confs = [configurations(struct, 0), configurations(struct, 1)]
+    ratio = []
+    for s in range(2):
+        Orbs = A[s].conj().T
+        logs = []
+        for c in confs[s]:
+            As = Orbs[:, c]
+            sign, ld = np.linalg.slogdet(As.conj().T @ As)
+            logs.append(ld + np.log(sign))
+        logs = np.array(logs)
+        r = np.exp(logs - logs[0])
+        r[0] = 1.0
+        ratio.append(r)
+    cp0, ci0, cv0 = struct["csr"][0]
+    ov = 0.0
+    for u in range(len(cp0) - 1):
+        for j in range(cp0[u], cp0[u + 1]):
+            ov += abs(cv0[j]) ** 2 * ratio[0][u] * ratio[1][ci0[j]]
+    G = np.zeros((2, nact, nmo), np.complex128)
+    for s in range(2):
+        o = 1 - s
+        Orbs = A[s].conj().T
+        cp, ci, cv = struct["csr"][s]
+        for nd, c in enumerate(confs[s]):
+            As = Orbs[:, c]
+            G2 = np.linalg.inv(As.conj().T @ As).T @ As.T
+            w = ratio[s][nd] * sum(ratio[o][ci[j]] * abs(cv[j]) ** 2 for j in range(cp[nd], cp[nd + 1]))
+            for i, orb in enumerate(c):
+                G[s, orb] += w * G2[i]
+        cpo, cio, cvo = struct["csr"][o]
+        for nd in range(len(cpo) - 1):
+            for n1 in range(cpo[nd], cpo[nd + 1]):
+                c1 = sorted(confs[s][cio[n1]])
+                for n2 in range(n1 + 1, cpo[nd + 1]):
+                    c2 = sorted(confs[s][cio[n2]])
+                    holes = [i for i, x in enumerate(c1) if x not in c2]
+                    if len(holes) != 1:
+                        continue
+                    part = [x for x in c2 if x not in c1][0]
+                    w = cvo[n1] * np.conj(cvo[n2])
+                    o0 = c1[holes[0]]
+                    G[s, part] += w * A[s][o0]
+                    G[s, o0] += np.conj(w) * A[s][part]
+    G /= ov
+    return np.sqrt(dt) * (np.einsum("ak,nak->n", G[0], Lna[0]) + np.einsum("ak,nak->n", G[1], Lna[1]))

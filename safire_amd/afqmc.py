@@ -90,6 +90,7 @@ SIGNATURES = {
     "afq_ph_energy": (_i, [_vp, _i, _vp, _l, _vp, _vp, _vp]),
     "afq_ph_mixed_density_matrix": (_i, [_vp, _i, _vp, _l, _i, _vp, _vp, _vp]),
     "afq_ph_vbias": (_i, [_vp, _i, _vp, _l, _d, _vp, _vp]),
+    "afq_ph_vmf": (_i, [_vp, _d, _vp, _vp]),
     "afq_ph_compact_R": (_i, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp]),
     "afq_ph_accumulate_R": (_i, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "afq_split_seed": (C.c_ulonglong, [_i, _i, C.c_uint]),
@@ -310,6 +311,13 @@ class PHMSD:
                                                 int(bool(full)), ptr(G), ptr(ov),
                                                 stream_ptr(stream)))
         return G, ov
+
+    def vMF(self, dt, stream=None):
+        """PHMSD::vMF (PHMSD.cpp:1329-1565): the mean-field potential [nchol]."""
+        torch = _torch()
+        v = torch.empty(self.nchol, dtype=torch.complex128, device="cuda")
+        check(lib().afq_ph_vmf(self.handle, float(dt), ptr(v), stream_ptr(stream)))
+        return v
 
     def vbias(self, wbuf, dt, v=None, stream=None):
         """PHMSD::vbias (PHMSD.cpp:53-72): v [W][nchol]."""

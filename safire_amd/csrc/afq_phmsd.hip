@@ -17,12 +17,15 @@
 //                    numerically irrelevant: every kernel is the GPU form of one reference call
 //                    (k_phmsd.hip), in the reference's order.
 //   afq_ph_mixed_density_matrix = PHMSD::MixedDensityMatrix (PHMSD.cpp:549-795), compact or full.
+//   afq_ph_vmf     = PHMSD::vMF (PHMSD.cpp:1329-1565): the trial's mean-field density on the host
+//                    (as the reference's host path forms it), then vbias on the GPU.
 //   afq_ph_vbias   = PHMSD::vbias (PHMSD.cpp:53-72): the compact G of both spins contracted with
 //                    the half-rotated Cholesky vectors of both spins stacked per vector
 //                    (Real3IndexFactorization.hpp:328-361, sqrt(dt) for COLLINEAR walkers).
 #include "afq_internal.hpp"
 
 #include <algorithm>
+#include <complex>
 #include <memory>
 #include <new>
 #include <vector>
@@ -43,6 +46,10 @@ struct Shell {
 
 struct Spin {
   int nel = 0, nunique = 1;
+  std::vector<std::vector<int>> confs;          // host: occupation of every unique excitation
+  std::vector<int> hcp, hci;                    // host copy of the couplings (CSR)
+  std::vector<std::complex<double>> hcv;
+  std::vector<std::complex<double>> hA;         // host copy of the orbital rows [nact][nmo]
   double* A = nullptr;             // c128 [nact][nmo]
   int* refc = nullptr;             // [nel]
   double* Lna = nullptr;           // c128 [nchol][nact][nmo]
@@ -89,6 +96,48 @@ int excitation_of(const std::vector<int>& ref, const std::vector<int>& conf, std
       }
   ex.insert(ex.end(), parts.begin(), parts.end());
   return sign;
+}
+
+using zc = std::complex<double>;
+
+// LU with partial pivoting of a [n][n] row-major matrix in place; returns log det (the sign of
+// the row exchanges included) and leaves the inverse in inv when it is given
+zc host_logdet_inverse(std::vector<zc> a, int n, std::vector<zc>* inv) {
+  std::vector<int> piv(n);
+  zc ld = 0.0;
+  for (int j = 0; j < n; ++j) {
+    int p = j;
+    for (int r = j + 1; r < n; ++r)
+      if (std::abs(a[r * n + j]) > std::abs(a[p * n + j])) p = r;
+    piv[j] = p;
+    if (p != j) {
+      for (int c = 0; c < n; ++c) std::swap(a[j * n + c], a[p * n + c]);
+      ld += zc(0.0, M_PI);
+    }
+    const zc d = a[j * n + j];
+    ld += std::log(d);
+    for (int r = j + 1; r < n; ++r) {
+      const zc f = a[r * n + j] / d;
+      a[r * n + j] = f;
+      for (int c = j + 1; c < n; ++c) a[r * n + c] -= f * a[j * n + c];
+    }
+  }
+  if (inv) {
+    inv->assign((size_t)n * n, 0.0);
+    for (int col = 0; col < n; ++col) {
+      std::vector<zc> b(n, 0.0);
+      b[col] = 1.0;
+      for (int j = 0; j < n; ++j) std::swap(b[j], b[piv[j]]);
+      for (int r = 0; r < n; ++r)
+        for (int c = 0; c < r; ++c) b[r] -= a[r * n + c] * b[c];
+      for (int r = n - 1; r >= 0; --r) {
+        for (int c = r + 1; c < n; ++c) b[r] -= a[r * n + c] * b[c];
+        b[r] /= a[r * n + r];
+      }
+      for (int r = 0; r < n; ++r) (*inv)[(size_t)r * n + col] = b[r];
+    }
+  }
+  return ld;
 }
 
 }  // namespace
@@ -286,6 +335,14 @@ int afq_ph_create(int nmo, int nup, int ndn, int nchol, int ndet, const int* occ
         max_nex = std::max(max_nex, n);
       }
       S.nunique = row;
+      S.confs.assign(1, std::vector<int>());
+      for (int k = 0; k < nel[s]; ++k) S.confs[0].push_back(ref[s][k] - s * nmo);
+      for (int n = 1; n <= nel[s]; ++n)
+        for (const auto& e : unique[s][n]) {
+          std::vector<int> c = S.confs[0];
+          for (int p = 0; p < n; ++p) c[e[p]] = e[n + p];
+          S.confs.push_back(c);
+        }
       std::vector<int> rc(nel[s]);
       for (int k = 0; k < nel[s]; ++k) rc[k] = ref[s][k] - s * nmo;
       S.refc = dev_copy(own, rc.data(), rc.size());
@@ -313,6 +370,10 @@ int afq_ph_create(int nmo, int nup, int ndn, int nchol, int ndet, const int* occ
         }
         cp.push_back((int)cix.size());
       }
+      P->spin[s].hcp = cp;
+      P->spin[s].hci = cix;
+      for (size_t e = 0; e < cix.size(); ++e)
+        P->spin[s].hcv.emplace_back(cv[2 * e], cv[2 * e + 1]);
       P->spin[s].cp = dev_copy(own, cp.data(), cp.size());
       P->spin[s].ci = dev_copy(own, cix.data(), cix.size());
       P->spin[s].cv = dev_copy(own, cv.data(), cv.size());
@@ -325,6 +386,9 @@ int afq_ph_create(int nmo, int nup, int ndn, int nchol, int ndet, const int* occ
     for (int s = 0; s < 2; ++s) {
       Spin& S = P->spin[s];
       S.A = dev_copy(own, orbmats + (nref == 2 ? s : 0) * 2 * nmm, 2 * nmm);
+      const double* ho = orbmats + (nref == 2 ? s : 0) * 2 * nmm;
+      S.hA.resize(nmm);
+      for (size_t e = 0; e < nmm; ++e) S.hA[e] = {ho[2 * e], ho[2 * e + 1]};
       S.Lna = dev_alloc<double>(own, 2 * nmm * nchol);
       S.haj = dev_alloc<double>(own, 2 * nmm);
       launch_half_rotate(st, nmo, nmo, nchol, S.A, dh, dL, S.haj, S.Lna);
@@ -510,6 +574,113 @@ int afq_ph_vbias(afq_ph ph, int nw, const double* d_wbuf, long ldw, double dt, d
       launch_gemm_nt_z(s, AFQ_FP64, nb, K, kx, std::sqrt(dt), ph->Gblk, kx, ph->Lstack, kx, true,
                        d_v + 2L * w0 * K, K, ph->split, ph->split_elems);
     }
+  });
+}
+
+int afq_ph_vmf(afq_ph ph, double dt, double* d_vmf, void* stream) {
+  return ph_guard([&] {
+    AFQ_CHECK(ph != nullptr, AFQ_EINVAL, "afq_ph_vmf: null handle");
+    AFQ_CHECK(d_vmf != nullptr && dt > 0.0, AFQ_EINVAL, "afq_ph_vmf: null output or dt <= 0");
+    const int nmo = ph->nmo, na = ph->nact;
+    // 1. overlap of every unique configuration relative to the reference: log det(As^H As),
+    //    As the configuration's orbitals (columns of the conjugate-transposed orbital rows)
+    std::vector<zc> ratio[2];
+    for (int s = 0; s < 2; ++s) {
+      const Spin& S = ph->spin[s];
+      const int ne = S.nel;
+      std::vector<zc> lds;
+      for (const auto& c : S.confs) {
+        std::vector<zc> T((size_t)ne * ne, 0.0);
+        for (int i = 0; i < ne; ++i)
+          for (int j = 0; j < ne; ++j) {
+            zc acc = 0.0;
+            for (int k = 0; k < nmo; ++k)
+              acc += S.hA[(size_t)c[i] * nmo + k] * std::conj(S.hA[(size_t)c[j] * nmo + k]);
+            T[(size_t)i * ne + j] = acc;
+          }
+        lds.push_back(host_logdet_inverse(T, ne, nullptr));
+      }
+      for (size_t d = 0; d < lds.size(); ++d) ratio[s].push_back(d ? std::exp(lds[d] - lds[0]) : 1.0);
+    }
+    zc ov = 0.0;
+    {
+      const Spin& S = ph->spin[0];
+      for (int u = 0; u + 1 < (int)S.hcp.size(); ++u)
+        for (int j = S.hcp[u]; j < S.hcp[u + 1]; ++j)
+          ov += std::norm(S.hcv[j]) * ratio[0][u] * ratio[1][S.hci[j]];
+    }
+    // 2. diagonal and off-diagonal terms of each spin's density
+    std::vector<zc> G((size_t)2 * na * nmo, 0.0);
+    for (int s = 0; s < 2; ++s) {
+      const int o = 1 - s;
+      const Spin& S = ph->spin[s];
+      const Spin& O = ph->spin[o];
+      const int ne = S.nel;
+      zc* Gs = G.data() + (size_t)s * na * nmo;
+      for (size_t nd = 0; nd < S.confs.size(); ++nd) {
+        const auto& c = S.confs[nd];
+        // compact G of the configuration with itself: (As^H As)^-T As^T
+        std::vector<zc> T((size_t)ne * ne), Ti;
+        for (int i = 0; i < ne; ++i)
+          for (int j = 0; j < ne; ++j) {
+            zc acc = 0.0;
+            for (int k = 0; k < nmo; ++k)
+              acc += S.hA[(size_t)c[i] * nmo + k] * std::conj(S.hA[(size_t)c[j] * nmo + k]);
+            T[(size_t)i * ne + j] = acc;
+          }
+        host_logdet_inverse(T, ne, &Ti);
+        zc w = 0.0;
+        for (int j = S.hcp[nd]; j < S.hcp[nd + 1]; ++j)
+          w += ratio[o][S.hci[j]] * std::norm(S.hcv[j]);
+        w *= ratio[s][nd];
+        for (int i = 0; i < ne; ++i)
+          for (int k = 0; k < nmo; ++k) {
+            zc g = 0.0;
+            for (int b = 0; b < ne; ++b)
+              g += Ti[(size_t)b * ne + i] * std::conj(S.hA[(size_t)c[b] * nmo + k]);
+            Gs[(size_t)c[i] * nmo + k] += w * g;
+          }
+      }
+      // pairs of this spin's configurations in one row of the other spin's couplings that
+      // differ by one orbital (no fermionic sign, the orbital rows as the reference adds them)
+      for (int nd = 0; nd + 1 < (int)O.hcp.size(); ++nd)
+        for (int n1 = O.hcp[nd]; n1 < O.hcp[nd + 1]; ++n1) {
+          std::vector<int> c1 = S.confs[O.hci[n1]];
+          std::sort(c1.begin(), c1.end());
+          for (int n2 = n1 + 1; n2 < O.hcp[nd + 1]; ++n2) {
+            std::vector<int> c2 = S.confs[O.hci[n2]];
+            std::sort(c2.begin(), c2.end());
+            int nh = 0, hole = -1, part = -1;
+            for (int x : c1)
+              if (!std::binary_search(c2.begin(), c2.end(), x)) {
+                ++nh;
+                hole = x;
+              }
+            if (nh != 1) continue;
+            for (int x : c2)
+              if (!std::binary_search(c1.begin(), c1.end(), x)) part = x;
+            const zc w = O.hcv[n1] * std::conj(O.hcv[n2]);
+            for (int k = 0; k < nmo; ++k) {
+              Gs[(size_t)part * nmo + k] += w * S.hA[(size_t)hole * nmo + k];
+              Gs[(size_t)hole * nmo + k] += std::conj(w) * S.hA[(size_t)part * nmo + k];
+            }
+          }
+        }
+    }
+    for (auto& g : G) g /= ov;
+    // 3. vbias of the mean-field density: sqrt(dt) sum_s sum_{a,k} Lna_s[n][a][k] G_s[a][k]
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    std::vector<double> flat(2 * G.size());
+    for (size_t e = 0; e < G.size(); ++e) {
+      flat[2 * e] = G[e].real();
+      flat[2 * e + 1] = G[e].imag();
+    }
+    AFQ_HIP(hipMemcpyAsync(ph->Gblk, flat.data(), flat.size() * sizeof(double),
+                           hipMemcpyHostToDevice, st));
+    const int kx = 2 * na * nmo;
+    launch_gemm_nt_z(st, AFQ_FP64, 1, ph->nchol, kx, std::sqrt(dt), ph->Gblk, kx, ph->Lstack, kx,
+                     true, d_vmf, ph->nchol);
+    AFQ_HIP(hipStreamSynchronize(st));
   });
 }
 

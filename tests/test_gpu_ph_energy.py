@@ -117,3 +117,16 @@ def test_ph_vbias_c5_shape():
     vo = P.vbias(s["struct"], s["A"], s["Lna"], 0.005, Pu, Pd)
     np.testing.assert_allclose(v.cpu().numpy(), vo, rtol=1e-9, atol=1e-10 * np.abs(vo).max())
     ph.close()
+
+
+@pytest.mark.parametrize("uhf,nmo,nel,ndet,rank", [(False, 9, 4, 40, 3), (True, 10, 3, 60, 2),
+                                                    (False, 64, 10, 1000, 4)])
+def test_ph_vmf_matches_oracle(uhf, nmo, nel, ndet, rank):
+    rng = np.random.default_rng(100 + nmo)
+    s = ph_system(rng, nmo, nel, nel - 1, 12, ndet, rank, uhf=uhf)
+    orb = np.stack(s["A"][:2]) if uhf else s["A"][0][None]
+    ph = A.PHMSD(s["occs"], s["ci"], orb, s["h"], s["L"].transpose(1, 2, 0), 0.0)
+    v = ph.vMF(0.005).cpu().numpy()
+    vo = P.vmf(s["struct"], s["A"], s["Lna"], 0.005)
+    np.testing.assert_allclose(v, vo, rtol=1e-10, atol=1e-12 * np.abs(vo).max())
+    ph.close()

@@ -172,3 +172,63 @@ def test_mixed_density_matrix_and_vbias_match_brute_force(uhf):
     v = P.vbias(s["struct"], s["A"], s["Lna"], dt, Phi_up, Phi_dn)
     vb = np.sqrt(dt) * np.einsum("nlk,wslk->wn", s["L"], Gb)
     np.testing.assert_allclose(v, vb, rtol=1e-10, atol=1e-11 * np.abs(vb).max())
+
+
+def test_vmf_single_determinant_closed_form():
+    """One determinant: vMF_n = sqrt(dt) sum_s sum_{a occupied} (A_s L_n A_s^H)[a][a]."""
+    rng = np.random.default_rng(90)
+    s = ph_system(rng, 8, 3, 2, 5, 1, 1, uhf=True)
+    v = P.vmf(s["struct"], s["A"], s["Lna"], 0.01)
+    ref = np.zeros(5, complex)
+    for sp, ne in ((0, 3), (1, 2)):
+        M = np.einsum("al,nlk,bk->nab", s["A"][sp], s["L"], s["A"][sp].conj())
+        ref += np.einsum("naa->n", M[:, :ne, :ne])
+    np.testing.assert_allclose(v, 0.1 * ref, rtol=1e-12, atol=1e-14)
+
+
+def test_vmf_is_the_trial_density_when_no_pair_differs_by_one_orbital():
+    """Orthonormal orbitals and determinants that pairwise differ by two or more orbitals (in the
+    same spin, with the same other-spin part): <Psi|a+a|Psi> / <Psi|Psi> is then
+    sum_D |c_D|^2 rho_D / sum_D |c_D|^2 exactly, and vMF its vbias."""
+    rng = np.random.default_rng(91)
+    nmo, nup, ndn = 8, 3, 2
+    X = rng.standard_normal((nchol := 4, nmo, nmo))
+    L = (X + X.transpose(0, 2, 1)) / 2
+    Q, _ = np.linalg.qr(rng.standard_normal((nmo, nmo)))
+    A = [Q.T.copy(), Q.T.copy()]
+    occs = np.array([[0, 1, 2, 8, 9], [0, 4, 5, 8, 9], [3, 6, 7, 8, 9], [0, 1, 2, 12, 13]], np.int32)
+    ci = np.array([0.9, 0.3 - 0.1j, -0.2j, 0.25])
+    st = P.build_ph_struct(ci, occs, nmo, nup, ndn)
+    Lna = [np.einsum("al,nlk->nak", A[s], L) for s in range(2)]
+    v = P.vmf(st, A, Lna, 0.04)
+    rho = np.zeros((2, nmo, nmo))
+    for occ, c in zip(occs, ci):
+        for sp, orbs in ((0, occ[:nup]), (1, occ[nup:] - nmo)):
+            rho[sp] += abs(c) ** 2 * sum(np.outer(Q[:, o], Q[:, o]) for o in orbs)
+    rho /= np.sum(np.abs(ci) ** 2)
+    ref = 0.2 * np.einsum("nlk,slk->n", L, rho)
+    np.testing.assert_allclose(v, ref, rtol=1e-12, atol=1e-13)
+
+
+def test_vmf_single_excitation_pair_matches_the_exact_density():
+    """Two determinants of real orthonormal orbitals differing by the up orbital 2 -> 5 (no occupied
+    orbital between the two in the sorted order, so the fermionic sign is +1 and the reference's
+    unsigned pair term is exact): vMF is the vbias of the exact trial density matrix
+    (|c1|^2 rho1 + |c2|^2 rho2 + c1 c2 (phi2 phi5^T + phi5 phi2^T)) / (|c1|^2 + |c2|^2)."""
+    rng = np.random.default_rng(92)
+    nmo, nup, ndn = 7, 3, 2
+    X = rng.standard_normal((4, nmo, nmo))
+    L = (X + X.transpose(0, 2, 1)) / 2
+    Q, _ = np.linalg.qr(rng.standard_normal((nmo, nmo)))
+    A = [Q.T.copy(), Q.T.copy()]
+    occs = np.array([[0, 1, 2, 7, 8], [0, 1, 5, 7, 8]], np.int32)
+    c1, c2 = 0.9, -0.4
+    st = P.build_ph_struct(np.array([c1, c2], complex), occs, nmo, nup, ndn)
+    Lna = [np.einsum("al,nlk->nak", A[s], L) for s in range(2)]
+    v = P.vmf(st, A, Lna, 1.0)
+    P_ = lambda orbs: sum(np.outer(Q[:, o], Q[:, o]) for o in orbs)
+    up = c1 ** 2 * P_([0, 1, 2]) + c2 ** 2 * P_([0, 1, 5]) + c1 * c2 * (np.outer(Q[:, 2], Q[:, 5]) + np.outer(Q[:, 5], Q[:, 2]))
+    dn = (c1 ** 2 + c2 ** 2) * P_([0, 1])
+    rho = np.stack([up, dn]) / (c1 ** 2 + c2 ** 2)
+    ref = np.einsum("nlk,slk->n", L, rho)
+    np.testing.assert_allclose(v, ref, rtol=1e-12, atol=1e-13)
