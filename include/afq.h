@@ -300,6 +300,28 @@ void afq_ph_destroy(afq_ph ph);
  * pair terms within rows of the other spin's couplings, division by sum |c|^2 ov_up ov_dn).
  * Synchronises the stream (the host part needs no device data; the upload is ordered on it). */
 int afq_ph_vmf(afq_ph ph, double dt, double* d_vmf, void* stream);
+/* The PHMSD walker step (COLLINEAR walkers: rows of ldw c128 holding Phi_up [nmo][nup],
+ * Phi_dn [nmo][ndn], then the AFQ_P_COUNT properties of the NOMSD layout).
+ * afq_ph_prepare: the propagator for time step dt (AFQMCBasePropagator.cpp:85-157: vMF of the
+ * trial with its imaginary part dropped, v0, H1 and P1 = exp(-H1/2)) and the workspace for up to
+ * W walkers; the step parameters start at afq_default_params. Synchronises the stream. */
+int afq_ph_prepare(afq_ph ph, double dt, int W, void* stream);
+int afq_ph_set_params(afq_ph ph, const afq_step_params* p);
+/* OVLP, E1/EXX/EJ and the pseudo energy of every walker from the PHMSD energy (as
+ * afq_seed_properties does for NOMSD). */
+int afq_ph_seed_properties(afq_ph ph, int W, double* d_wbuf, long ldw, void* stream);
+/* AFQMCBasePropagator::Propagate (AFQMCBasePropagator.cpp:289-392) with the PHMSD trial:
+ * PHMSD::vbias, assemble_X with the mean-field factor, vHS = sqrt(dt) L X, both spins'
+ * Slater matrices propagated (P1, Taylor exp(iV), P1), the new PHMSD overlap (hybrid) or
+ * PHMSD::energy, walker_update and the weight bound. d_xi [W][nchol] f64 N(0,1) fields. */
+int afq_ph_step(afq_ph ph, int W, double* d_wbuf, long ldw, const double* d_xi, double Eshift,
+                void* stream);
+/* Orthogonalize (AFQMCBasePropagator.cpp:735-741): QR of both spins' Slater matrices, then the
+ * PHMSD overlap into OVLP. */
+int afq_ph_orthogonalize(afq_ph ph, int W, double* d_wbuf, long ldw, void* stream);
+/* population_control (population_control.hpp:34-257) over the COLLINEAR walker layout. */
+int afq_ph_population_control(afq_ph ph, int W, double* d_wbuf, long ldw, double uniform,
+                              void* stream);
 /* Unique excitations per spin (the reference counted as index 0) and non-empty ranks. */
 int afq_ph_info(afq_ph ph, int* nunique, int* nshells);
 /* PHMSD::energy / energy_alg1 (PHMSD.cpp:377-544): d_E [nw][3] c128 (E1, EXX, EJ of the

@@ -247,6 +247,32 @@ def meanfield_factor(X, vMF):
     return mf
 
 
+def one_body_propagator(h, Likn, vMF, dt):
+    """generateP1 (AFQMCBasePropagator.cpp:85-157): (P1, H1) for the mean field vMF."""
+    h, L = _c(h, np.float64), _c(Likn, np.float64)
+    M, K = h.shape[0], L.shape[2]
+    v0 = np.ascontiguousarray(exchange_v0(L), np.float64)
+    P1 = np.zeros((M, M), np.complex128)
+    H1 = np.zeros((M, M), np.complex128)
+    lib().orc_one_body_propagator(M, K, dt, _p(h), _p(v0), _p(L), _p(_c(vMF)), _p(P1), _p(H1))
+    return P1, H1
+
+
+def walker_update(buf, poff, hybrid, dt, Eshift, E0, new_ov, new_E, mf, HW, free_projection=False,
+                  use_cp=False, lower_scale=1.0, upper_scale=10.0, weight_floor=100.0,
+                  weight_fraction=0.1):
+    """walker_update + bound_local_energy (WalkerSetUpdate.hpp:46-191), then the weight bound
+    of the step (target population = W), in place on buf [W][ldw]."""
+    W, ldw = buf.shape
+    stats = np.zeros(3, np.int64)
+    lib().orc_walker_update(W, _p(buf), ldw, poff, int(hybrid), int(free_projection),
+                            int(use_cp), dt, Eshift, E0, _p(_c(new_ov)), _p(_c(new_E)),
+                            _p(_c(mf)), _p(_c(HW)), lower_scale, upper_scale, _p(stats))
+    wst = np.zeros(2, np.int64)
+    lib().orc_bound_weights(W, _p(buf), ldw, poff, weight_floor, weight_fraction, float(W),
+                            _p(wst))
+
+
 def comb(weights, r, num_target=None):
     w = np.ascontiguousarray(weights, np.float64)
     n = len(w) if num_target is None else num_target

@@ -300,3 +300,32 @@ def vmf(struct, A, Lna, dt):
                     G[s, o0] += np.conj(w) * A[s][part]
     G /= ov
     return np.sqrt(dt) * (np.einsum("ak,nak->n", G[0], Lna[0]) + np.einsum("ak,nak->n", G[1], Lna[1]))
+
+
+def ph_step(struct, A, Lna, haj, E0, Likn, vMF, P1, dt, buf, nmo, nup, ndn, xi, Eshift,
+            hybrid, taylor_n=6, vbias_bound=50.0):
+    """AFQMCBasePropagator::Propagate (AFQMCBasePropagator.cpp:289-392) with the PHMSD trial over
+    COLLINEAR walkers: vbias (PHMSD::vbias), assemble_X and the mean-field factor, vHS, both
+    spins' Slater matrices propagated with the same P1 and vHS, the new overlap (hybrid) or the
+    PHMSD local energy and overlap, walker_update and the weight bound. Returns the new buffer."""
+    buf = np.array(buf, np.complex128)
+    W = buf.shape[0]
+    nu = nmo * nup
+    poff = nmo * (nup + ndn)
+    Pu = buf[:, :nu].reshape(W, nmo, nup)
+    Pd = buf[:, nu:poff].reshape(W, nmo, ndn)
+    v = vbias(struct, A, Lna, dt, Pu, Pd)
+    X, HW = O.construct_X(v, xi, vMF, vbias_bound)
+    mf = O.meanfield_factor(X, vMF)
+    V = O.vhs(X, Likn, dt)
+    Pu = O.propagate(Pu, P1, V, taylor_n)
+    Pd = O.propagate(Pd, P1, V, taylor_n)
+    buf[:, :nu] = Pu.reshape(W, -1)
+    buf[:, nu:poff] = Pd.reshape(W, -1)
+    if hybrid:
+        _, new_ov = mixed_density_matrix(struct, A, Pu, Pd)
+        new_E = np.zeros((W, 3), np.complex128)
+    else:
+        new_E, new_ov = energy_alg1(struct, A, Lna, haj, E0, Pu, Pd)
+    O.walker_update(buf, poff, hybrid, dt, Eshift, E0, new_ov, new_E, mf, HW)
+    return buf

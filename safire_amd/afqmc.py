@@ -91,6 +91,12 @@ SIGNATURES = {
     "afq_ph_mixed_density_matrix": (_i, [_vp, _i, _vp, _l, _i, _vp, _vp, _vp]),
     "afq_ph_vbias": (_i, [_vp, _i, _vp, _l, _d, _vp, _vp]),
     "afq_ph_vmf": (_i, [_vp, _d, _vp, _vp]),
+    "afq_ph_prepare": (_i, [_vp, _d, _i, _vp]),
+    "afq_ph_set_params": (_i, [_vp, _vp]),
+    "afq_ph_seed_properties": (_i, [_vp, _i, _vp, _l, _vp]),
+    "afq_ph_step": (_i, [_vp, _i, _vp, _l, _vp, _d, _vp]),
+    "afq_ph_orthogonalize": (_i, [_vp, _i, _vp, _l, _vp]),
+    "afq_ph_population_control": (_i, [_vp, _i, _vp, _l, _d, _vp]),
     "afq_ph_compact_R": (_i, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp]),
     "afq_ph_accumulate_R": (_i, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "afq_split_seed": (C.c_ulonglong, [_i, _i, C.c_uint]),
@@ -311,6 +317,50 @@ class PHMSD:
                                                 int(bool(full)), ptr(G), ptr(ov),
                                                 stream_ptr(stream)))
         return G, ov
+
+    @property
+    def ldw(self):
+        return self.nmo * (self.nup + self.ndn) + P_COUNT
+
+    def prepare(self, dt, W, stream=None):
+        """The propagator for time step dt and the workspace of W walkers."""
+        self.dt = float(dt)
+        check(lib().afq_ph_prepare(self.handle, self.dt, int(W), stream_ptr(stream)))
+        self.params = default_params()
+
+    def set_params(self, **kw):
+        for k, v in kw.items():
+            setattr(self.params, k, v)
+        check(lib().afq_ph_set_params(self.handle, C.byref(self.params)))
+
+    def new_walkers(self, W):
+        """W walkers at the reference determinant of the trial, weight 1 (host, c128)."""
+        up = self.orbmats[0][:self.nup].conj().T
+        dn = self.orbmats[-1][:self.ndn].conj().T
+        buf = np.zeros((W, self.ldw), np.complex128)
+        buf[:, :self.nmo * self.nup] = up.reshape(-1)
+        buf[:, self.nmo * self.nup:self.nmo * (self.nup + self.ndn)] = dn.reshape(-1)
+        off = self.nmo * (self.nup + self.ndn)
+        buf[:, off + P_WEIGHT] = 1.0
+        buf[:, off + P_PHASE] = 1.0
+        buf[:, off + P_IS_UNITARY] = 1.0
+        return buf
+
+    def seed(self, wbuf, stream=None):
+        check(lib().afq_ph_seed_properties(self.handle, wbuf.shape[0], ptr(wbuf), wbuf.shape[1],
+                                           stream_ptr(stream)))
+
+    def step(self, wbuf, xi, Eshift, stream=None):
+        check(lib().afq_ph_step(self.handle, wbuf.shape[0], ptr(wbuf), wbuf.shape[1], ptr(xi),
+                                float(Eshift), stream_ptr(stream)))
+
+    def Orthogonalize(self, wbuf, stream=None):
+        check(lib().afq_ph_orthogonalize(self.handle, wbuf.shape[0], ptr(wbuf), wbuf.shape[1],
+                                         stream_ptr(stream)))
+
+    def population_control(self, wbuf, uniform, stream=None):
+        check(lib().afq_ph_population_control(self.handle, wbuf.shape[0], ptr(wbuf),
+                                              wbuf.shape[1], float(uniform), stream_ptr(stream)))
 
     def vMF(self, dt, stream=None):
         """PHMSD::vMF (PHMSD.cpp:1329-1565): the mean-field potential [nchol]."""

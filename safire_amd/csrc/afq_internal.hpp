@@ -248,6 +248,11 @@ void launch_population_control(hipStream_t s, int W, double* wbuf, long ldw, lon
                                unsigned long long* stats);
 // k_setup.hip
 void setup_hamiltonian(afq_ham_s* H, hipStream_t s);
+void launch_v0(hipStream_t s, int M, int K, const double* Likn, double* v0);
+void launch_zero_imag(hipStream_t s, int n, double* v);
+void build_one_body_propagator(hipStream_t s, int M, int K, double dt, const double* h,
+                               const double* v0, const double* Likn, const double* vMF,
+                               double* H1, double* P1);
 // k_phmsd.hip
 void launch_ph_2body(hipStream_t s, int nw, int ndet, int nex, int nact, int nelec, int nchol,
                      const int* iexcit, const int* refc, const double* Twina, const double* R,

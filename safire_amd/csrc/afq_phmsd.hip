@@ -149,6 +149,18 @@ struct afq_ph_t {
   double E0 = 0.0;
   afq::Spin spin[2];
   double* Lstack = nullptr;        // c128 [nchol][2 nact][nmo]: both spins' Lna per vector
+  double* h = nullptr;             // f64 [nmo][nmo]
+  double* Likn = nullptr;          // f64 [nmo * nmo][nchol]
+  // propagator state (afq_ph_prepare) and the step's per-walker workspace
+  double dt = 0.0;
+  afq_step_params params;
+  double *vMF = nullptr, *v0 = nullptr, *H1 = nullptr, *P1 = nullptr;
+  int wcap = 0;
+  double *X = nullptr, *HW = nullptr, *mf = nullptr, *V = nullptr, *ovw = nullptr, *Ew = nullptr,
+         *cum = nullptr;
+  int *off = nullptr, *src = nullptr, *ok = nullptr;
+  unsigned long long* stats = nullptr;
+  std::vector<void*> step_owned;
   std::vector<void*> owned;
   // workspace for one walker block
   double *G[2] = {nullptr, nullptr}, *QQ0[2] = {nullptr, nullptr}, *work = nullptr;
@@ -161,6 +173,7 @@ struct afq_ph_t {
   long split_elems = 0;
   ~afq_ph_t() {
     for (void* p : owned) (void)hipFree(p);
+    for (void* p : step_owned) (void)hipFree(p);
   }
 };
 
@@ -229,6 +242,222 @@ void ph_density_block(afq_ph_t* ph, hipStream_t s, int nb, const double* const P
                  d_Gc);
   }
   launch_ph_log_overlap(s, nb, ph->Ov, ph->log_ov, d_logov);
+}
+
+// PHMSD::energy_alg1 over all walkers, in walker blocks
+void ph_energy_all(afq_ph_t* ph, hipStream_t s, int nw, const double* d_wbuf, long ldw,
+                   double* d_E, double* d_ov) {
+    const int nmo = ph->nmo, na = ph->nact, K = ph->nchol;
+    Spin* S = ph->spin;
+    const int nu = S[0].nunique, nd = S[1].nunique;
+    for (int w0 = 0; w0 < nw; w0 += ph->wblock) {
+      const int nb = std::min(ph->wblock, nw - w0);
+      const double* Phi[2] = {d_wbuf + 2 * (long)w0 * ldw,
+                              d_wbuf + 2 * ((long)w0 * ldw + (long)nmo * S[0].nel)};
+      launch_ph_fill(s, 3L * nb, 0.0, 0.0, ph->acc);
+      // 1. reference G, log overlap, QQ0 and the excitation overlap ratios of both spins
+      ph_overlaps_block(ph, s, nb, Phi, ldw);
+      // 2a. weights and the multi-determinant overlap
+      launch_ph_spin_weights(s, nb, nu, S[0].cp, S[0].ci, S[0].cv, ph->ratios[0], ph->ratios[1],
+                             ph->wgt[0], ph->Ov);
+      // 3a. reference energy of the up spin (KE row 0 = its Cholesky trace), times Ov
+      launch_ph_reference_energy(s, nb, S[0].nel, nmo, K, na, 1.0, ph->E0, ph->G[0], S[0].Lna,
+                                 S[0].haj, ph->Eref, ph->KEa, ph->Twina, ph->Swia);
+      launch_ph_accumulate(s, nb, ph->Eref, ph->Ov, nullptr, ph->acc);
+      // 4a. excited up determinants: E corrections and their Cholesky vectors KE (rows 1..)
+      launch_ph_fill(s, 3L * nb, 0.0, 0.0, ph->Ex);
+      for (const Shell& sh : S[0].shells)
+        for (int d0 = 0; d0 < sh.ndet; d0 += kPhDetBlock) {
+          const int nbk = std::min(kPhDetBlock, sh.ndet - d0);
+          const int* iex = sh.iexcit + 2L * sh.nex * d0;
+          const double* w = ph->wgt[0] + 2L * (sh.row + d0) * nb;
+          launch_ph_compact_R(s, nb, nbk, sh.nex, na, S[0].nel, iex, S[0].refc, ph->QQ0[0],
+                              ph->R);
+          launch_ph_1body(s, nb, nbk, sh.nex, na, S[0].nel, iex, S[0].refc, ph->Swia, ph->R, w,
+                          ph->Ex);
+          launch_ph_2body(s, nb, nbk, sh.nex, na, S[0].nel, K, iex, S[0].refc, ph->Twina, ph->R,
+                          w, ph->Ex + 2L * nb, ph->Ex + 4L * nb,
+                          ph->KEa + 2L * (sh.row + d0) * nb * K);
+        }
+      launch_ph_accumulate(s, nb, nullptr, nullptr, ph->Ex, ph->acc);
+      // KEb[d] = ratio_dn[d] sum_u C[d][u] ratio_up[u] KEa[u]
+      launch_ph_couple_ke(s, nb, K, nd, S[1].cp, S[1].ci, S[1].cv, ph->ratios[0], ph->KEa,
+                          ph->ratios[1], ph->KEb);
+      // 2b. weights of the down determinants
+      launch_ph_spin_weights(s, nb, nd, S[1].cp, S[1].ci, S[1].cv, ph->ratios[1], ph->ratios[0],
+                             ph->wgt[1], ph->Ov2);
+      // 3b. reference energy of the down spin (E0 only with the up spin), the cross term
+      launch_ph_reference_energy(s, nb, S[1].nel, nmo, K, na, 1.0, 0.0, ph->G[1], S[1].Lna,
+                                 S[1].haj, ph->Eref, ph->KEa, ph->Twina, ph->Swia);
+      launch_ph_accumulate(s, nb, ph->Eref, ph->Ov, nullptr, ph->acc);
+      launch_ph_ke_dot(s, nb, K, 1, ph->KEb, ph->KEa, ph->acc + 4, 3);
+      // 4b. excited down determinants, contracted with the coupled up vectors
+      launch_ph_fill(s, 3L * nb, 0.0, 0.0, ph->Ex);
+      for (const Shell& sh : S[1].shells)
+        for (int d0 = 0; d0 < sh.ndet; d0 += kPhDetBlock) {
+          const int nbk = std::min(kPhDetBlock, sh.ndet - d0);
+          const int* iex = sh.iexcit + 2L * sh.nex * d0;
+          const double* w = ph->wgt[1] + 2L * (sh.row + d0) * nb;
+          launch_ph_compact_R(s, nb, nbk, sh.nex, na, S[1].nel, iex, S[1].refc, ph->QQ0[1],
+                              ph->R);
+          launch_ph_1body(s, nb, nbk, sh.nex, na, S[1].nel, iex, S[1].refc, ph->Swia, ph->R, w,
+                          ph->Ex);
+          launch_ph_2body(s, nb, nbk, sh.nex, na, S[1].nel, K, iex, S[1].refc, ph->Twina, ph->R,
+                          w, ph->Ex + 2L * nb, ph->Ex + 4L * nb, ph->KEl);
+          launch_ph_ke_dot(s, nb, K, nbk, ph->KEb + 2L * (sh.row + d0) * nb * K, ph->KEl,
+                           ph->Ex + 4L * nb, 1);
+        }
+      launch_ph_accumulate(s, nb, nullptr, nullptr, ph->Ex, ph->acc);
+      // E /= Ov, Ov -> log Ov + log of the reference overlaps
+      launch_ph_finalize(s, nb, ph->acc, ph->Ov, ph->log_ov, d_E + 6L * w0, d_ov + 2L * w0);
+    }
+}
+
+// PHMSD::vbias over all walkers, in walker blocks
+void ph_vbias_all(afq_ph_t* ph, hipStream_t s, int nw, const double* d_wbuf, long ldw, double dt,
+                  double* d_v) {
+    const int nmo = ph->nmo, na = ph->nact, K = ph->nchol;
+    const int kx = 2 * na * nmo;
+    for (int w0 = 0; w0 < nw; w0 += ph->wblock) {
+      const int nb = std::min(ph->wblock, nw - w0);
+      const double* Phi[2] = {d_wbuf + 2 * (long)w0 * ldw,
+                              d_wbuf + 2 * ((long)w0 * ldw + (long)nmo * ph->spin[0].nel)};
+      ph_density_block(ph, s, nb, Phi, ldw, ph->Gblk, ph->Eref);   // log overlap unused
+      launch_gemm_nt_z(s, AFQ_FP64, nb, K, kx, std::sqrt(dt), ph->Gblk, kx, ph->Lstack, kx, true,
+                       d_v + 2L * w0 * K, K, ph->split, ph->split_elems);
+    }
+}
+
+// PHMSD::vMF (host density, device vbias); synchronises the stream
+void ph_vmf(afq_ph_t* ph, hipStream_t st, double dt, double* d_vmf) {
+    const int nmo = ph->nmo, na = ph->nact;
+    // 1. overlap of every unique configuration relative to the reference: log det(As^H As),
+    //    As the configuration's orbitals (columns of the conjugate-transposed orbital rows)
+    std::vector<zc> ratio[2];
+    for (int s = 0; s < 2; ++s) {
+      const Spin& S = ph->spin[s];
+      const int ne = S.nel;
+      std::vector<zc> lds;
+      for (const auto& c : S.confs) {
+        std::vector<zc> T((size_t)ne * ne, 0.0);
+        for (int i = 0; i < ne; ++i)
+          for (int j = 0; j < ne; ++j) {
+            zc acc = 0.0;
+            for (int k = 0; k < nmo; ++k)
+              acc += S.hA[(size_t)c[i] * nmo + k] * std::conj(S.hA[(size_t)c[j] * nmo + k]);
+            T[(size_t)i * ne + j] = acc;
+          }
+        lds.push_back(host_logdet_inverse(T, ne, nullptr));
+      }
+      for (size_t d = 0; d < lds.size(); ++d) ratio[s].push_back(d ? std::exp(lds[d] - lds[0]) : 1.0);
+    }
+    zc ov = 0.0;
+    {
+      const Spin& S = ph->spin[0];
+      for (int u = 0; u + 1 < (int)S.hcp.size(); ++u)
+        for (int j = S.hcp[u]; j < S.hcp[u + 1]; ++j)
+          ov += std::norm(S.hcv[j]) * ratio[0][u] * ratio[1][S.hci[j]];
+    }
+    // 2. diagonal and off-diagonal terms of each spin's density
+    std::vector<zc> G((size_t)2 * na * nmo, 0.0);
+    for (int s = 0; s < 2; ++s) {
+      const int o = 1 - s;
+      const Spin& S = ph->spin[s];
+      const Spin& O = ph->spin[o];
+      const int ne = S.nel;
+      zc* Gs = G.data() + (size_t)s * na * nmo;
+      for (size_t nd = 0; nd < S.confs.size(); ++nd) {
+        const auto& c = S.confs[nd];
+        // compact G of the configuration with itself: (As^H As)^-T As^T
+        std::vector<zc> T((size_t)ne * ne), Ti;
+        for (int i = 0; i < ne; ++i)
+          for (int j = 0; j < ne; ++j) {
+            zc acc = 0.0;
+            for (int k = 0; k < nmo; ++k)
+              acc += S.hA[(size_t)c[i] * nmo + k] * std::conj(S.hA[(size_t)c[j] * nmo + k]);
+            T[(size_t)i * ne + j] = acc;
+          }
+        host_logdet_inverse(T, ne, &Ti);
+        zc w = 0.0;
+        for (int j = S.hcp[nd]; j < S.hcp[nd + 1]; ++j)
+          w += ratio[o][S.hci[j]] * std::norm(S.hcv[j]);
+        w *= ratio[s][nd];
+        for (int i = 0; i < ne; ++i)
+          for (int k = 0; k < nmo; ++k) {
+            zc g = 0.0;
+            for (int b = 0; b < ne; ++b)
+              g += Ti[(size_t)b * ne + i] * std::conj(S.hA[(size_t)c[b] * nmo + k]);
+            Gs[(size_t)c[i] * nmo + k] += w * g;
+          }
+      }
+      // pairs of this spin's configurations in one row of the other spin's couplings that
+      // differ by one orbital (no fermionic sign, the orbital rows as the reference adds them)
+      for (int nd = 0; nd + 1 < (int)O.hcp.size(); ++nd)
+        for (int n1 = O.hcp[nd]; n1 < O.hcp[nd + 1]; ++n1) {
+          std::vector<int> c1 = S.confs[O.hci[n1]];
+          std::sort(c1.begin(), c1.end());
+          for (int n2 = n1 + 1; n2 < O.hcp[nd + 1]; ++n2) {
+            std::vector<int> c2 = S.confs[O.hci[n2]];
+            std::sort(c2.begin(), c2.end());
+            int nh = 0, hole = -1, part = -1;
+            for (int x : c1)
+              if (!std::binary_search(c2.begin(), c2.end(), x)) {
+                ++nh;
+                hole = x;
+              }
+            if (nh != 1) continue;
+            for (int x : c2)
+              if (!std::binary_search(c1.begin(), c1.end(), x)) part = x;
+            const zc w = O.hcv[n1] * std::conj(O.hcv[n2]);
+            for (int k = 0; k < nmo; ++k) {
+              Gs[(size_t)part * nmo + k] += w * S.hA[(size_t)hole * nmo + k];
+              Gs[(size_t)hole * nmo + k] += std::conj(w) * S.hA[(size_t)part * nmo + k];
+            }
+          }
+        }
+    }
+    for (auto& g : G) g /= ov;
+    // 3. vbias of the mean-field density: sqrt(dt) sum_s sum_{a,k} Lna_s[n][a][k] G_s[a][k]
+    std::vector<double> flat(2 * G.size());
+    for (size_t e = 0; e < G.size(); ++e) {
+      flat[2 * e] = G[e].real();
+      flat[2 * e + 1] = G[e].imag();
+    }
+    AFQ_HIP(hipMemcpyAsync(ph->Gblk, flat.data(), flat.size() * sizeof(double),
+                           hipMemcpyHostToDevice, st));
+    const int kx = 2 * na * nmo;
+    launch_gemm_nt_z(st, AFQ_FP64, 1, ph->nchol, kx, std::sqrt(dt), ph->Gblk, kx, ph->Lstack, kx,
+                     true, d_vmf, ph->nchol);
+    AFQ_HIP(hipStreamSynchronize(st));
+}
+
+// PHMSD overlap (the log overlap of MixedDensityMatrix without G) over all walkers
+void ph_overlap_all(afq_ph_t* ph, hipStream_t s, int nw, const double* d_wbuf, long ldw,
+                    double* d_ov) {
+  const int nmo = ph->nmo;
+  Spin* S = ph->spin;
+  for (int w0 = 0; w0 < nw; w0 += ph->wblock) {
+    const int nb = std::min(ph->wblock, nw - w0);
+    const double* Phi[2] = {d_wbuf + 2 * (long)w0 * ldw,
+                            d_wbuf + 2 * ((long)w0 * ldw + (long)nmo * S[0].nel)};
+    ph_overlaps_block(ph, s, nb, Phi, ldw);
+    launch_ph_spin_weights(s, nb, S[0].nunique, S[0].cp, S[0].ci, S[0].cv, ph->ratios[0],
+                           ph->ratios[1], ph->wgt[0], ph->Ov);
+    launch_ph_log_overlap(s, nb, ph->Ov, ph->log_ov, d_ov + 2L * w0);
+  }
+}
+
+long ph_poff(const afq_ph_t* ph) { return (long)ph->nmo * (ph->spin[0].nel + ph->spin[1].nel); }
+
+void ph_check_walkers(const afq_ph_t* ph, int W, long ldw, bool prepared) {
+  AFQ_CHECK(ph != nullptr, AFQ_EINVAL, "PHMSD: null handle");
+  AFQ_CHECK(W >= 0, AFQ_EBADSHAPE, "PHMSD: negative walker count");
+  AFQ_CHECK(ldw >= ph_poff(ph) + AFQ_P_COUNT, AFQ_EBADSHAPE,
+            "PHMSD: ldw smaller than the Slater matrices and the walker properties");
+  if (prepared) {
+    AFQ_CHECK(ph->dt > 0.0, AFQ_EINVAL, "PHMSD: afq_ph_prepare has not been called");
+    AFQ_CHECK(W <= ph->wcap, AFQ_ECAPACITY, "PHMSD: more walkers than afq_ph_prepare sized for");
+  }
 }
 
 }  // namespace
@@ -383,6 +612,8 @@ int afq_ph_create(int nmo, int nup, int ndn, int nchol, int ndet, const int* occ
     const size_t nmm = (size_t)nmo * nmo;
     double* dh = dev_copy(own, h1, nmm);
     double* dL = dev_copy(own, likn, nmm * nchol);
+    P->h = dh;
+    P->Likn = dL;
     for (int s = 0; s < 2; ++s) {
       Spin& S = P->spin[s];
       S.A = dev_copy(own, orbmats + (nref == 2 ? s : 0) * 2 * nmm, 2 * nmm);
@@ -459,71 +690,7 @@ int afq_ph_energy(afq_ph ph, int nw, const double* d_wbuf, long ldw, double* d_E
     AFQ_CHECK(ldw >= need, AFQ_EBADSHAPE, "afq_ph_energy: ldw smaller than the Slater matrices");
     if (nw == 0) return;
     AFQ_CHECK(d_wbuf && d_E && d_ov, AFQ_EINVAL, "afq_ph_energy: null operand");
-    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    const int nmo = ph->nmo, na = ph->nact, K = ph->nchol;
-    Spin* S = ph->spin;
-    const int nu = S[0].nunique, nd = S[1].nunique;
-    for (int w0 = 0; w0 < nw; w0 += ph->wblock) {
-      const int nb = std::min(ph->wblock, nw - w0);
-      const double* Phi[2] = {d_wbuf + 2 * (long)w0 * ldw,
-                              d_wbuf + 2 * ((long)w0 * ldw + (long)nmo * S[0].nel)};
-      launch_ph_fill(s, 3L * nb, 0.0, 0.0, ph->acc);
-      // 1. reference G, log overlap, QQ0 and the excitation overlap ratios of both spins
-      ph_overlaps_block(ph, s, nb, Phi, ldw);
-      // 2a. weights and the multi-determinant overlap
-      launch_ph_spin_weights(s, nb, nu, S[0].cp, S[0].ci, S[0].cv, ph->ratios[0], ph->ratios[1],
-                             ph->wgt[0], ph->Ov);
-      // 3a. reference energy of the up spin (KE row 0 = its Cholesky trace), times Ov
-      launch_ph_reference_energy(s, nb, S[0].nel, nmo, K, na, 1.0, ph->E0, ph->G[0], S[0].Lna,
-                                 S[0].haj, ph->Eref, ph->KEa, ph->Twina, ph->Swia);
-      launch_ph_accumulate(s, nb, ph->Eref, ph->Ov, nullptr, ph->acc);
-      // 4a. excited up determinants: E corrections and their Cholesky vectors KE (rows 1..)
-      launch_ph_fill(s, 3L * nb, 0.0, 0.0, ph->Ex);
-      for (const Shell& sh : S[0].shells)
-        for (int d0 = 0; d0 < sh.ndet; d0 += kPhDetBlock) {
-          const int nbk = std::min(kPhDetBlock, sh.ndet - d0);
-          const int* iex = sh.iexcit + 2L * sh.nex * d0;
-          const double* w = ph->wgt[0] + 2L * (sh.row + d0) * nb;
-          launch_ph_compact_R(s, nb, nbk, sh.nex, na, S[0].nel, iex, S[0].refc, ph->QQ0[0],
-                              ph->R);
-          launch_ph_1body(s, nb, nbk, sh.nex, na, S[0].nel, iex, S[0].refc, ph->Swia, ph->R, w,
-                          ph->Ex);
-          launch_ph_2body(s, nb, nbk, sh.nex, na, S[0].nel, K, iex, S[0].refc, ph->Twina, ph->R,
-                          w, ph->Ex + 2L * nb, ph->Ex + 4L * nb,
-                          ph->KEa + 2L * (sh.row + d0) * nb * K);
-        }
-      launch_ph_accumulate(s, nb, nullptr, nullptr, ph->Ex, ph->acc);
-      // KEb[d] = ratio_dn[d] sum_u C[d][u] ratio_up[u] KEa[u]
-      launch_ph_couple_ke(s, nb, K, nd, S[1].cp, S[1].ci, S[1].cv, ph->ratios[0], ph->KEa,
-                          ph->ratios[1], ph->KEb);
-      // 2b. weights of the down determinants
-      launch_ph_spin_weights(s, nb, nd, S[1].cp, S[1].ci, S[1].cv, ph->ratios[1], ph->ratios[0],
-                             ph->wgt[1], ph->Ov2);
-      // 3b. reference energy of the down spin (E0 only with the up spin), the cross term
-      launch_ph_reference_energy(s, nb, S[1].nel, nmo, K, na, 1.0, 0.0, ph->G[1], S[1].Lna,
-                                 S[1].haj, ph->Eref, ph->KEa, ph->Twina, ph->Swia);
-      launch_ph_accumulate(s, nb, ph->Eref, ph->Ov, nullptr, ph->acc);
-      launch_ph_ke_dot(s, nb, K, 1, ph->KEb, ph->KEa, ph->acc + 4, 3);
-      // 4b. excited down determinants, contracted with the coupled up vectors
-      launch_ph_fill(s, 3L * nb, 0.0, 0.0, ph->Ex);
-      for (const Shell& sh : S[1].shells)
-        for (int d0 = 0; d0 < sh.ndet; d0 += kPhDetBlock) {
-          const int nbk = std::min(kPhDetBlock, sh.ndet - d0);
-          const int* iex = sh.iexcit + 2L * sh.nex * d0;
-          const double* w = ph->wgt[1] + 2L * (sh.row + d0) * nb;
-          launch_ph_compact_R(s, nb, nbk, sh.nex, na, S[1].nel, iex, S[1].refc, ph->QQ0[1],
-                              ph->R);
-          launch_ph_1body(s, nb, nbk, sh.nex, na, S[1].nel, iex, S[1].refc, ph->Swia, ph->R, w,
-                          ph->Ex);
-          launch_ph_2body(s, nb, nbk, sh.nex, na, S[1].nel, K, iex, S[1].refc, ph->Twina, ph->R,
-                          w, ph->Ex + 2L * nb, ph->Ex + 4L * nb, ph->KEl);
-          launch_ph_ke_dot(s, nb, K, nbk, ph->KEb + 2L * (sh.row + d0) * nb * K, ph->KEl,
-                           ph->Ex + 4L * nb, 1);
-        }
-      launch_ph_accumulate(s, nb, nullptr, nullptr, ph->Ex, ph->acc);
-      // E /= Ov, Ov -> log Ov + log of the reference overlaps
-      launch_ph_finalize(s, nb, ph->acc, ph->Ov, ph->log_ov, d_E + 6L * w0, d_ov + 2L * w0);
-    }
+    ph_energy_all(ph, reinterpret_cast<hipStream_t>(stream), nw, d_wbuf, ldw, d_E, d_ov);
   });
 }
 
@@ -564,16 +731,7 @@ int afq_ph_vbias(afq_ph ph, int nw, const double* d_wbuf, long ldw, double dt, d
     if (nw == 0) return;
     AFQ_CHECK(d_wbuf && d_v, AFQ_EINVAL, "afq_ph_vbias: null operand");
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    const int nmo = ph->nmo, na = ph->nact, K = ph->nchol;
-    const int kx = 2 * na * nmo;
-    for (int w0 = 0; w0 < nw; w0 += ph->wblock) {
-      const int nb = std::min(ph->wblock, nw - w0);
-      const double* Phi[2] = {d_wbuf + 2 * (long)w0 * ldw,
-                              d_wbuf + 2 * ((long)w0 * ldw + (long)nmo * ph->spin[0].nel)};
-      ph_density_block(ph, s, nb, Phi, ldw, ph->Gblk, ph->Eref);   // log overlap unused
-      launch_gemm_nt_z(s, AFQ_FP64, nb, K, kx, std::sqrt(dt), ph->Gblk, kx, ph->Lstack, kx, true,
-                       d_v + 2L * w0 * K, K, ph->split, ph->split_elems);
-    }
+    ph_vbias_all(ph, s, nw, d_wbuf, ldw, dt, d_v);
   });
 }
 
@@ -581,106 +739,120 @@ int afq_ph_vmf(afq_ph ph, double dt, double* d_vmf, void* stream) {
   return ph_guard([&] {
     AFQ_CHECK(ph != nullptr, AFQ_EINVAL, "afq_ph_vmf: null handle");
     AFQ_CHECK(d_vmf != nullptr && dt > 0.0, AFQ_EINVAL, "afq_ph_vmf: null output or dt <= 0");
-    const int nmo = ph->nmo, na = ph->nact;
-    // 1. overlap of every unique configuration relative to the reference: log det(As^H As),
-    //    As the configuration's orbitals (columns of the conjugate-transposed orbital rows)
-    std::vector<zc> ratio[2];
-    for (int s = 0; s < 2; ++s) {
-      const Spin& S = ph->spin[s];
-      const int ne = S.nel;
-      std::vector<zc> lds;
-      for (const auto& c : S.confs) {
-        std::vector<zc> T((size_t)ne * ne, 0.0);
-        for (int i = 0; i < ne; ++i)
-          for (int j = 0; j < ne; ++j) {
-            zc acc = 0.0;
-            for (int k = 0; k < nmo; ++k)
-              acc += S.hA[(size_t)c[i] * nmo + k] * std::conj(S.hA[(size_t)c[j] * nmo + k]);
-            T[(size_t)i * ne + j] = acc;
-          }
-        lds.push_back(host_logdet_inverse(T, ne, nullptr));
-      }
-      for (size_t d = 0; d < lds.size(); ++d) ratio[s].push_back(d ? std::exp(lds[d] - lds[0]) : 1.0);
+    ph_vmf(ph, reinterpret_cast<hipStream_t>(stream), dt, d_vmf);
+  });
+}
+
+int afq_ph_prepare(afq_ph ph, double dt, int W, void* stream) {
+  return ph_guard([&] {
+    AFQ_CHECK(ph != nullptr, AFQ_EINVAL, "afq_ph_prepare: null handle");
+    AFQ_CHECK(dt > 0.0 && W >= 0, AFQ_EINVAL, "afq_ph_prepare: dt <= 0 or negative W");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const int M = ph->nmo, K = ph->nchol;
+    auto& own = ph->step_owned;
+    if (ph->vMF == nullptr) {
+      ph->vMF = dev_alloc<double>(own, 2 * (size_t)K);
+      ph->v0 = dev_alloc<double>(own, (size_t)M * M);
+      ph->H1 = dev_alloc<double>(own, 2 * (size_t)M * M);
+      ph->P1 = dev_alloc<double>(own, 2 * (size_t)M * M);
+      ph->stats = dev_alloc<unsigned long long>(own, 7);
+      AFQ_HIP(hipMemset(ph->stats, 0, 7 * sizeof(unsigned long long)));
+      afq_default_params(&ph->params);
     }
-    zc ov = 0.0;
-    {
-      const Spin& S = ph->spin[0];
-      for (int u = 0; u + 1 < (int)S.hcp.size(); ++u)
-        for (int j = S.hcp[u]; j < S.hcp[u + 1]; ++j)
-          ov += std::norm(S.hcv[j]) * ratio[0][u] * ratio[1][S.hci[j]];
+    // the mean field of the trial with its imaginary part dropped (AFQMCBasePropagator.cpp:
+    // 85-107), v0, and the one-body propagator built from them
+    ph_vmf(ph, s, dt, ph->vMF);
+    launch_zero_imag(s, K, ph->vMF);
+    launch_v0(s, M, K, ph->Likn, ph->v0);
+    build_one_body_propagator(s, M, K, dt, ph->h, ph->v0, ph->Likn, ph->vMF, ph->H1, ph->P1);
+    ph->dt = dt;
+    if (W > ph->wcap) {
+      const size_t w = (size_t)W;
+      ph->X = dev_alloc<double>(own, 2 * w * K);
+      ph->HW = dev_alloc<double>(own, 2 * w);
+      ph->mf = dev_alloc<double>(own, 2 * w);
+      ph->V = dev_alloc<double>(own, 2 * w * M * M);
+      ph->ovw = dev_alloc<double>(own, 2 * w);
+      ph->Ew = dev_alloc<double>(own, 6 * w);
+      ph->cum = dev_alloc<double>(own, w);
+      ph->off = dev_alloc<int>(own, w);
+      ph->src = dev_alloc<int>(own, w);
+      ph->ok = dev_alloc<int>(own, 1);
+      ph->wcap = W;
     }
-    // 2. diagonal and off-diagonal terms of each spin's density
-    std::vector<zc> G((size_t)2 * na * nmo, 0.0);
-    for (int s = 0; s < 2; ++s) {
-      const int o = 1 - s;
-      const Spin& S = ph->spin[s];
-      const Spin& O = ph->spin[o];
-      const int ne = S.nel;
-      zc* Gs = G.data() + (size_t)s * na * nmo;
-      for (size_t nd = 0; nd < S.confs.size(); ++nd) {
-        const auto& c = S.confs[nd];
-        // compact G of the configuration with itself: (As^H As)^-T As^T
-        std::vector<zc> T((size_t)ne * ne), Ti;
-        for (int i = 0; i < ne; ++i)
-          for (int j = 0; j < ne; ++j) {
-            zc acc = 0.0;
-            for (int k = 0; k < nmo; ++k)
-              acc += S.hA[(size_t)c[i] * nmo + k] * std::conj(S.hA[(size_t)c[j] * nmo + k]);
-            T[(size_t)i * ne + j] = acc;
-          }
-        host_logdet_inverse(T, ne, &Ti);
-        zc w = 0.0;
-        for (int j = S.hcp[nd]; j < S.hcp[nd + 1]; ++j)
-          w += ratio[o][S.hci[j]] * std::norm(S.hcv[j]);
-        w *= ratio[s][nd];
-        for (int i = 0; i < ne; ++i)
-          for (int k = 0; k < nmo; ++k) {
-            zc g = 0.0;
-            for (int b = 0; b < ne; ++b)
-              g += Ti[(size_t)b * ne + i] * std::conj(S.hA[(size_t)c[b] * nmo + k]);
-            Gs[(size_t)c[i] * nmo + k] += w * g;
-          }
-      }
-      // pairs of this spin's configurations in one row of the other spin's couplings that
-      // differ by one orbital (no fermionic sign, the orbital rows as the reference adds them)
-      for (int nd = 0; nd + 1 < (int)O.hcp.size(); ++nd)
-        for (int n1 = O.hcp[nd]; n1 < O.hcp[nd + 1]; ++n1) {
-          std::vector<int> c1 = S.confs[O.hci[n1]];
-          std::sort(c1.begin(), c1.end());
-          for (int n2 = n1 + 1; n2 < O.hcp[nd + 1]; ++n2) {
-            std::vector<int> c2 = S.confs[O.hci[n2]];
-            std::sort(c2.begin(), c2.end());
-            int nh = 0, hole = -1, part = -1;
-            for (int x : c1)
-              if (!std::binary_search(c2.begin(), c2.end(), x)) {
-                ++nh;
-                hole = x;
-              }
-            if (nh != 1) continue;
-            for (int x : c2)
-              if (!std::binary_search(c1.begin(), c1.end(), x)) part = x;
-            const zc w = O.hcv[n1] * std::conj(O.hcv[n2]);
-            for (int k = 0; k < nmo; ++k) {
-              Gs[(size_t)part * nmo + k] += w * S.hA[(size_t)hole * nmo + k];
-              Gs[(size_t)hole * nmo + k] += std::conj(w) * S.hA[(size_t)part * nmo + k];
-            }
-          }
-        }
-    }
-    for (auto& g : G) g /= ov;
-    // 3. vbias of the mean-field density: sqrt(dt) sum_s sum_{a,k} Lna_s[n][a][k] G_s[a][k]
-    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    std::vector<double> flat(2 * G.size());
-    for (size_t e = 0; e < G.size(); ++e) {
-      flat[2 * e] = G[e].real();
-      flat[2 * e + 1] = G[e].imag();
-    }
-    AFQ_HIP(hipMemcpyAsync(ph->Gblk, flat.data(), flat.size() * sizeof(double),
-                           hipMemcpyHostToDevice, st));
-    const int kx = 2 * na * nmo;
-    launch_gemm_nt_z(st, AFQ_FP64, 1, ph->nchol, kx, std::sqrt(dt), ph->Gblk, kx, ph->Lstack, kx,
-                     true, d_vmf, ph->nchol);
-    AFQ_HIP(hipStreamSynchronize(st));
+  });
+}
+
+int afq_ph_set_params(afq_ph ph, const afq_step_params* p) {
+  return ph_guard([&] {
+    AFQ_CHECK(ph != nullptr && p != nullptr, AFQ_EINVAL, "afq_ph_set_params: null argument");
+    AFQ_CHECK(ph->dt > 0.0, AFQ_EINVAL, "afq_ph_set_params: call afq_ph_prepare first");
+    ph->params = *p;
+  });
+}
+
+int afq_ph_seed_properties(afq_ph ph, int W, double* d_wbuf, long ldw, void* stream) {
+  return ph_guard([&] {
+    ph_check_walkers(ph, W, ldw, true);
+    if (W == 0) return;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    ph_energy_all(ph, s, W, d_wbuf, ldw, ph->Ew, ph->ovw);
+    launch_set_properties(s, W, d_wbuf, ldw, ph_poff(ph), ph->ovw, ph->Ew, 1);
+  });
+}
+
+int afq_ph_step(afq_ph ph, int W, double* d_wbuf, long ldw, const double* d_xi, double Eshift,
+                void* stream) {
+  return ph_guard([&] {
+    ph_check_walkers(ph, W, ldw, true);
+    if (W == 0) return;
+    AFQ_CHECK(d_wbuf && d_xi, AFQ_EINVAL, "afq_ph_step: null operand");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const int M = ph->nmo, K = ph->nchol;
+    const afq_step_params& p = ph->params;
+    // 1. vbias of the multi-determinant G (PHMSD::vbias), 2. X, HW, mean-field factor
+    ph_vbias_all(ph, s, W, d_wbuf, ldw, ph->dt, ph->X);
+    launch_construct_X(s, W, K, p.vbias_bound, p.use_cp_constraint || p.project_force_bias,
+                       p.free_projection, ph->vMF, d_xi, ph->X, ph->HW, ph->mf);
+    // 3. vHS = sqrt(dt) L X (Real3IndexFactorization::vHS), 4. both spins' Slater matrices
+    launch_gemm_nt_z(s, AFQ_FP64, W, M * M, K, std::sqrt(ph->dt), ph->X, K, ph->Likn, K, false,
+                     ph->V, (long)M * M);
+    launch_propagate(s, W, M, ph->spin[0].nel, d_wbuf, ldw, ph->P1, ph->V, p.taylor_n);
+    launch_propagate(s, W, M, ph->spin[1].nel, d_wbuf + 2L * M * ph->spin[0].nel, ldw, ph->P1,
+                     ph->V, p.taylor_n);
+    // 5. new overlap (hybrid) or local energy and overlap (PHMSD::energy)
+    if (p.hybrid)
+      ph_overlap_all(ph, s, W, d_wbuf, ldw, ph->ovw);
+    else
+      ph_energy_all(ph, s, W, d_wbuf, ldw, ph->Ew, ph->ovw);
+    // 6.-7. weights, phases, energies, bounds
+    launch_walker_update(s, W, d_wbuf, ldw, ph_poff(ph), p, ph->dt, Eshift, ph->E0, ph->ovw,
+                         ph->Ew, ph->mf, ph->HW, p.hybrid, ph->stats);
+  });
+}
+
+int afq_ph_orthogonalize(afq_ph ph, int W, double* d_wbuf, long ldw, void* stream) {
+  return ph_guard([&] {
+    ph_check_walkers(ph, W, ldw, true);
+    if (W == 0) return;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const int M = ph->nmo;
+    launch_orthogonalize(s, W, M, ph->spin[0].nel, d_wbuf, ldw, nullptr);
+    launch_orthogonalize(s, W, M, ph->spin[1].nel, d_wbuf + 2L * M * ph->spin[0].nel, ldw,
+                         nullptr);
+    ph_overlap_all(ph, s, W, d_wbuf, ldw, ph->ovw);
+    launch_set_properties(s, W, d_wbuf, ldw, ph_poff(ph), ph->ovw, nullptr, 0);
+  });
+}
+
+int afq_ph_population_control(afq_ph ph, int W, double* d_wbuf, long ldw, double uniform,
+                              void* stream) {
+  return ph_guard([&] {
+    ph_check_walkers(ph, W, ldw, true);
+    AFQ_CHECK(uniform >= 0.0 && uniform < 1.0, AFQ_EINVAL, "uniform must be in [0, 1)");
+    if (W == 0) return;
+    launch_population_control(reinterpret_cast<hipStream_t>(stream), W, d_wbuf, ldw,
+                              ph_poff(ph), uniform, ph->cum, ph->off, ph->src, ph->ok, ph->stats);
   });
 }
 

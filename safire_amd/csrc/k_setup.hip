@@ -117,37 +117,37 @@ void launch_half_rotate(hipStream_t s, int M, int N, int K, const double* PsiT, 
   AFQ_HIP(hipGetLastError());
 }
 
-void setup_hamiltonian(afq_ham_s* H, hipStream_t s) {
-  const int M = H->M, N = H->N, K = H->K;
-  const long nL = (long)K * N * M;
-  hipLaunchKernelGGL(k_half_rotate, blocks(nL + (long)N * M), dim3(256), 0, s, M, N, K,
-                     (const cd*)H->PsiT, H->h, H->Likn, (cd*)H->haj, (cd*)H->Lnak);
-  hipLaunchKernelGGL(k_v0, blocks((long)M * M), dim3(256), 0, s, M, K, H->Likn, H->v0);
+// v0[i][l] = -1/2 sum_{j,n} L[i][j][n] L[l][j][n]  (RealDenseHamiltonian.cpp:163-180)
+void launch_v0(hipStream_t s, int M, int K, const double* Likn, double* v0) {
+  hipLaunchKernelGGL(k_v0, blocks((long)M * M), dim3(256), 0, s, M, K, Likn, v0);
   AFQ_HIP(hipGetLastError());
+}
 
-  // scratch: Phi0 [M*N], Gt [N*M], ms [M*M], A/P/Q [M*M] (c128)
+// vMF <- Re vMF (zero_imag, AFQMCBasePropagator.cpp:107)
+void launch_zero_imag(hipStream_t s, int n, double* v) {
+  hipLaunchKernelGGL(k_zero_imag, blocks(n), dim3(256), 0, s, n, (cd*)v);
+  AFQ_HIP(hipGetLastError());
+}
+
+// H1 = dt (h + v0) + vHS(vMF), hermitized, and P1 = exp(-H1/2) (getOneBodyPropagatorMatrix,
+// Real3IndexFactorization.hpp:99-124; generateP1, AFQMCBasePropagator.cpp:141-157) by scaling
+// and squaring of a degree-18 Taylor polynomial. Synchronises the stream (the scaling is chosen
+// on the host from ||A||_1).
+void build_one_body_propagator(hipStream_t s, int M, int K, double dt, const double* h,
+                               const double* v0, const double* Likn, const double* vMF,
+                               double* H1, double* P1) {
+  // scratch: ms, A, P, Q [M*M] (c128)
   double* tmp = nullptr;
-  const size_t ntmp = 2 * ((size_t)M * N * 2 + (size_t)M * M * 4);
-  AFQ_HIP(hipMalloc(&tmp, ntmp * sizeof(double)));
-  double* Phi0 = tmp;
-  double* Gt = Phi0 + 2 * (size_t)M * N;
-  double* ms = Gt + 2 * (size_t)M * N;
+  AFQ_HIP(hipMalloc(&tmp, 2 * (size_t)M * M * 4 * sizeof(double)));
+  double* ms = tmp;
   double* A = ms + 2 * (size_t)M * M;
   double* P = A + 2 * (size_t)M * M;
   double* Q = P + 2 * (size_t)M * M;
   try {
-    hipLaunchKernelGGL(k_trial_phi, blocks((long)M * N), dim3(256), 0, s, M, N,
-                       (const cd*)H->PsiT, (cd*)Phi0);
-    launch_green(s, 1, M, N, H->PsiT, Phi0, (long)M * N, Gt, nullptr, 0, 1.0, cmk(0, 0),
-                 nullptr, 0.0, 0.0, nullptr, 0);
-    const double vscale = std::sqrt(H->dt) * (H->walker_type == AFQ_CLOSED ? 2.0 : 1.0);
-    launch_gemm_nt_z(s, AFQ_FP64, 1, K, N * M, vscale, Gt, (long)N * M, H->Lnak, (long)N * M,
-                     true, H->vMF, K);
-    hipLaunchKernelGGL(k_zero_imag, blocks(K), dim3(256), 0, s, K, (cd*)H->vMF);
-    launch_gemm_nt_z(s, AFQ_FP64, 1, M * M, K, std::sqrt(H->dt), H->vMF, K, H->Likn, K, false,
-                     ms, (long)M * M);
-    hipLaunchKernelGGL(k_build_H1, blocks((long)M * M), dim3(256), 0, s, M, H->dt, H->h,
-                       H->v0, (const cd*)ms, (cd*)H->H1, (cd*)A);
+    launch_gemm_nt_z(s, AFQ_FP64, 1, M * M, K, std::sqrt(dt), vMF, K, Likn, K, false, ms,
+                     (long)M * M);
+    hipLaunchKernelGGL(k_build_H1, blocks((long)M * M), dim3(256), 0, s, M, dt, h, v0,
+                       (const cd*)ms, (cd*)H1, (cd*)A);
     AFQ_HIP(hipGetLastError());
     // ||A||_1 on the host chooses the scaling
     std::vector<double> hA(2 * (size_t)M * M);
@@ -180,8 +180,40 @@ void setup_hamiltonian(afq_ham_s* H, hipStream_t s) {
                          (const cd*)P, (const cd*)P, 0, (cd*)Q);
       std::swap(P, Q);
     }
-    AFQ_HIP(hipMemcpyAsync(H->P1, P, 2 * (size_t)M * M * sizeof(double),
-                           hipMemcpyDeviceToDevice, s));
+    AFQ_HIP(hipMemcpyAsync(P1, P, 2 * (size_t)M * M * sizeof(double), hipMemcpyDeviceToDevice,
+                           s));
+    AFQ_HIP(hipGetLastError());
+    AFQ_HIP(hipStreamSynchronize(s));
+  } catch (...) {
+    (void)hipFree(tmp);
+    throw;
+  }
+  AFQ_HIP(hipFree(tmp));
+}
+
+void setup_hamiltonian(afq_ham_s* H, hipStream_t s) {
+  const int M = H->M, N = H->N, K = H->K;
+  const long nL = (long)K * N * M;
+  hipLaunchKernelGGL(k_half_rotate, blocks(nL + (long)N * M), dim3(256), 0, s, M, N, K,
+                     (const cd*)H->PsiT, H->h, H->Likn, (cd*)H->haj, (cd*)H->Lnak);
+  launch_v0(s, M, K, H->Likn, H->v0);
+  AFQ_HIP(hipGetLastError());
+
+  // scratch: Phi0 [M*N], Gt [N*M] (c128)
+  double* tmp = nullptr;
+  AFQ_HIP(hipMalloc(&tmp, 2 * (size_t)M * N * 2 * sizeof(double)));
+  double* Phi0 = tmp;
+  double* Gt = Phi0 + 2 * (size_t)M * N;
+  try {
+    hipLaunchKernelGGL(k_trial_phi, blocks((long)M * N), dim3(256), 0, s, M, N,
+                       (const cd*)H->PsiT, (cd*)Phi0);
+    launch_green(s, 1, M, N, H->PsiT, Phi0, (long)M * N, Gt, nullptr, 0, 1.0, cmk(0, 0),
+                 nullptr, 0.0, 0.0, nullptr, 0);
+    const double vscale = std::sqrt(H->dt) * (H->walker_type == AFQ_CLOSED ? 2.0 : 1.0);
+    launch_gemm_nt_z(s, AFQ_FP64, 1, K, N * M, vscale, Gt, (long)N * M, H->Lnak, (long)N * M,
+                     true, H->vMF, K);
+    launch_zero_imag(s, K, H->vMF);
+    build_one_body_propagator(s, M, K, H->dt, H->h, H->v0, H->Likn, H->vMF, H->H1, H->P1);
     // FP32 copies of the Cholesky tensors for the FP32 contraction form
     hipLaunchKernelGGL(k_to_f32, blocks(2 * nL), dim3(256), 0, s, 2 * nL, H->Lnak, H->Lnak_f);
     hipLaunchKernelGGL(k_to_f32, blocks((long)M * M * K), dim3(256), 0, s, (long)M * M * K,

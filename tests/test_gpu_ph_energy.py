@@ -5,6 +5,7 @@ import numpy as np
 import pytest
 
 from oracle import ph_energy as P
+from oracle import ph_energy as P_
 from tests._helpers import rand_c
 from tests.test_oracle_ph_energy import ph_system
 
@@ -129,4 +130,60 @@ def test_ph_vmf_matches_oracle(uhf, nmo, nel, ndet, rank):
     v = ph.vMF(0.005).cpu().numpy()
     vo = P.vmf(s["struct"], s["A"], s["Lna"], 0.005)
     np.testing.assert_allclose(v, vo, rtol=1e-10, atol=1e-12 * np.abs(vo).max())
+    ph.close()
+
+
+@pytest.mark.parametrize("hybrid", [True, False])
+def test_ph_step_matches_oracle(hybrid):
+    """Seeded properties, then two walker steps and an orthogonalization of COLLINEAR walkers
+    with the PHMSD trial, each from the same buffer on both sides (step-level parity)."""
+    from oracle import oracle as O
+    rng = np.random.default_rng(120 + hybrid)
+    nmo, nup, ndn, K = 12, 4, 3, 10
+    s = ph_system(rng, nmo, nup, ndn, K, 30, 3)
+    Likn = np.ascontiguousarray(s["L"].transpose(1, 2, 0))
+    E0, dt, W = 0.5, 0.01, 20
+    ph = A.PHMSD(s["occs"], s["ci"], s["A"][0][None], s["h"], Likn, E0, walker_block=8)
+    ph.prepare(dt, W)
+    ph.set_params(hybrid=int(hybrid))
+    buf = ph.new_walkers(W)
+    P = nmo * (nup + ndn)
+    buf[:, :P] += 0.05 * rand_c(rng, W, P)
+    bd = A.to_device(buf)
+    ph.seed(bd)
+    host = bd.cpu().numpy()
+    Pu = host[:, :nmo * nup].reshape(W, nmo, nup)
+    Pd = host[:, nmo * nup:P].reshape(W, nmo, ndn)
+    Eo, ovo = P_.energy_alg1(s["struct"], s["A"], s["Lna"], s["haj"], E0, Pu, Pd)
+    np.testing.assert_allclose(host[:, P + A.P_E1:P + A.P_EJ + 1], Eo, rtol=1e-10,
+                               atol=1e-10 * np.abs(Eo).max())
+    d = host[:, P + A.P_OVLP] - ovo
+    d.imag = (d.imag + np.pi) % (2 * np.pi) - np.pi
+    assert np.abs(d).max() < 1e-10
+    vMF = P_.vmf(s["struct"], s["A"], s["Lna"], dt).real.astype(complex)
+    P1, _ = O.one_body_propagator(s["h"], Likn, vMF, dt)
+    Eshift = float(Eo.sum(1).real.mean())
+    for t in range(2):
+        xi = rng.standard_normal((W, K))
+        before = bd.cpu().numpy()
+        ph.step(bd, A.to_device(xi, torch.float64), Eshift)
+        got = bd.cpu().numpy()
+        ref = P_.ph_step(s["struct"], s["A"], s["Lna"], s["haj"], E0, Likn, vMF, P1, dt, before,
+                         nmo, nup, ndn, xi, Eshift, hybrid)
+        np.testing.assert_allclose(got[:, :P], ref[:, :P], rtol=1e-10,
+                                   atol=1e-11 * np.abs(ref[:, :P]).max())
+        for p in (A.P_WEIGHT, A.P_PHASE, A.P_PSEUDO_ELOC, A.P_E1, A.P_EXX, A.P_EJ):
+            np.testing.assert_allclose(got[:, P + p], ref[:, P + p], rtol=1e-9,
+                                       atol=1e-9 * max(1.0, np.abs(ref[:, P + p]).max()), err_msg=str(p))
+    before = bd.cpu().numpy()
+    ph.Orthogonalize(bd)
+    got = bd.cpu().numpy()
+    for sp, (lo, n) in enumerate(((0, nup), (nmo * nup, ndn))):
+        Q, _ = O.orthogonalize(before[:, lo:lo + nmo * n].reshape(W, nmo, n))
+        np.testing.assert_allclose(got[:, lo:lo + nmo * n].reshape(W, nmo, n), Q, rtol=1e-10, atol=1e-12)
+    _, ovq = P_.mixed_density_matrix(s["struct"], s["A"], got[:, :nmo * nup].reshape(W, nmo, nup),
+                                     got[:, nmo * nup:P].reshape(W, nmo, ndn))
+    d = got[:, P + A.P_OVLP] - ovq
+    d.imag = (d.imag + np.pi) % (2 * np.pi) - np.pi
+    assert np.abs(d).max() < 1e-10
     ph.close()
