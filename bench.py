@@ -191,6 +191,64 @@ def cpu_baseline(s, W, ortho, seconds):
                        f"(QR every {ortho}), C++ FP64 oracle, OpenMP over walkers, {el:.1f} s")}
 
 
+def bench_phmsd(args, A, torch, synth):
+    """c5: the PHMSD walker step (COLLINEAR walkers, 1000 determinants) in the local-energy
+    form, FP64 (the PHMSD energy has no reduced-precision form). A step = one pass of
+    Propagate over all walkers; QR every 5 steps and population control every 10, as the c2 leg.
+    Eager launches, timed with HIP events per step (a progress line per step)."""
+    import numpy as np
+    s, W, steps, ortho = synth.workload("c5")
+    steps = args.steps if args.steps != 100 else 5
+    warmup = min(args.warmup, 1)
+    occs, ci = synth.phmsd_expansion(s.M, s.N, 1000, s.seed)
+    # all M orbitals of the trial's QR as rows (its first N are the c1-c3 trial's)
+    orb = np.ascontiguousarray(s.meta["orbitals"].conj().T).astype(np.complex128)[None]
+    ph = A.PHMSD(occs, ci, orb, s.h, s.Likn, 0.0, walker_block=64)
+    ph.prepare(s.dt, W)
+    ph.set_params(hybrid=0)
+    xi = torch.from_numpy(synth.fields(s.seed, warmup + steps, W, s.K)).to("cuda")
+    uniforms = np.random.default_rng(s.seed).random(warmup + steps)
+    print(f"[bench] c5: M={s.M} N={s.N}+{s.N} K={s.K} W={W}, {len(ci)} determinants "
+          f"({ph.nunique[0]} + {ph.nunique[1]} unique excitations)", file=sys.stderr, flush=True)
+    b = A.to_device(ph.new_walkers(W))
+    ph.seed(b)
+    E, _ = ph.energy(b)
+    Eshift = float(E.sum(1).real.mean())
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    total = 0.0
+    for t in range(warmup + steps):
+        start.record()
+        ph.step(b, xi[t], Eshift)
+        if t % ortho == 0:
+            ph.Orthogonalize(b)
+        if t % POPCTL_INTERVAL == 0:
+            ph.population_control(b, float(uniforms[t]))
+        stop.record()
+        torch.cuda.synchronize()
+        ms = start.elapsed_time(stop)
+        if t >= warmup:
+            total += ms
+        print(f"[bench] c5 step {t}: {ms:.1f} ms", file=sys.stderr, flush=True)
+    ms = total / steps
+    value = W / (ms / 1e3)
+    print(f"[bench] c5 fp64: {ms:.2f} ms/step, {value:.0f} walker-steps/s", file=sys.stderr,
+          flush=True)
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "walker-steps/s", "n_gpus": 1,
+        "steps": steps, "warmup": warmup, "ms_per_step": round(ms, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp64",
+        "data": "synthetic",
+        "config": {"workload": f"c5: PHMSD trial, NMO={s.M}, {s.N}+{s.N} electrons, "
+                               f"Nchol={s.K}, {len(ci)} determinants of rank 1-4, {W} walkers, "
+                               f"dt={s.dt}, local-energy step, QR every {ortho}",
+                   "form": "FP64, eager launches", "parallelism": "single GPU"},
+        "roofline": None, "cpu_baseline": None,
+    }
+    ph.close()
+    print(json.dumps(out))
+    return 0
+
+
 def bench_batch(args, A, torch, synth):
     """c4: 32 independent systems of uneven size. Each form runs every system's steps one system
     at a time (eager launches: the reference's way, one job per system), then as one captured
@@ -265,7 +323,7 @@ def main(argv=None):
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
     ap.add_argument("--streams", type=int, default=4,
                     help="c4: capture streams the systems are spread over")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -290,6 +348,8 @@ def main(argv=None):
 
     if args.workload == "c4":
         return bench_batch(args, A, torch, synth)
+    if args.workload == "c5":
+        return bench_phmsd(args, A, torch, synth)
     s, W, _, ortho = synth.workload(args.workload)
     M, N, K = s.M, s.N, s.K
     nst = max(args.warmup + args.steps, 31)   # the accuracy gate replays 31 steps

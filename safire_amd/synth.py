@@ -12,6 +12,9 @@ The five workloads of the walker-step benchmark, built deterministically from fi
 * E0 = 0, dt = 0.005, every walker starts at the trial with weight 1.
 * Fields xi[step][w][n] ~ N(0,1), one seeded stream per workload (they replace probit(U) of
   construct_X_kernel.hpp:71; U is uniform so probit(U) ~ N(0,1) too).
+* c5 (PHMSD): the same Hamiltonian construction at M = 64, 10 + 10 electrons, Nchol = 320; the
+  trial orbitals are all M columns of Q (as rows: Q^H), with the CI expansion of
+  phmsd_expansion (1000 determinants of rank 1-4).
 """
 from __future__ import annotations
 
@@ -59,7 +62,41 @@ def make_system(M: int, N: int, K: int, seed: int, decay: float = 24.0, scale: f
     Q, _ = np.linalg.qr(rng.standard_normal((M, M)))
     PsiT = np.ascontiguousarray(Q[:, :N].conj().T).astype(np.complex128)
     return System(M=M, N=N, K=K, h=np.ascontiguousarray(h), Likn=Likn, PsiT=PsiT, seed=seed,
-                  name=name)
+                  name=name, meta={"orbitals": Q})
+
+
+def phmsd_expansion(M: int, N: int, ndet: int, seed: int, max_rank: int = 4):
+    """The c5 CI expansion over the RHF reference (orbitals 0..N-1 of each spin): determinant 0 is
+    the reference; every other determinant has a total excitation rank r uniform in 1..max_rank,
+    split between the spins uniformly over the feasible (r_up, r_dn), with holes drawn uniformly
+    from the occupied orbitals and particles from the virtual ones; duplicates are redrawn.
+    c_k ~ N(0,1) exp(-r_k), scaled so that c_0 = 0.9 and sum_k c_k^2 = 1.
+    Returns (occs [ndet][2N] int32, sorted, down spin offset by M; ci [ndet] complex128)."""
+    rng = np.random.default_rng([seed, 5])
+    occs = [list(range(N)) + list(range(M, M + N))]
+    ranks = [0]
+    seen = {tuple(occs[0])}
+    while len(occs) < ndet:
+        r = int(rng.integers(1, max_rank + 1))
+        ru = int(rng.integers(max(0, r - N), min(r, N) + 1))
+        conf = []
+        for nexc, base in ((ru, 0), (r - ru, M)):
+            occ = list(range(base, base + N))
+            holes = rng.choice(N, nexc, replace=False)
+            parts = rng.choice(np.arange(base + N, base + M), nexc, replace=False)
+            for hh, p in zip(holes, parts):
+                occ[hh] = int(p)
+            conf += sorted(occ)
+        if tuple(conf) in seen:
+            continue
+        seen.add(tuple(conf))
+        occs.append(conf)
+        ranks.append(r)
+    ranks = np.array(ranks)
+    c = rng.standard_normal(ndet) * np.exp(-ranks)
+    c[1:] *= np.sqrt(1.0 - 0.81) / np.linalg.norm(c[1:])
+    c[0] = 0.9
+    return np.array(occs, np.int32), c.astype(np.complex128)
 
 
 def batch_workload(count: int = 32, seed: int = SEEDS["c4"]) -> tuple[list, int, int]:
