@@ -645,8 +645,20 @@ int afq_ph_excited_2body_energy(int nw, int ndet, int nex, int nact, int nelec, 
   return guard([&] {
     AFQ_CHECK(nw >= 0 && ndet >= 0 && nact > 0 && nelec > 0 && nelec <= nact && nchol >= 0,
               AFQ_EBADSHAPE, "ph 2-body energy: bad sizes");
-    launch_ph_2body(S(stream), nw, ndet, nex, nact, nelec, nchol, iexcit, refc, Twina, R, wgt,
-                    EX, EJ, KE);
+    // the staged kernel's group partials: a temporary of this call (the PHMSD handle keeps its
+    // own for the walker step)
+    const long need = ph_2body_workspace(nw, nact, nelec, nchol);
+    double* ws = nullptr;
+    if (need > 0 && nw > 0 && ndet > 0) AFQ_HIP(hipMalloc(&ws, need * sizeof(double)));
+    try {
+      launch_ph_2body(S(stream), nw, ndet, nex, nact, nelec, nchol, iexcit, refc, Twina, R,
+                      wgt, EX, EJ, KE, ws, need);
+      if (ws) AFQ_HIP(hipStreamSynchronize(S(stream)));
+    } catch (...) {
+      if (ws) (void)hipFree(ws);
+      throw;
+    }
+    if (ws) AFQ_HIP(hipFree(ws));
   });
 }
 

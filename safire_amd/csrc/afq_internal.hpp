@@ -254,9 +254,12 @@ void build_one_body_propagator(hipStream_t s, int M, int K, double dt, const dou
                                const double* v0, const double* Likn, const double* vMF,
                                double* H1, double* P1);
 // k_phmsd.hip
+// ws: ph_2body_workspace() doubles for the staged kernel (NULL or too small: the per-walker one)
+long ph_2body_workspace(int nw, int nact, int nelec, int nchol);
 void launch_ph_2body(hipStream_t s, int nw, int ndet, int nex, int nact, int nelec, int nchol,
                      const int* iexcit, const int* refc, const double* Twina, const double* R,
-                     const double* wgt, double* EX, double* EJ, double* KE);
+                     const double* wgt, double* EX, double* EJ, double* KE,
+                     double* ws = nullptr, long ws_doubles = 0);
 void launch_ph_1body(hipStream_t s, int nw, int ndet, int nex, int nact, int nelec,
                      const int* iexcit, const int* refc, const double* S, const double* R,
                      const double* wgt, double* E);

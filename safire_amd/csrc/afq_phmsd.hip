@@ -169,8 +169,8 @@ struct afq_ph_t {
          *acc = nullptr;
   double *Twina = nullptr, *Swia = nullptr, *KEa = nullptr, *KEb = nullptr, *KEl = nullptr,
          *R = nullptr;
-  double *Racc = nullptr, *Gblk = nullptr, *split = nullptr;
-  long split_elems = 0;
+  double *Racc = nullptr, *Gblk = nullptr, *split = nullptr, *p2ws = nullptr;
+  long split_elems = 0, p2ws_doubles = 0;
   ~afq_ph_t() {
     for (void* p : owned) (void)hipFree(p);
     for (void* p : step_owned) (void)hipFree(p);
@@ -277,7 +277,7 @@ void ph_energy_all(afq_ph_t* ph, hipStream_t s, int nw, const double* d_wbuf, lo
                           ph->Ex);
           launch_ph_2body(s, nb, nbk, sh.nex, na, S[0].nel, K, iex, S[0].refc, ph->Twina, ph->R,
                           w, ph->Ex + 2L * nb, ph->Ex + 4L * nb,
-                          ph->KEa + 2L * (sh.row + d0) * nb * K);
+                          ph->KEa + 2L * (sh.row + d0) * nb * K, ph->p2ws, ph->p2ws_doubles);
         }
       launch_ph_accumulate(s, nb, nullptr, nullptr, ph->Ex, ph->acc);
       // KEb[d] = ratio_dn[d] sum_u C[d][u] ratio_up[u] KEa[u]
@@ -303,7 +303,8 @@ void ph_energy_all(afq_ph_t* ph, hipStream_t s, int nw, const double* d_wbuf, lo
           launch_ph_1body(s, nb, nbk, sh.nex, na, S[1].nel, iex, S[1].refc, ph->Swia, ph->R, w,
                           ph->Ex);
           launch_ph_2body(s, nb, nbk, sh.nex, na, S[1].nel, K, iex, S[1].refc, ph->Twina, ph->R,
-                          w, ph->Ex + 2L * nb, ph->Ex + 4L * nb, ph->KEl);
+                          w, ph->Ex + 2L * nb, ph->Ex + 4L * nb, ph->KEl, ph->p2ws,
+                          ph->p2ws_doubles);
           launch_ph_ke_dot(s, nb, K, nbk, ph->KEb + 2L * (sh.row + d0) * nb * K, ph->KEl,
                            ph->Ex + 4L * nb, 1);
         }
@@ -663,6 +664,8 @@ int afq_ph_create(int nmo, int nup, int ndn, int nchol, int ndet, const int* occ
     P->R = dev_alloc<double>(own, 2 * nb * dblk * max_nex * na);
     P->Racc = dev_alloc<double>(own, 2 * nb * maxel * na);
     P->Gblk = dev_alloc<double>(own, 2 * nb * 2 * na * nmo);
+    P->p2ws_doubles = ph_2body_workspace((int)nb, (int)na, maxel, (int)K);
+    if (P->p2ws_doubles > 0) P->p2ws = dev_alloc<double>(own, P->p2ws_doubles);
     P->split_elems = (long)kMaxSplits * nb * K;
     P->split = dev_alloc<double>(own, 2 * P->split_elems);
     *out = P.release();

@@ -24,6 +24,7 @@ constexpr int kPhThreads = 256;
 constexpr int kPhMaxEx = 8;        // excitation rank handled per determinant
 constexpr int kPhMaxAct = 256;     // active orbitals
 constexpr int kPhMaxElec = 64;
+constexpr long kP2Lds = 64 * 1024;   // LDS bound of the staged 2-body kernel (bytes)
 
 __device__ inline cd block_sum(cd v, cd* red) {
   const int t = threadIdx.x;
@@ -411,6 +412,170 @@ __global__ __launch_bounds__(256) void k_ph_spin_weights(int nw, int nu, const i
   if (threadIdx.x == 0) ov[w] = sum;
 }
 
+// The same 2-body energy with the Cholesky index split over workgroups: a workgroup takes one
+// walker and kP2N consecutive vectors n, stages T[i][n][a] of those vectors in LDS once and
+// sweeps every determinant against them (kP2D determinant lanes per vector), so Twina is read
+// from memory once per walker instead of once per determinant. Per (n, d) the thread forms
+// Z[j][e] = sum_a T[j][n][a] R_e[a] row by row from LDS (R_d from the L1 cache) and keeps
+// Y[e][f] = Z[p_e][f]; the terms are those of k_ph_2body. Each workgroup writes its weighted
+// sums (a fixed-order workgroup reduction) to part[w][group]; k_ph_2body_finish adds the groups
+// of a walker in order.
+constexpr int kP2N = 4;     // Cholesky vectors per workgroup
+constexpr int kP2D = 32;    // determinant lanes per vector
+constexpr int kP2Threads = kP2N * kP2D;
+
+template <int NEX>
+__global__ __launch_bounds__(kP2Threads) void k_ph_2body_lds(
+    int nw, int ndet, int nact, int nelec, int nchol, int ngrp, const int* __restrict__ iexcit,
+    const int* __restrict__ refc, const cd* __restrict__ Twina, const cd* __restrict__ R,
+    const cd* __restrict__ wgt, cd* __restrict__ part, cd* __restrict__ KE) {
+  extern __shared__ cd sT[];                 // [kP2N][nelec][nact]
+  __shared__ int sRef[kPhMaxElec];
+  __shared__ int sBad;
+  __shared__ cd red[kP2Threads / 64];
+  const int w = blockIdx.x, grp = blockIdx.y;
+  const int t = threadIdx.x, nl = t / kP2D, dl = t % kP2D;
+  const int n0 = grp * kP2N, n = n0 + nl;
+  const cd* Tw = Twina + (long)w * nelec * nchol * nact;
+  const int per = nelec * nact;
+  for (int e = t; e < kP2N * per; e += kP2Threads) {
+    const int q = e / per, rem = e - q * per, i = rem / nact, a = rem - i * nact;
+    sT[e] = (n0 + q < nchol) ? Tw[((long)i * nchol + n0 + q) * nact + a] : cmk(0.0, 0.0);
+  }
+  if (t == 0) {
+    int bad = 0;
+    for (int i = 0; i < nelec; ++i) bad |= (refc[i] < 0 || refc[i] >= nact);
+    sBad = bad;
+  }
+  for (int i = t; i < nelec; i += kP2Threads) sRef[i] = min(max(refc[i], 0), nact - 1);
+  __syncthreads();
+  const bool bad_ref = sBad != 0;
+  const cd* Tn = sT + (long)nl * per;        // T[i][n][a] = Tn[i * nact + a]
+  cd ex_acc = cmk(0.0, 0.0), ej_acc = cmk(0.0, 0.0);
+  if (n < nchol) {
+    cd f0 = cmk(0.0, 0.0);
+    for (int i = 0; i < nelec; ++i) f0 += Tn[i * nact + sRef[i]];
+    const cd ej0 = f0 * f0;
+    for (int d = dl; d < ndet; d += kP2D) {
+      const int* ex = iexcit + (long)d * 2 * NEX;
+      int hole[NEX], p[NEX], v[NEX];
+      bool bad = bad_ref;
+#pragma unroll
+      for (int e = 0; e < NEX; ++e) {
+        hole[e] = ex[e];
+        bad |= (ex[e] < 0 || ex[e] >= nelec || ex[NEX + e] < 0 || ex[NEX + e] >= nact);
+        p[e] = min(max(ex[e], 0), nelec - 1);
+        v[e] = min(max(ex[NEX + e], 0), nact - 1);
+      }
+      // occupation of row i: the reference's, or the orbital the first excitation of i goes to
+      auto occ = [&](int i) {
+        int o = sRef[i];
+        bool hit = false;
+#pragma unroll
+        for (int e = 0; e < NEX; ++e)
+          if (!hit && hole[e] == i) {
+            o = v[e];
+            hit = true;
+          }
+        return o;
+      };
+      const cd* Rd = R + (((long)w * ndet + d) * NEX) * nact;
+      cd Y[NEX][NEX];
+#pragma unroll
+      for (int e = 0; e < NEX; ++e)
+#pragma unroll
+        for (int f = 0; f < NEX; ++f) Y[e][f] = cmk(0.0, 0.0);
+      cd eX = cmk(0.0, 0.0), kn = cmk(0.0, 0.0);
+      for (int j = 0; j < nelec; ++j) {
+        cd z[NEX];
+#pragma unroll
+        for (int e = 0; e < NEX; ++e) z[e] = cmk(0.0, 0.0);
+        const cd* Tj = Tn + j * nact;
+        for (int a = 0; a < nact; ++a) {
+          const cd tj = Tj[a];
+#pragma unroll
+          for (int e = 0; e < NEX; ++e) z[e] += tj * Rd[e * nact + a];
+        }
+        // R_e against the occupied columns: T[p_e][n][occ j] Z[j][e]
+        const int oj = occ(j);
+#pragma unroll
+        for (int e = 0; e < NEX; ++e) eX += 2.0 * (Tn[p[e] * nact + oj] * z[e]);
+#pragma unroll
+        for (int e = 0; e < NEX; ++e)
+          if (p[e] == j)
+#pragma unroll
+            for (int f = 0; f < NEX; ++f) Y[e][f] = z[f];
+      }
+#pragma unroll
+      for (int e = 0; e < NEX; ++e) {
+        eX += Y[e][e] * Y[e][e];
+        kn += Y[e][e];
+#pragma unroll
+        for (int f = e + 1; f < NEX; ++f) eX += 2.0 * (Y[f][e] * Y[e][f]);
+      }
+      // rows whose occupation changed
+      for (int i = 0; i < nelec; ++i) {
+        const int oi = occ(i), ri = sRef[i];
+        const cd tio = Tn[i * nact + oi], tir = Tn[i * nact + ri];
+        if (oi != ri) eX += tio * tio - tir * tir;
+        for (int j = i + 1; j < nelec; ++j) {
+          const int oj = occ(j), rj = sRef[j];
+          if (oi != ri || oj != rj)
+            eX += 2.0 * (Tn[i * nact + oj] * Tn[j * nact + oi] -
+                         Tn[i * nact + rj] * Tn[j * nact + ri]);
+        }
+        kn += tio;
+      }
+      KE[((long)d * nw + w) * nchol + n] = kn;
+      const cd wd = wgt[(long)d * nw + w];
+      if (bad) eX = cmk(__builtin_nan(""), __builtin_nan(""));
+      ex_acc += wd * eX;
+      ej_acc += wd * (kn * kn - ej0);
+    }
+  }
+  // fixed-order workgroup sums (every thread takes part in both)
+  cd sx, sj;
+  {
+    cd v = ex_acc;
+    for (int off = 32; off > 0; off >>= 1) {
+      v.re += __shfl_xor(v.re, off, 64);
+      v.im += __shfl_xor(v.im, off, 64);
+    }
+    if ((t & 63) == 0) red[t >> 6] = v;
+    __syncthreads();
+    sx = cmk(0.0, 0.0);
+    for (int i = 0; i < kP2Threads / 64; ++i) sx += red[i];
+    __syncthreads();
+    v = ej_acc;
+    for (int off = 32; off > 0; off >>= 1) {
+      v.re += __shfl_xor(v.re, off, 64);
+      v.im += __shfl_xor(v.im, off, 64);
+    }
+    if ((t & 63) == 0) red[t >> 6] = v;
+    __syncthreads();
+    sj = cmk(0.0, 0.0);
+    for (int i = 0; i < kP2Threads / 64; ++i) sj += red[i];
+  }
+  if (t == 0) {
+    part[2 * ((long)w * ngrp + grp)] = sx;
+    part[2 * ((long)w * ngrp + grp) + 1] = sj;
+  }
+}
+
+// EX[w] -= 1/2 sum_g part[w][g].ex, EJ[w] += 1/2 sum_g part[w][g].ej, groups in order
+__global__ void k_ph_2body_finish(int nw, int ngrp, const cd* __restrict__ part,
+                                  cd* __restrict__ EX, cd* __restrict__ EJ) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= nw) return;
+  cd sx = cmk(0.0, 0.0), sj = cmk(0.0, 0.0);
+  for (int g = 0; g < ngrp; ++g) {
+    sx += part[2 * ((long)w * ngrp + g)];
+    sj += part[2 * ((long)w * ngrp + g) + 1];
+  }
+  EX[w] -= 0.5 * sx;
+  EJ[w] += 0.5 * sj;
+}
+
 // ph_ref_energy_impl's reductions (Real3IndexFactorization.hpp:646-694) over one walker's
 // Twina [nel][nchol][nact] and Swia [nel][nact], with the reference at orbitals 0..nel-1 as the
 // reference assumes:
@@ -562,13 +727,47 @@ __global__ void k_ph_fill(long n, cd v, cd* __restrict__ x) {
 
 }  // namespace
 
+long ph_2body_workspace(int nw, int nact, int nelec, int nchol) {
+  // doubles of the LDS kernel's group partials; 0 when its Twina slices do not fit in LDS
+  if ((long)kP2N * nelec * nact * 16 > kP2Lds) return 0;
+  return 4L * nw * ((nchol + kP2N - 1) / kP2N);
+}
+
 void launch_ph_2body(hipStream_t s, int nw, int ndet, int nex, int nact, int nelec, int nchol,
                      const int* iexcit, const int* refc, const double* Twina, const double* R,
-                     const double* wgt, double* EX, double* EJ, double* KE) {
+                     const double* wgt, double* EX, double* EJ, double* KE, double* ws,
+                     long ws_doubles) {
   if (nw <= 0) return;
   AFQ_CHECK(nex >= 1 && nex <= kPhMaxEx, AFQ_EUNSUPPORTED, "ph 2-body energy: nex must be 1..8");
   AFQ_CHECK(nact <= kPhMaxAct && nelec <= kPhMaxElec, AFQ_EUNSUPPORTED,
             "ph 2-body energy: nact <= 256 and nelec <= 64");
+  const long need = ph_2body_workspace(nw, nact, nelec, nchol);
+  if (ndet > 0 && nchol > 0 && need > 0 && ws != nullptr && ws_doubles >= need) {
+    const int ngrp = (nchol + kP2N - 1) / kP2N;
+    const size_t lds = (size_t)kP2N * nelec * nact * sizeof(cd);
+    auto go2 = [&](auto kern) {
+      hipLaunchKernelGGL(kern, dim3(nw, ngrp), dim3(kP2Threads), lds, s, nw, ndet, nact, nelec,
+                         nchol, ngrp, iexcit, refc, reinterpret_cast<const cd*>(Twina),
+                         reinterpret_cast<const cd*>(R), reinterpret_cast<const cd*>(wgt),
+                         reinterpret_cast<cd*>(ws), reinterpret_cast<cd*>(KE));
+    };
+    switch (nex) {
+      case 1: go2(k_ph_2body_lds<1>); break;
+      case 2: go2(k_ph_2body_lds<2>); break;
+      case 3: go2(k_ph_2body_lds<3>); break;
+      case 4: go2(k_ph_2body_lds<4>); break;
+      case 5: go2(k_ph_2body_lds<5>); break;
+      case 6: go2(k_ph_2body_lds<6>); break;
+      case 7: go2(k_ph_2body_lds<7>); break;
+      default: go2(k_ph_2body_lds<8>); break;
+    }
+    AFQ_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_ph_2body_finish, dim3((nw + 255) / 256), dim3(256), 0, s, nw, ngrp,
+                       reinterpret_cast<const cd*>(ws), reinterpret_cast<cd*>(EX),
+                       reinterpret_cast<cd*>(EJ));
+    AFQ_HIP(hipGetLastError());
+    return;
+  }
   auto go = [&](auto kern) {
     hipLaunchKernelGGL(kern, dim3(nw), dim3(kPhThreads), 0, s, nw, ndet, nact, nelec, nchol,
                        iexcit, refc, reinterpret_cast<const cd*>(Twina),
