@@ -42,7 +42,7 @@ GATE_HA = 1e-5
 # HBM bytes per launch of each form's exchange kernel (FETCH_SIZE and WRITE_SIZE counter runs
 # scaled by a copy kernel of known size), as recorded in profiles/walker_step_c2_hbm_traffic.md;
 # null for a workload or form without such a measurement
-TRAFFIC_BYTES = {("c2", "int8"): 38.55e6, ("c2", "fp64"): 53.22e6}
+TRAFFIC_BYTES = {("c2", "int8"): 38.54e6, ("c2", "fp64"): 53.29e6}
 TRAFFIC_SOURCE = "profiles/walker_step_c2_hbm_traffic.md"
 
 

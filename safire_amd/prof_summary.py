@@ -72,6 +72,8 @@ def traffic(fetch_dir: str, write_dir: str, cal_fetch: str, cal_write: str, top:
            "HBM write (MB) | total (MB) |", "|---|---:|---:|---:|---:|---:|---:|"]
     rows = []
     for k in set(fetch) & set(write):
+        if not k.startswith("k_"):
+            continue          # the library's kernels only (not torch's, in the bench's checks)
         rd = fetch[k][0] * ff / 1e6
         wr = write[k][0] * fw / 1e6
         rows.append((rd + wr, k, fetch[k], write[k], rd, wr))
