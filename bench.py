@@ -27,6 +27,7 @@ import argparse
 import json
 import os
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -261,7 +262,8 @@ def bench_batch(args, A, torch, synth):
     steps = args.steps if args.steps != 100 else steps
     res = {}
     for name, prec in (("fp64", A.FP64), ("int8", A.INT8_SPLIT)):
-        b = SystemBatch(systems, steps=steps, ortho=ortho, precision=prec)
+        b = SystemBatch(systems, steps=steps, ortho=ortho, precision=prec,
+                        tile_cache=args.tile_cache)
         b.run_sequential()                                  # warm-up (kernel attributes)
         b.reset()
         torch.cuda.synchronize()
@@ -283,7 +285,8 @@ def bench_batch(args, A, torch, synth):
         bat = start.elapsed_time(stop) / 1e3
         same = all(torch.equal(it["wset"].buf, r) for it, r in zip(b.items, ref))
         ws = b.walker_steps()
-        res[name] = {"batched_walker_steps_per_s": round(ws / bat, 1),
+        res[name] = {"tilings_autotuned": len(b.tilings),
+                     "batched_walker_steps_per_s": round(ws / bat, 1),
                      "one_at_a_time_walker_steps_per_s": round(ws / seq, 1),
                      "batched_ms_per_step": round(bat * 1e3 / steps, 4),
                      "one_at_a_time_ms_per_step": round(seq * 1e3 / steps, 4),
@@ -326,6 +329,9 @@ def main(argv=None):
     ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
     ap.add_argument("--streams", type=int, default=4,
                     help="c4: capture streams the systems are spread over")
+    ap.add_argument("--tile-cache", default=os.path.join(tempfile.gettempdir(),
+                                                         "safire_amd_tiles.json"),
+                    help="c4: the GEMM autotuner's JSON cache (every system's tilings are tuned)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--forms", default="fp64,fp32,bf16x3,int8")

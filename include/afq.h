@@ -103,6 +103,14 @@ int afq_ham_destroy(afq_ham ham);
 int afq_ham_reserve(afq_ham ham, int max_walkers);
 int afq_ham_set_precision(afq_ham ham, int precision);
 int afq_ham_set_params(afq_ham ham, const afq_step_params* p);
+/* Autotuner (no counterpart in the reference, which leaves tile shapes to its libraries): times
+ * every tiling of the vbias and vHS GEMMs this precision has for W walkers (afq_ham_reserve
+ * capacity) and keeps the fastest for the handle; results do not depend on the choice (every
+ * tiling sums k in the same order). cache_path (NULL: none) is a JSON object of
+ * "gemm/precision/W/shape": tiling entries, read first (a cached shape is not timed again) and
+ * written back. cfg_out[2] (may be NULL) receives the (vbias, vHS) tilings. The walker buffer is
+ * not touched; the workspace is. afq_ham_set_precision resets the tilings. */
+int afq_autotune(afq_ham ham, int W, const char* cache_path, int* cfg_out, void* stream);
 int afq_ham_get_params(afq_ham ham, afq_step_params* p);
 /* Copies the derived tensors to host memory (any pointer may be NULL): haj [N][M] c128,
  * Lnak [K][N][M] c128, v0 [M][M] f64, vMF [K] c128, P1 [M][M] c128, H1 [M][M] c128. */

@@ -91,6 +91,7 @@ SIGNATURES = {
     "afq_ph_mixed_density_matrix": (_i, [_vp, _i, _vp, _l, _i, _vp, _vp, _vp]),
     "afq_ph_vbias": (_i, [_vp, _i, _vp, _l, _d, _vp, _vp]),
     "afq_ph_vmf": (_i, [_vp, _d, _vp, _vp]),
+    "afq_autotune": (_i, [_vp, _i, C.c_char_p, _vp, _vp]),
     "afq_ph_prepare": (_i, [_vp, _d, _i, _vp]),
     "afq_ph_set_params": (_i, [_vp, _vp]),
     "afq_ph_seed_properties": (_i, [_vp, _i, _vp, _l, _vp]),
@@ -199,6 +200,15 @@ class HamiltonianOperations:
     def reserve(self, W):
         check(lib().afq_ham_reserve(self.handle, W))
         self.capacity = max(self.capacity, W)
+
+    def autotune(self, W, cache_path=None, stream=None):
+        """Picks the fastest GEMM tilings for W walkers (cached in cache_path, a JSON file);
+        returns the (vbias, vHS) tilings."""
+        self.reserve(W)
+        out = (C.c_int * 2)()
+        path = None if cache_path is None else os.fsencode(cache_path)
+        check(lib().afq_autotune(self.handle, int(W), path, out, stream_ptr(stream)))
+        return out[0], out[1]
 
     def set_precision(self, precision):
         check(lib().afq_ham_set_precision(self.handle, precision))

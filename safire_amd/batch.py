@@ -22,14 +22,19 @@ POPCTL_INTERVAL = 10
 
 class SystemBatch:
     def __init__(self, systems, steps: int, ortho: int = 5, precision: int = A.FP64,
-                 hybrid: bool = False, seed: int = 0):
+                 hybrid: bool = False, seed: int = 0, tile_cache: str | None = None):
+        """tile_cache: the autotuner's JSON cache; when given, every system's GEMM tilings are
+        tuned for its shape and walker count (cached shapes are not timed again)."""
         torch = A._torch()
         self.steps, self.ortho = steps, ortho
         self.items = []
+        self.tilings = []
         for i, (s, W) in enumerate(systems):
             ham = A.HamiltonianOperations(s.h, s.Likn, s.PsiT, s.E0, s.dt)
             ham.set_params(hybrid=int(hybrid))
             ham.set_precision(precision)
+            if tile_cache is not None:
+                self.tilings.append(ham.autotune(W, tile_cache))
             prop = A.Propagator(ham)
             xi = torch.from_numpy(synth.fields(s.seed, steps, W, s.K)).to("cuda")
             u = np.random.default_rng([s.seed, 31]).random(steps)

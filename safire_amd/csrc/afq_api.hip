@@ -5,7 +5,11 @@
 
 #include <cmath>
 #include <complex>
+#include <cctype>
 #include <cstring>
+#include <fstream>
+#include <sstream>
+#include <string>
 #include <map>
 #include <mutex>
 #include <utility>
@@ -305,6 +309,7 @@ int afq_ham_set_precision(afq_ham H, int precision) {
     AFQ_CHECK(precision != AFQ_INT8_SPLIT || H->Lnak_q != nullptr, AFQ_EUNSUPPORTED,
               "INT8_SPLIT exchange needs N <= 32 and M <= 128");
     H->precision = precision;
+    H->cfg_vbias = H->cfg_vhs = 0;   // tilings are tuned per precision (afq_autotune)
   });
 }
 
@@ -353,6 +358,129 @@ int afq_ham_stats(afq_ham H, long long* eloc3, long long* weight2, long long* po
       for (int i = 0; i < 2; ++i) weight2[i] = (long long)st[3 + i];
     if (popctl2)
       for (int i = 0; i < 2; ++i) popctl2[i] = (long long)st[5 + i];
+  });
+}
+
+// ------------------------------------------------------------------ autotuner
+}  // extern "C"
+
+namespace {
+
+// the cache: a JSON object of "key": tiling entries (string keys, integer values)
+std::map<std::string, int> read_tile_cache(const char* path) {
+  std::map<std::string, int> out;
+  if (path == nullptr) return out;
+  std::ifstream f(path);
+  std::stringstream buf;
+  buf << f.rdbuf();
+  const std::string text = buf.str();
+  size_t pos = 0;
+  while (true) {
+    const size_t a = text.find('"', pos);
+    if (a == std::string::npos) break;
+    const size_t b = text.find('"', a + 1);
+    if (b == std::string::npos) break;
+    const size_t c = text.find(':', b);
+    if (c == std::string::npos) break;
+    size_t e = c + 1;
+    while (e < text.size() && (text[e] == ' ' || text[e] == '\t' || text[e] == '\n')) ++e;
+    size_t f2 = e;
+    while (f2 < text.size() && (std::isdigit((unsigned char)text[f2]) || text[f2] == '-')) ++f2;
+    if (f2 > e) {
+      try {
+        out[text.substr(a + 1, b - a - 1)] = std::stoi(text.substr(e, f2 - e));
+      } catch (...) {
+      }
+    }
+    pos = f2 > b ? f2 : b + 1;
+  }
+  return out;
+}
+
+void write_tile_cache(const char* path, const std::map<std::string, int>& m) {
+  if (path == nullptr) return;
+  std::ofstream f(path);
+  AFQ_CHECK(f.good(), AFQ_EINVAL, std::string("afq_autotune: cannot write ") + path);
+  f << "{\n";
+  size_t i = 0;
+  for (const auto& kv : m) f << "  \"" << kv.first << "\": " << kv.second << (++i < m.size() ? ",\n" : "\n");
+  f << "}\n";
+}
+
+// mean time (ms) of `reps` launches of fn on stream s
+template <typename F>
+float time_launches(hipStream_t s, int reps, F&& fn) {
+  hipEvent_t a, b;
+  AFQ_HIP(hipEventCreate(&a));
+  AFQ_HIP(hipEventCreate(&b));
+  float ms = 0.0f;
+  try {
+    fn();
+    fn();
+    AFQ_HIP(hipEventRecord(a, s));
+    for (int r = 0; r < reps; ++r) fn();
+    AFQ_HIP(hipEventRecord(b, s));
+    AFQ_HIP(hipEventSynchronize(b));
+    AFQ_HIP(hipEventElapsedTime(&ms, a, b));
+  } catch (...) {
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    throw;
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return ms / reps;
+}
+
+}  // namespace
+
+extern "C" {
+
+int afq_autotune(afq_ham H, int W, const char* cache_path, int* cfg_out, void* stream) {
+  return guard([&] {
+    check_ham(H);
+    AFQ_CHECK(W > 0 && W <= H->ws.cap, AFQ_ECAPACITY,
+              "afq_autotune: W must be positive and within afq_ham_reserve's capacity");
+    hipStream_t s = S(stream);
+    const int M = H->M, N = H->N, K = H->K, p = H->precision;
+    const long NM = (long)N * M;
+    Workspace& ws = H->ws;
+    auto cache = read_tile_cache(cache_path);
+    auto key = [&](const char* what, long a, long b) {
+      std::ostringstream k;
+      k << what << "/p" << p << "/W" << W << "/" << a << "x" << b;
+      return k.str();
+    };
+    auto tune = [&](const std::string& k, bool bc, auto&& launch) {
+      const auto it = cache.find(k);
+      if (it != cache.end() && it->second >= 0 && it->second < gemm_tile_configs(p, bc))
+        return it->second;
+      int best = 0;
+      float tbest = 0.0f;
+      for (int c = 0; c < gemm_tile_configs(p, bc); ++c) {
+        const float t = time_launches(s, 10, [&] { launch(c); });
+        if (c == 0 || t < tbest) {
+          best = c;
+          tbest = t;
+        }
+      }
+      cache[k] = best;
+      return best;
+    };
+    H->cfg_vbias = tune(key("vbias", K, NM), true, [&](int c) {
+      launch_gemm_nt_z(s, p, W, K, (int)NM, std::sqrt(H->dt) * scl(H), ws.G, NM, L_for(H), NM,
+                       true, ws.X, K, ws.split, ws.split_elems, c);
+    });
+    H->cfg_vhs = tune(key("vhs", (long)M * M, K), false, [&](int c) {
+      launch_gemm_nt_z(s, p, W, M * M, K, std::sqrt(H->dt), ws.X, K, Likn_for(H), K, false, ws.V,
+                       (long)M * M, nullptr, 0, c);
+    });
+    AFQ_HIP(hipStreamSynchronize(s));
+    write_tile_cache(cache_path, cache);
+    if (cfg_out) {
+      cfg_out[0] = H->cfg_vbias;
+      cfg_out[1] = H->cfg_vhs;
+    }
   });
 }
 
@@ -425,7 +553,8 @@ int afq_vbias(afq_ham H, int W, const double* G, double* v, void* stream) {
     const long NM = (long)H->N * H->M;
     AFQ_CHECK(W <= H->ws.cap, AFQ_ECAPACITY, "more walkers than reserved");
     launch_gemm_nt_z(S(stream), H->precision, W, H->K, (int)NM, std::sqrt(H->dt) * scl(H), G,
-                     NM, L_for(H), NM, true, v, H->K, H->ws.split, H->ws.split_elems);
+                     NM, L_for(H), NM, true, v, H->K, H->ws.split, H->ws.split_elems,
+                     H->cfg_vbias);
   });
 }
 
@@ -435,7 +564,7 @@ int afq_vhs(afq_ham H, int W, const double* X, double* V, void* stream) {
     AFQ_CHECK(W >= 0, AFQ_EBADSHAPE, "negative walker count");
     const int MM = H->M * H->M;
     launch_gemm_nt_z(S(stream), H->precision, W, MM, H->K, std::sqrt(H->dt), X, H->K,
-                     Likn_for(H), H->K, false, V, MM);
+                     Likn_for(H), H->K, false, V, MM, nullptr, 0, H->cfg_vhs);
   });
 }
 
@@ -511,7 +640,7 @@ int afq_step(afq_ham H, int W, double* wbuf, long ldw, const double* xi, double 
     {
       RegionScope r(tm, s, R_VBIAS);
       launch_gemm_nt_z(s, H->precision, W, K, (int)NM, std::sqrt(H->dt) * scl(H), ws.G, NM,
-                       L_for(H), NM, true, ws.X, K, ws.split, ws.split_elems);
+                       L_for(H), NM, true, ws.X, K, ws.split, ws.split_elems, H->cfg_vbias);
     }
     // 2. X, HW, mean-field factor     (assemble_X)
     {
@@ -523,7 +652,7 @@ int afq_step(afq_ham H, int W, double* wbuf, long ldw, const double* xi, double 
     {
       RegionScope r(tm, s, R_VHS);
       launch_gemm_nt_z(s, H->precision, W, M * M, K, std::sqrt(H->dt), ws.X, K, Likn_for(H),
-                       K, false, ws.V, (long)M * M);
+                       K, false, ws.V, (long)M * M, nullptr, 0, H->cfg_vhs);
     }
     {
       RegionScope r(tm, s, R_PROPAGATE);

@@ -155,6 +155,7 @@ struct afq_ham_s {
   double dt = 0.005, E0 = 0.0;
   double ci0_re = 1.0, ci0_im = 0.0;
   int precision = AFQ_FP64;
+  int cfg_vbias = 0, cfg_vhs = 0;   // GEMM tilings (afq_autotune)
   afq_step_params params;
   int device = 0;
   // device tensors (built on the GPU at create)
@@ -194,9 +195,12 @@ void launch_green(hipStream_t s, int nb, int M, int N, const double* PsiT, const
 // split_ws (c128, split_ws_elems complex numbers) enables a deterministic split of k; NULL = none.
 constexpr int kMaxSplits = 8;
 int gemm_splits(int m, int n, int k);
+// tilings launch_gemm_nt_z can run for a precision and kind of B (the cfg argument: 0 .. n-1)
+int gemm_tile_configs(int precision, bool b_complex);
 void launch_gemm_nt_z(hipStream_t s, int precision, int m, int n, int k, double alpha,
                       const double* A, long lda, const void* B, long ldb, bool b_complex,
-                      double* C, long ldc, double* split_ws = nullptr, long split_ws_elems = 0);
+                      double* C, long ldc, double* split_ws = nullptr, long split_ws_elems = 0,
+                      int cfg = 0);
 // k_energy.hip
 int device_cus();                        // compute units of the current device (cached)
 int exx_nchunks(int W, int N, int K);   // chunks of Cholesky vectors the exchange kernel uses

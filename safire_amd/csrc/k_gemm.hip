@@ -196,9 +196,16 @@ int gemm_splits(int m, int n, int k) {
   return std::max(1, std::min(s, kMaxSplits));
 }
 
+int gemm_tile_configs(int precision, bool b_complex) {
+  // FP32: 16- or 32-wide k slices, and for real B also 128-wide column tiles; FP64: one tiling
+  // (a 64 x 128 FP64 tile or 32-wide FP64 slices would not fit the register or LDS budget)
+  if (precision == AFQ_FP64) return 1;
+  return b_complex ? 2 : 4;
+}
+
 void launch_gemm_nt_z(hipStream_t s, int precision, int m, int n, int k, double alpha,
                       const double* A, long lda, const void* B, long ldb, bool b_complex,
-                      double* C, long ldc, double* split_ws, long split_ws_elems) {
+                      double* C, long ldc, double* split_ws, long split_ws_elems, int cfg) {
   if (m <= 0 || n <= 0) return;
   int splits = split_ws ? gemm_splits(m, n, k) : 1;
   if ((long)splits * m * n > split_ws_elems) splits = 1;
@@ -210,7 +217,10 @@ void launch_gemm_nt_z(hipStream_t s, int precision, int m, int n, int k, double 
   // workgroups, two per CU; in the FP32 form 64 x 128 tiles with 32-wide slices (one workgroup
   // per CU) measured 63 us against 56 us, and the FP32 vbias 43 us with 32-wide slices against
   // 36 us
-  const int tn = 64;
+  // cfg (FP32 only; < 0 or out of range: 0): bit 0 = 32-wide k slices, bit 1 = 128-wide column
+  // tiles (real B). Every tiling sums k in the same order, so the choice changes no result.
+  if (cfg < 0 || cfg >= gemm_tile_configs(precision, b_complex)) cfg = 0;
+  const int tn = (cfg & 2) ? 128 : 64;
   // real B (vHS, the large Cholesky tensor): column tiles fastest (XCD-local B)
   const int nmajor = b_complex ? 0 : 1;
   const unsigned mt = (m + kTile - 1) / kTile, nt = (n + tn - 1) / tn;
@@ -227,12 +237,33 @@ void launch_gemm_nt_z(hipStream_t s, int precision, int m, int n, int k, double 
       hipLaunchKernelGGL((k_gemm_nt<double, false, 16, 64>), grid, dim3(256), 0, s, m, n, k,
                          kchunk, a, A, lda, (const double*)B, ldb, out, ldo, sstride, nmajor);
   } else {
-    if (b_complex)
-      hipLaunchKernelGGL((k_gemm_nt<float, true, 16, 64>), grid, dim3(256), 0, s, m, n, k, kchunk,
-                         a, A, lda, (const float*)B, ldb, out, ldo, sstride, nmajor);
-    else
-      hipLaunchKernelGGL((k_gemm_nt<float, false, 16, 64>), grid, dim3(256), 0, s, m, n, k,
-                         kchunk, a, A, lda, (const float*)B, ldb, out, ldo, sstride, nmajor);
+    const float* Bf = (const float*)B;
+    if (b_complex) {
+      if (cfg & 1)
+        hipLaunchKernelGGL((k_gemm_nt<float, true, 32, 64>), grid, dim3(256), 0, s, m, n, k,
+                           kchunk, a, A, lda, Bf, ldb, out, ldo, sstride, nmajor);
+      else
+        hipLaunchKernelGGL((k_gemm_nt<float, true, 16, 64>), grid, dim3(256), 0, s, m, n, k,
+                           kchunk, a, A, lda, Bf, ldb, out, ldo, sstride, nmajor);
+    } else {
+      switch (cfg) {
+        case 1:
+          hipLaunchKernelGGL((k_gemm_nt<float, false, 32, 64>), grid, dim3(256), 0, s, m, n, k,
+                             kchunk, a, A, lda, Bf, ldb, out, ldo, sstride, nmajor);
+          break;
+        case 2:
+          hipLaunchKernelGGL((k_gemm_nt<float, false, 16, 128>), grid, dim3(256), 0, s, m, n, k,
+                             kchunk, a, A, lda, Bf, ldb, out, ldo, sstride, nmajor);
+          break;
+        case 3:
+          hipLaunchKernelGGL((k_gemm_nt<float, false, 32, 128>), grid, dim3(256), 0, s, m, n, k,
+                             kchunk, a, A, lda, Bf, ldb, out, ldo, sstride, nmajor);
+          break;
+        default:
+          hipLaunchKernelGGL((k_gemm_nt<float, false, 16, 64>), grid, dim3(256), 0, s, m, n, k,
+                             kchunk, a, A, lda, Bf, ldb, out, ldo, sstride, nmajor);
+      }
+    }
   }
   AFQ_HIP(hipGetLastError());
   if (splits > 1) {
