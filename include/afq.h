@@ -332,7 +332,15 @@ int afq_ph_population_control(afq_ph ph, int W, double* d_wbuf, long ldw, double
                               void* stream);
 /* Unique excitations per spin (the reference counted as index 0) and non-empty ranks. */
 int afq_ph_info(afq_ph ph, int* nunique, int* nshells);
-/* PHMSD::energy / energy_alg1 (PHMSD.cpp:377-544): d_E [nw][3] c128 (E1, EXX, EJ of the
+/* PHMSD's energy algorithm (PHMSDEnergyAlgorithm, parameters.hpp:32-37; chosen at PHMSD.hpp:98
+ * from the "algorithm" option, dispatched at PHMSD.cpp:89-96): AFQ_PH_ENERGY_WOODBURY (the
+ * default, energy_alg1, PHMSD.cpp:377-544) or AFQ_PH_ENERGY_REFERENCE (energy_alg0,
+ * PHMSD.cpp:122-370: every unique configuration's own G and energy; at most 32 electrons per
+ * spin). Used by afq_ph_energy, afq_ph_seed_properties and the local-energy form of afq_ph_step. */
+#define AFQ_PH_ENERGY_REFERENCE 0
+#define AFQ_PH_ENERGY_WOODBURY 1
+int afq_ph_set_energy_algorithm(afq_ph ph, int algorithm);
+/* PHMSD::energy (PHMSD.cpp:89-96, energy_alg1 or energy_alg0): d_E [nw][3] c128 (E1, EXX, EJ of the
  * multi-determinant local energy) and d_ov [nw] c128 = log sum_D conj(c_D) <D|phi_w>. Walkers:
  * rows of ldw c128 holding Phi_up [nmo][nup] then Phi_dn [nmo][ndn] (the COLLINEAR WalkerSet). */
 int afq_ph_energy(afq_ph ph, int nw, const double* d_wbuf, long ldw, double* d_E, double* d_ov,

@@ -9,6 +9,8 @@ composes them, and restates in numpy the parts that are plain array algebra:
   indices, the CI couplings conj(ci) in CSR form, both directions).
 * ph_reference_energy: Real3IndexFactorization.hpp:212-233 and ph_ref_energy_impl 598-696.
 * energy_alg1: PHMSD.cpp:377-544 (COLLINEAR walkers).
+* energy_alg0: PHMSD.cpp:122-370 (COLLINEAR walkers), with Real3IndexFactorization::energy
+  (hpp:176-210, 476-596) for each configuration.
 """
 from __future__ import annotations
 
@@ -192,6 +194,70 @@ def energy_alg1(struct, A, Lna, haj, E0, Phi_up, Phi_dn):
     Ed[2] += np.einsum("dwn,dwn->w", KEr[1:nd], KEd[1:nd])
     Eacc += Ed.T
     return Eacc / Ov[:, None], np.log(Ov) + log_ov
+
+
+def _config_energy(G, Lna, haj, E0):
+    """Real3IndexFactorization::energy (hpp:176-210) with energy_impl (476-596), COLLINEAR, over
+    G [nw][nact][nmo] zero outside the configuration's rows: (E [nw][3], EJn [nw][nchol])."""
+    E = np.zeros((G.shape[0], 3), np.complex128)
+    E[:, 0] = E0 + np.einsum("wak,ak->w", G, haj, optimize=True)
+    T = np.einsum("wbk,nak->wbna", G, Lna, optimize=True)          # Twbna
+    E[:, 1] = -0.5 * np.einsum("wanb,wbna->w", T, T, optimize=True)
+    EJn = np.einsum("wak,nak->wn", G, Lna, optimize=True)
+    E[:, 2] = 0.5 * np.einsum("wn,wn->w", EJn, EJn)
+    return E, EJn
+
+
+def energy_alg0(struct, A, Lna, haj, E0, Phi_up, Phi_dn):
+    """PHMSD::energy_alg0 (PHMSD.cpp:122-370) for COLLINEAR walkers, arguments as energy_alg1:
+    for each spin and unique configuration (get_configuration), the configuration's compact G and
+    log overlap (MixedDensityMatrixFromConfiguration, density_matrix.hpp:726-775: the same getrf,
+    log_determinant_from_getrf and getri_or_zero as ph_woodbury over the configuration's rows)
+    scattered into the active rows, its energy and Cholesky trace (Real3IndexFactorization::
+    energy), the overlap ratio exp(log ov_n - log ov_0); the up spin's traces scaled by their
+    ratios and carried to the down configurations through the couplings (csrmm); the CI-weighted
+    sums, the opposite-spin Coulomb term, division by the overlap. Returns (E [nw][3], log ov)."""
+    Phis = [Phi_up, Phi_dn]
+    nw = Phi_up.shape[0]
+    nchol = Lna[0].shape[0]
+    nact, nmo = A[0].shape
+    nex = struct["nunique"]
+    maxc = max(nex)
+    log_ov = np.zeros((2, nw), np.complex128)
+    Ovmsd = np.zeros((2, maxc, nw), np.complex128)
+    Emsd = np.zeros((2, maxc, nw, 3), np.complex128)
+    KEright = np.zeros((maxc, nw, nchol), np.complex128)
+    opSpinEJ = np.zeros(nw, np.complex128)
+    for s in range(2):
+        for nd, confg in enumerate(configurations(struct, s)):
+            Gw, ov, _ = O.ph_woodbury(A[s], confg, Phis[s])
+            if nd == 0:
+                log_ov[s] = ov
+                Ovmsd[s, 0] = 1.0
+            else:
+                Ovmsd[s, nd] = np.exp(ov - log_ov[s])
+            G = np.zeros((nw, nact, nmo), np.complex128)
+            G[:, confg, :] = Gw
+            Ec, EJn = _config_energy(G, Lna[s], haj[s], E0 if s == 0 else 0.0)
+            Emsd[s, nd] = Ec
+            if s == 0:
+                KEright[nd] = EJn
+            else:
+                opSpinEJ += np.einsum("wn,wn->w", KEright[nd], EJn) * Ovmsd[1, nd]
+        if s == 0:
+            KEright[:nex[0]] *= Ovmsd[0, :nex[0], :, None]
+            Tdn = KEright[:nex[0]].copy()
+            KEright[:nex[1]] = np.einsum("du,uwn->dwn", struct["C"][1], Tdn, optimize=True)
+    wgt = struct["C"][0] @ Ovmsd[1, :nex[1]]
+    Ov = np.einsum("uw,uw->w", Ovmsd[0, :nex[0]], wgt)
+    wgt = wgt * Ovmsd[0, :nex[0]]
+    Emsd[0, :nex[0]] *= wgt[:, :, None]
+    wgt = (struct["C"][1] @ Ovmsd[0, :nex[0]]) * Ovmsd[1, :nex[1]]
+    Emsd[1, :nex[1]] *= wgt[:, :, None]
+    E = Emsd.sum(axis=(0, 1))
+    E[:, 2] += opSpinEJ
+    E /= Ov[:, None]
+    return E, np.log(Ov) + log_ov[1] + log_ov[0]
 
 
 def mixed_density_matrix(struct, A, Phi_up, Phi_dn, compact=True):

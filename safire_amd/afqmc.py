@@ -87,6 +87,7 @@ SIGNATURES = {
     "afq_ph_create": (_i, [_i, _i, _i, _i, _i, _vp, _vp, _i, _vp, _vp, _vp, _d, _i, _vp]),
     "afq_ph_destroy": (None, [_vp]),
     "afq_ph_info": (_i, [_vp, _vp, _vp]),
+    "afq_ph_set_energy_algorithm": (_i, [_vp, _i]),
     "afq_ph_energy": (_i, [_vp, _i, _vp, _l, _vp, _vp, _vp]),
     "afq_ph_mixed_density_matrix": (_i, [_vp, _i, _vp, _l, _i, _vp, _vp, _vp]),
     "afq_ph_vbias": (_i, [_vp, _i, _vp, _l, _d, _vp, _vp]),
@@ -292,6 +293,7 @@ class PHMSD:
         nu, ns = (C.c_int * 2)(), (C.c_int * 2)()
         check(lib().afq_ph_info(self.handle, nu, ns))
         self.nunique = (nu[0], nu[1])
+        self.energy_algorithm = "woodbury"
 
     def close(self):
         if getattr(self, "handle", None) is not None and self.handle.value:
@@ -304,8 +306,19 @@ class PHMSD:
         except Exception:
             pass
 
+    ENERGY_ALGORITHMS = {"reference": 0, "woodbury": 1}
+
+    def set_energy_algorithm(self, algorithm):
+        """The "algorithm" option of PHMSD (PHMSDEnergyAlgorithm, parameters.hpp:32-37):
+        "woodbury" (the default, energy_alg1) or "reference" (energy_alg0, PHMSD.cpp:122-370)."""
+        if algorithm not in self.ENERGY_ALGORITHMS:
+            raise ValueError(f"unknown PHMSD energy algorithm {algorithm!r}")
+        check(lib().afq_ph_set_energy_algorithm(self.handle, self.ENERGY_ALGORITHMS[algorithm]))
+        self.energy_algorithm = algorithm
+
     def energy(self, wbuf, E=None, ov=None, stream=None):
-        """PHMSD::energy (energy_alg1, PHMSD.cpp:377-544) of the walkers in wbuf [W][ldw]
+        """PHMSD::energy (PHMSD.cpp:89-96: energy_alg1, PHMSD.cpp:377-544, or energy_alg0,
+        PHMSD.cpp:122-370, as set_energy_algorithm chose) of the walkers in wbuf [W][ldw]
         (c128, Phi_up [nmo][nup] then Phi_dn [nmo][ndn] per row): (E [W][3], log overlap [W])."""
         torch = _torch()
         W = wbuf.shape[0]

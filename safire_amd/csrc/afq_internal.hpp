@@ -297,6 +297,16 @@ void launch_ph_full_g(hipStream_t s, int nw, int nact, int nmo, const double* A0
                       const double* A1, const double* Gc, double* Gf);
 void launch_ph_log_overlap(hipStream_t s, int nw, const double* Ov, const double* log_ov,
                            double* out);
+void launch_ph0_products(hipStream_t s, int nw, int nmo, int nel, int nact, int nchol,
+                         const double* Phi, long ldw, const double* A, const double* haj,
+                         const double* Lna, double* Phit, double* TAB, double* Hp, double* P);
+void launch_ph0_config(hipStream_t s, int nw, int nc, int nel, int nact, int nchol,
+                       const int* conf, const double* TAB, const double* Hp, const double* P,
+                       double E0, double* ld, double* Emsd, double* KE, const double* KEr);
+void launch_ph0_ratios(hipStream_t s, int nw, int nc, double* r, double* log_ov);
+void launch_ph0_assemble(hipStream_t s, int nw, int nu, int nd, const double* wgt0,
+                         const double* Em0, const double* wgt1, const double* Em1,
+                         const double* r1, double* acc);
 void launch_ph_accumulate_R(hipStream_t s, int nw, int ndet, int nex, int nact, int nelec,
                             const int* iexcit, const int* refc, const double* Rc,
                             const double* ov, const double* wgt, double* R);

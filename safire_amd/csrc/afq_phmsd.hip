@@ -47,6 +47,7 @@ struct Shell {
 struct Spin {
   int nel = 0, nunique = 1;
   std::vector<std::vector<int>> confs;          // host: occupation of every unique excitation
+  int* dconf = nullptr;            // [nunique][nel]: confs on the GPU
   std::vector<int> hcp, hci;                    // host copy of the couplings (CSR)
   std::vector<std::complex<double>> hcv;
   std::vector<std::complex<double>> hA;         // host copy of the orbital rows [nact][nmo]
@@ -146,6 +147,7 @@ zc host_logdet_inverse(std::vector<zc> a, int n, std::vector<zc>* inv) {
 
 struct afq_ph_t {
   int nmo = 0, nchol = 0, nact = 0, wblock = 0, device = 0;
+  int energy_alg = AFQ_PH_ENERGY_WOODBURY;
   double E0 = 0.0;
   afq::Spin spin[2];
   double* Lstack = nullptr;        // c128 [nchol][2 nact][nmo]: both spins' Lna per vector
@@ -170,6 +172,7 @@ struct afq_ph_t {
   double *Twina = nullptr, *Swia = nullptr, *KEa = nullptr, *KEb = nullptr, *KEl = nullptr,
          *R = nullptr;
   double *Racc = nullptr, *Gblk = nullptr, *split = nullptr, *p2ws = nullptr;
+  double* Emsd = nullptr;          // c128 [2][max nunique][wblock][4] (energy_alg0)
   long split_elems = 0, p2ws_doubles = 0;
   ~afq_ph_t() {
     for (void* p : owned) (void)hipFree(p);
@@ -245,8 +248,49 @@ void ph_density_block(afq_ph_t* ph, hipStream_t s, int nb, const double* const P
 }
 
 // PHMSD::energy_alg1 over all walkers, in walker blocks
+// PHMSD::energy_alg0 (PHMSD.cpp:122-370), COLLINEAR walkers, in walker blocks: each unique
+// configuration's own G and single-determinant energy, its overlap relative to the spin's
+// reference configuration, the up spin's Cholesky traces carried to the down configurations
+// through the couplings (KEr[d] = sum_u C[d][u] ratio_up[u] KE_up[u]), then the CI-weighted sums.
+void ph_energy_alg0(afq_ph_t* ph, hipStream_t s, int nw, const double* d_wbuf, long ldw,
+                    double* d_E, double* d_ov) {
+  const int nmo = ph->nmo, na = ph->nact, K = ph->nchol;
+  Spin* S = ph->spin;
+  const int nu = S[0].nunique, nd = S[1].nunique;
+  const long em1 = 8L * std::max(nu, nd) * ph->wblock;   // doubles per spin of Emsd
+  for (int w0 = 0; w0 < nw; w0 += ph->wblock) {
+    const int nb = std::min(ph->wblock, nw - w0);
+    const double* Phi[2] = {d_wbuf + 2 * (long)w0 * ldw,
+                            d_wbuf + 2 * ((long)w0 * ldw + (long)nmo * S[0].nel)};
+    launch_ph_fill(s, nb, 0.0, 0.0, ph->log_ov);
+    for (int sp = 0; sp < 2; ++sp) {
+      launch_ph0_products(s, nb, nmo, S[sp].nel, na, K, Phi[sp], ldw, S[sp].A, S[sp].haj,
+                          S[sp].Lna, ph->G[sp], ph->QQ0[sp], ph->Swia, ph->Twina);
+      launch_ph0_config(s, nb, S[sp].nunique, S[sp].nel, na, K, S[sp].dconf, ph->QQ0[sp],
+                        ph->Swia, ph->Twina, sp == 0 ? ph->E0 : 0.0, ph->ratios[sp],
+                        ph->Emsd + sp * em1, sp == 0 ? ph->KEa : nullptr,
+                        sp == 1 ? ph->KEb : nullptr);
+      launch_ph0_ratios(s, nb, S[sp].nunique, ph->ratios[sp], ph->log_ov);
+      if (sp == 0)
+        launch_ph_couple_ke(s, nb, K, nd, S[1].cp, S[1].ci, S[1].cv, ph->ratios[0], ph->KEa,
+                            nullptr, ph->KEb);
+    }
+    launch_ph_spin_weights(s, nb, nu, S[0].cp, S[0].ci, S[0].cv, ph->ratios[0], ph->ratios[1],
+                           ph->wgt[0], ph->Ov);
+    launch_ph_spin_weights(s, nb, nd, S[1].cp, S[1].ci, S[1].cv, ph->ratios[1], ph->ratios[0],
+                           ph->wgt[1], ph->Ov2);
+    launch_ph0_assemble(s, nb, nu, nd, ph->wgt[0], ph->Emsd, ph->wgt[1], ph->Emsd + em1,
+                        ph->ratios[1], ph->acc);
+    launch_ph_finalize(s, nb, ph->acc, ph->Ov, ph->log_ov, d_E + 6L * w0, d_ov + 2L * w0);
+  }
+}
+
 void ph_energy_all(afq_ph_t* ph, hipStream_t s, int nw, const double* d_wbuf, long ldw,
                    double* d_E, double* d_ov) {
+    if (ph->energy_alg == AFQ_PH_ENERGY_REFERENCE) {
+      ph_energy_alg0(ph, s, nw, d_wbuf, ldw, d_E, d_ov);
+      return;
+    }
     const int nmo = ph->nmo, na = ph->nact, K = ph->nchol;
     Spin* S = ph->spin;
     const int nu = S[0].nunique, nd = S[1].nunique;
@@ -576,6 +620,9 @@ int afq_ph_create(int nmo, int nup, int ndn, int nchol, int ndet, const int* occ
       std::vector<int> rc(nel[s]);
       for (int k = 0; k < nel[s]; ++k) rc[k] = ref[s][k] - s * nmo;
       S.refc = dev_copy(own, rc.data(), rc.size());
+      std::vector<int> flat;
+      for (const auto& c : S.confs) flat.insert(flat.end(), c.begin(), c.end());
+      S.dconf = dev_copy(own, flat.data(), flat.size());
     }
     AFQ_CHECK(max_nex <= 8, AFQ_EUNSUPPORTED, "afq_ph_create: excitations of rank > 8");
     // couplings: spin 0 rows u, columns d; spin 1 rows d, columns u; entries sorted by column
@@ -664,6 +711,7 @@ int afq_ph_create(int nmo, int nup, int ndn, int nchol, int ndet, const int* occ
     P->R = dev_alloc<double>(own, 2 * nb * dblk * max_nex * na);
     P->Racc = dev_alloc<double>(own, 2 * nb * maxel * na);
     P->Gblk = dev_alloc<double>(own, 2 * nb * 2 * na * nmo);
+    P->Emsd = dev_alloc<double>(own, 16L * maxu * nb);
     P->p2ws_doubles = ph_2body_workspace((int)nb, (int)na, maxel, (int)K);
     if (P->p2ws_doubles > 0) P->p2ws = dev_alloc<double>(own, P->p2ws_doubles);
     P->split_elems = (long)kMaxSplits * nb * K;
@@ -681,6 +729,19 @@ int afq_ph_info(afq_ph ph, int* nunique, int* nshells) {
       if (nunique) nunique[s] = ph->spin[s].nunique;
       if (nshells) nshells[s] = (int)ph->spin[s].shells.size();
     }
+  });
+}
+
+int afq_ph_set_energy_algorithm(afq_ph ph, int algorithm) {
+  return ph_guard([&] {
+    AFQ_CHECK(ph != nullptr, AFQ_EINVAL, "afq_ph_set_energy_algorithm: null handle");
+    AFQ_CHECK(algorithm == AFQ_PH_ENERGY_REFERENCE || algorithm == AFQ_PH_ENERGY_WOODBURY,
+              AFQ_EINVAL, "afq_ph_set_energy_algorithm: unknown algorithm");
+    if (algorithm == AFQ_PH_ENERGY_REFERENCE)
+      AFQ_CHECK(ph->spin[0].nel <= 32 && ph->spin[1].nel <= 32, AFQ_EUNSUPPORTED,
+                "afq_ph_set_energy_algorithm: the reference algorithm takes at most 32 "
+                "electrons per spin");
+    ph->energy_alg = algorithm;
   });
 }
 

@@ -16,6 +16,8 @@
 // it addresses Twina; a determinant with an out-of-range index makes the walker's results NaN.
 #include "afq_internal.hpp"
 
+#include <cfloat>
+
 namespace afq {
 
 namespace {
@@ -629,7 +631,7 @@ __global__ __launch_bounds__(256) void k_ph_couple_ke(int nw, int nchol, const i
                                                       cd* __restrict__ KEout) {
   const int d = blockIdx.y, w = blockIdx.x;
   const int j0 = cp[d], j1 = cp[d + 1];
-  const cd ro = rout[(long)d * nw + w];
+  const cd ro = rout ? rout[(long)d * nw + w] : cmk(1.0, 0.0);
   for (int n = threadIdx.x; n < nchol; n += kPhThreads) {
     cd acc = cmk(0.0, 0.0);
     for (int j = j0; j < j1; ++j) {
@@ -723,6 +725,222 @@ __global__ void k_ph_log_overlap(int nw, const cd* __restrict__ Ov, const cd* __
 __global__ void k_ph_fill(long n, cd v, cd* __restrict__ x) {
   const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e < n) x[e] = v;
+}
+
+// ------------------------------------------------------------------------------------------------
+// PHMSD::energy_alg0 (PHMSD.cpp:122-370), COLLINEAR walkers. Every unique configuration c of a spin
+// gets its own compact G (det_ops::MixedDensityMatrixFromConfiguration, density_matrix.hpp:726-775)
+// and single-determinant energy over the active rows it occupies (Real3IndexFactorization::energy,
+// hpp:176-210 and energy_impl 476-596). With the walker products, formed once per walker by GEMMs,
+//   TAB[w][j][a] = sum_k Phi_w[k][j] A[a][k]     Hp[w][j][a] = sum_k Phi_w[k][j] haj[a][k]
+//   P[w][j][n][a] = sum_k Phi_w[k][j] Lna[n][a][k]
+// the configuration's matrix is T_c[i][j] = TAB[w][j][conf_i] and G_c[a][k] = sum_j Tinv[j][a]
+// Phi_w[k][j], so
+//   E1 = E0 + sum_{a,j} Tinv[j][a] Hp[w][j][conf_a]
+//   Y^n[a][b] = sum_k Lna[n][conf_a][k] G_c[b][k] = sum_j P[w][j][n][conf_a] Tinv[j][b]
+//   KE[n] = sum_a Y^n[a][a]    EXX = -1/2 sum_n sum_{a,b} Y^n[a][b] Y^n[b][a]    EJ = 1/2 sum_n KE[n]^2
+// and the work per configuration is nel^3 per Cholesky vector rather than nel^2 nmo.
+
+// Phit[w][j][k] = Phi_w[k][j]
+__global__ void k_ph0_transpose(int nmo, int nel, const cd* __restrict__ Phi, long ldw,
+                                cd* __restrict__ Phit) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int w = blockIdx.y;
+  if (e >= (long)nel * nmo) return;
+  const int j = (int)(e / nmo), k = (int)(e % nmo);
+  Phit[(long)w * nel * nmo + e] = Phi[(long)w * ldw + (long)k * nel + j];
+}
+
+// One workgroup per (configuration c, walker w), c fastest, so that the workgroups in flight share
+// a walker's P in L2. Gauss-Jordan on [T_c | I] with getrf's pivots (first max of |re|+|im| in
+// the column among the rows not yet eliminated; the rows at and below step k see the same
+// updates as in getrf), log det = sum_k log(+-U_kk + DBL_MIN), the sign flipped where rows were
+// swapped, imaginary part wrapped to [-pi, pi] (log_determinant_from_getrf). A configuration with
+// an exactly zero pivot gets a zero inverse (getri_or_zero) and log det = -inf, i.e. ratio 0.
+// Outputs: ld[c][w]; Emsd[c][w][4] = (E1, EXX, EJ of the same spin, sum_n KEr[c][w][n] KE[n]);
+// KE[c][w][n] when KE is given. Fixed-order sums throughout.
+__global__ __launch_bounds__(kPhThreads) void k_ph0_config(
+    int nw, int nc, int nel, int nact, int nchol, int nbv, const int* __restrict__ conf,
+    const cd* __restrict__ TAB, const cd* __restrict__ Hp, const cd* __restrict__ P, cd E0,
+    cd* __restrict__ ld, cd* __restrict__ Emsd, cd* __restrict__ KE, const cd* __restrict__ KEr) {
+  extern __shared__ cd sm[];
+  __shared__ cd red[kPhThreads / 64];
+  __shared__ int sConf[kPhMaxElec];
+  __shared__ int sPiv, sSing;
+  const int t = threadIdx.x;
+  const int c = blockIdx.x % nc, w = blockIdx.x / nc;
+  const int n2 = 2 * nel;
+  cd* sM = sm;                       // [nel][2 nel]: [T | I] -> [I | T^-1]
+  cd* sRow = sM + nel * n2;          // [2 nel] the scaled pivot row
+  cd* sF = sRow + n2;                // [nel] the pivot column before the update
+  cd* sPc = sF + nel;                // [nbv][nel][nel]: P[w][j][n0 + v][conf_a] as [v][a][j]
+  cd* sDg = sPc + (long)nbv * nel * nel;   // [nbv][nel]: Y^n[a][a]
+  if (t < nel) sConf[t] = min(max(conf[(long)c * nel + t], 0), nact - 1);
+  __syncthreads();
+  const cd* TABw = TAB + (long)w * nel * nact;
+  for (int e = t; e < nel * n2; e += kPhThreads) {
+    const int i = e / n2, j = e % n2;
+    sM[e] = j < nel ? TABw[(long)j * nact + sConf[i]] : cmk(j - nel == i ? 1.0 : 0.0, 0.0);
+  }
+  __syncthreads();
+  cd logd = cmk(0.0, 0.0);           // thread 0
+  bool sing = false;                 // uniform: read from LDS after a barrier
+  for (int k = 0; k < nel; ++k) {
+    if (t < 64) {
+      double v = -1.0;
+      int idx = nel;
+      if (t >= k && t < nel) {
+        const cd x = sM[t * n2 + k];
+        v = fabs(x.re) + fabs(x.im);
+        idx = t;
+      }
+      for (int off = 32; off > 0; off >>= 1) {
+        const double v2 = __shfl_xor(v, off, 64);
+        const int i2 = __shfl_xor(idx, off, 64);
+        if (v2 > v || (v2 == v && i2 < idx)) {
+          v = v2;
+          idx = i2;
+        }
+      }
+      if (t == 0) {
+        sPiv = idx;
+        sSing = (v == 0.0) ? 1 : 0;
+      }
+    }
+    __syncthreads();
+    const int p = sPiv;
+    if (sSing) {
+      sing = true;
+      break;
+    }
+    if (p != k)
+      for (int e = t; e < n2; e += kPhThreads) {
+        const cd x = sM[k * n2 + e];
+        sM[k * n2 + e] = sM[p * n2 + e];
+        sM[p * n2 + e] = x;
+      }
+    __syncthreads();
+    const cd piv = sM[k * n2 + k];
+    if (t == 0) {
+      const cd u = p != k ? -piv : piv;
+      logd += clog_(cmk(u.re + DBL_MIN, u.im));
+    }
+    const cd inv = cdiv(cmk(1.0, 0.0), piv);
+    for (int e = t; e < n2; e += kPhThreads) sRow[e] = sM[k * n2 + e] * inv;
+    for (int e = t; e < nel; e += kPhThreads) sF[e] = sM[e * n2 + k];
+    __syncthreads();
+    for (int e = t; e < nel * n2; e += kPhThreads) {
+      const int r = e / n2, col = e % n2;
+      sM[e] = r == k ? sRow[col] : sM[e] - sF[r] * sRow[col];
+    }
+    __syncthreads();
+  }
+  if (sing) {
+    for (int e = t; e < nel * nel; e += kPhThreads) sM[(e / nel) * n2 + nel + e % nel] = cmk(0, 0);
+    __syncthreads();
+  }
+  if (t == 0) {
+    const double pi = 3.14159265358979323846;
+    if (sing) {
+      logd = cmk(-INFINITY, 0.0);
+    } else if (isfinite(logd.im)) {
+      while (logd.im > pi) logd.im -= 2 * pi;
+      while (logd.im < -pi) logd.im += 2 * pi;
+    }
+    ld[(long)c * nw + w] = logd;
+  }
+  const cd* Ti = sM + nel;           // Tinv[j][b] = Ti[j * n2 + b]
+  // one-body term
+  cd e1 = cmk(0.0, 0.0);
+  const cd* Hw = Hp + (long)w * nel * nact;
+  for (int e = t; e < nel * nel; e += kPhThreads) {
+    const int a = e / nel, j = e % nel;
+    e1 += Ti[j * n2 + a] * Hw[(long)j * nact + sConf[a]];
+  }
+  // Cholesky terms, nbv vectors at a time
+  const cd* Pw = P + (long)w * nel * nchol * nact;
+  const int npair = nel * (nel + 1) / 2;
+  cd exx = cmk(0.0, 0.0), ej = cmk(0.0, 0.0), cross = cmk(0.0, 0.0);
+  for (int n0 = 0; n0 < nchol; n0 += nbv) {
+    const int nv = min(nbv, nchol - n0);
+    __syncthreads();                 // the previous chunk's readers are done
+    for (int e = t; e < nv * nel * nel; e += kPhThreads) {
+      const int a = e % nel, j = (e / nel) % nel, v = e / (nel * nel);
+      sPc[(v * nel + a) * nel + j] = Pw[((long)j * nchol + n0 + v) * nact + sConf[a]];
+    }
+    __syncthreads();
+    for (int e = t; e < nv * npair; e += kPhThreads) {
+      const int v = e / npair;
+      int q = e % npair, a = 0;
+      while (q >= nel - a) {
+        q -= nel - a;
+        ++a;
+      }
+      const int b = a + q;
+      const cd* pa = sPc + (v * nel + a) * nel;
+      const cd* pb = sPc + (v * nel + b) * nel;
+      cd yab = cmk(0.0, 0.0), yba = cmk(0.0, 0.0);
+      for (int j = 0; j < nel; ++j) {
+        yab += pa[j] * Ti[j * n2 + b];
+        yba += pb[j] * Ti[j * n2 + a];
+      }
+      if (a == b) {
+        exx += yab * yab;
+        sDg[v * nel + a] = yab;
+      } else {
+        exx += 2.0 * (yab * yba);
+      }
+    }
+    __syncthreads();
+    if (t < nv) {
+      cd kv = cmk(0.0, 0.0);
+      for (int a = 0; a < nel; ++a) kv += sDg[t * nel + a];
+      ej += kv * kv;
+      const long o = ((long)c * nw + w) * nchol + n0 + t;
+      if (KE) KE[o] = kv;
+      if (KEr) cross += KEr[o] * kv;
+    }
+  }
+  const cd s1 = block_sum(e1, red);
+  const cd sx = block_sum(exx, red);
+  const cd sj = block_sum(ej, red);
+  const cd sc = block_sum(cross, red);
+  if (t == 0) {
+    cd* out = Emsd + ((long)c * nw + w) * 4;
+    out[0] = s1 + E0;
+    out[1] = -0.5 * sx;
+    out[2] = 0.5 * sj;
+    out[3] = sc;
+  }
+}
+
+// ld[c][w] -> ratio exp(ld[c][w] - ld[0][w]) in place (row 0 -> 1), log_ov[w] += ld[0][w]
+// (PHMSD.cpp:245-252, 364-369)
+__global__ void k_ph0_ratios(int nw, int nc, cd* __restrict__ r, cd* __restrict__ log_ov) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= nw) return;
+  const cd l0 = r[w];
+  for (int c = nc - 1; c >= 1; --c) r[(long)c * nw + w] = cexp_(r[(long)c * nw + w] - l0);
+  r[w] = cmk(1.0, 0.0);
+  log_ov[w] += l0;
+}
+
+// acc[w][k] = sum_u wgt0[u][w] Em0[u][w][k] + sum_d wgt1[d][w] Em1[d][w][k]
+//             (+ sum_d r1[d][w] Em1[d][w][3] for k = 2: the opposite-spin Coulomb term)
+// (PHMSD.cpp:301-359)
+__global__ void k_ph0_assemble(int nw, int nu, int nd, const cd* __restrict__ wgt0,
+                               const cd* __restrict__ Em0, const cd* __restrict__ wgt1,
+                               const cd* __restrict__ Em1, const cd* __restrict__ r1,
+                               cd* __restrict__ acc) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= 3 * nw) return;
+  const int w = e / 3, k = e % 3;
+  cd v = cmk(0.0, 0.0);
+  for (int u = 0; u < nu; ++u) v += wgt0[(long)u * nw + w] * Em0[((long)u * nw + w) * 4 + k];
+  for (int d = 0; d < nd; ++d) v += wgt1[(long)d * nw + w] * Em1[((long)d * nw + w) * 4 + k];
+  if (k == 2)
+    for (int d = 0; d < nd; ++d) v += r1[(long)d * nw + w] * Em1[((long)d * nw + w) * 4 + 3];
+  acc[e] = v;
 }
 
 }  // namespace
@@ -977,6 +1195,59 @@ void launch_ph_log_overlap(hipStream_t s, int nw, const double* Ov, const double
   hipLaunchKernelGGL(k_ph_log_overlap, ph_blocks(nw), dim3(256), 0, s, nw,
                      reinterpret_cast<const cd*>(Ov), reinterpret_cast<const cd*>(log_ov),
                      reinterpret_cast<cd*>(out));
+  AFQ_HIP(hipGetLastError());
+}
+
+// energy_alg0 (PHMSD.cpp:122-370): the walker products of one spin. Phit, TAB, Hp and P as in
+// k_ph0_config; Phi_w = Phi + w ldw (c128 [nmo][nel]).
+void launch_ph0_products(hipStream_t s, int nw, int nmo, int nel, int nact, int nchol,
+                         const double* Phi, long ldw, const double* A, const double* haj,
+                         const double* Lna, double* Phit, double* TAB, double* Hp, double* P) {
+  if (nw <= 0) return;
+  hipLaunchKernelGGL(k_ph0_transpose, dim3((unsigned)(((long)nel * nmo + 255) / 256), nw),
+                     dim3(256), 0, s, nmo, nel, reinterpret_cast<const cd*>(Phi), ldw,
+                     reinterpret_cast<cd*>(Phit));
+  AFQ_HIP(hipGetLastError());
+  launch_gemm_nt_z(s, AFQ_FP64, nw * nel, nact, nmo, 1.0, Phit, nmo, A, nmo, true, TAB, nact);
+  launch_gemm_nt_z(s, AFQ_FP64, nw * nel, nact, nmo, 1.0, Phit, nmo, haj, nmo, true, Hp, nact);
+  launch_gemm_nt_z(s, AFQ_FP64, nw * nel, nchol * nact, nmo, 1.0, Phit, nmo, Lna, nmo, true, P,
+                   (long)nchol * nact);
+}
+
+int ph0_vectors_per_chunk(int nel) { return std::max(1, std::min(16, 1536 / (nel * nel))); }
+
+void launch_ph0_config(hipStream_t s, int nw, int nc, int nel, int nact, int nchol,
+                       const int* conf, const double* TAB, const double* Hp, const double* P,
+                       double E0, double* ld, double* Emsd, double* KE, const double* KEr) {
+  if (nw <= 0 || nc <= 0) return;
+  AFQ_CHECK(nel >= 1 && nel <= 32, AFQ_EUNSUPPORTED,
+            "PHMSD reference energy: more than 32 electrons of one spin");
+  const int nbv = ph0_vectors_per_chunk(nel);
+  const size_t lds = sizeof(cd) * ((size_t)nel * 2 * nel + 2 * nel + nel +
+                                   (size_t)nbv * nel * nel + (size_t)nbv * nel);
+  hipLaunchKernelGGL(k_ph0_config, dim3((unsigned)((long)nc * nw)), dim3(kPhThreads), lds, s, nw,
+                     nc, nel, nact, nchol, nbv, conf, reinterpret_cast<const cd*>(TAB),
+                     reinterpret_cast<const cd*>(Hp), reinterpret_cast<const cd*>(P),
+                     cmk(E0, 0.0), reinterpret_cast<cd*>(ld), reinterpret_cast<cd*>(Emsd),
+                     reinterpret_cast<cd*>(KE), reinterpret_cast<const cd*>(KEr));
+  AFQ_HIP(hipGetLastError());
+}
+
+void launch_ph0_ratios(hipStream_t s, int nw, int nc, double* r, double* log_ov) {
+  if (nw <= 0) return;
+  hipLaunchKernelGGL(k_ph0_ratios, ph_blocks(nw), dim3(256), 0, s, nw, nc,
+                     reinterpret_cast<cd*>(r), reinterpret_cast<cd*>(log_ov));
+  AFQ_HIP(hipGetLastError());
+}
+
+void launch_ph0_assemble(hipStream_t s, int nw, int nu, int nd, const double* wgt0,
+                         const double* Em0, const double* wgt1, const double* Em1,
+                         const double* r1, double* acc) {
+  if (nw <= 0) return;
+  hipLaunchKernelGGL(k_ph0_assemble, ph_blocks(3L * nw), dim3(256), 0, s, nw, nu, nd,
+                     reinterpret_cast<const cd*>(wgt0), reinterpret_cast<const cd*>(Em0),
+                     reinterpret_cast<const cd*>(wgt1), reinterpret_cast<const cd*>(Em1),
+                     reinterpret_cast<const cd*>(r1), reinterpret_cast<cd*>(acc));
   AFQ_HIP(hipGetLastError());
 }
 

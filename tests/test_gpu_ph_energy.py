@@ -87,6 +87,52 @@ def test_ph_energy_c5_shape(ndet, max_rank):
     ph.close()
 
 
+@pytest.mark.parametrize("uhf,nmo,nup,ndn,ndet,rank,nw,block", [
+    (False, 9, 4, 3, 40, 3, 5, 32), (True, 9, 4, 3, 40, 3, 9, 4), (False, 24, 17, 12, 30, 2, 3, 2),
+    (False, 8, 3, 3, 1, 1, 4, 8)])
+def test_ph_energy_reference_algorithm_matches_oracle(uhf, nmo, nup, ndn, ndet, rank, nw, block):
+    """energy_alg0 (set_energy_algorithm("reference")): one G and energy per unique
+    configuration; 17 electrons of one spin cover pivoting past 16 rows, ndet = 1 the
+    single-determinant case."""
+    rng = np.random.default_rng(60 + uhf + nup)
+    s = ph_system(rng, nmo, nup, ndn, 7, ndet, rank, uhf=uhf)
+    Pu, Pd, buf = walkers(rng, nw, nmo, nup, ndn)
+    E0 = 1.25
+    orb = np.stack(s["A"][:2]) if uhf else s["A"][0][None]
+    ph = A.PHMSD(s["occs"], s["ci"], orb, s["h"], s["L"].transpose(1, 2, 0), E0, walker_block=block)
+    ph.set_energy_algorithm("reference")
+    E, ov = ph.energy(A.to_device(buf))
+    Eo, ovo = P.energy_alg0(s["struct"], s["A"], s["Lna"], s["haj"], E0, Pu, Pd)
+    np.testing.assert_allclose(E.cpu().numpy(), Eo, rtol=1e-10, atol=1e-11 * np.abs(Eo).max())
+    d = ov.cpu().numpy() - ovo
+    d.imag = (d.imag + np.pi) % (2 * np.pi) - np.pi
+    assert np.abs(d).max() < 1e-10
+    with pytest.raises(ValueError):
+        ph.set_energy_algorithm("dense")
+    ph.close()
+
+
+def test_ph_energy_algorithms_agree_c5_shape():
+    """NMO 64, 10 + 10 electrons, 1000 determinants of rank 1-4, 24 walkers in blocks of 16:
+    the reference algorithm and the Woodbury one on the GPU give the same local energy and
+    overlap (the Woodbury one is checked against the oracle at this shape above)."""
+    rng = np.random.default_rng(5)
+    s = ph_system(rng, 64, 10, 10, 48, 1000, 4)
+    Pu, Pd, buf = walkers(rng, 24, 64, 10, 10)
+    ph = A.PHMSD(s["occs"], s["ci"], s["A"][0][None], s["h"], s["L"].transpose(1, 2, 0), 0.0,
+                 walker_block=16)
+    wb = A.to_device(buf)
+    E1, ov1 = ph.energy(wb)
+    ph.set_energy_algorithm("reference")
+    E0, ov0 = ph.energy(wb)
+    E0, E1 = E0.cpu().numpy(), E1.cpu().numpy()
+    np.testing.assert_allclose(E0, E1, rtol=1e-9, atol=1e-10 * np.abs(E1).max())
+    d = ov0.cpu().numpy() - ov1.cpu().numpy()
+    d.imag = (d.imag + np.pi) % (2 * np.pi) - np.pi
+    assert np.abs(d).max() < 1e-9
+    ph.close()
+
+
 @pytest.mark.parametrize("uhf,nw,block", [(False, 6, 32), (True, 11, 4)])
 def test_ph_density_matrix_and_vbias_match_oracle(uhf, nw, block):
     rng = np.random.default_rng(80 + uhf)

@@ -131,6 +131,36 @@ def test_single_determinant_reduces_to_the_reference_energy():
     np.testing.assert_allclose(np.exp(logov), 0.9 * ovb / 0.9, rtol=1e-11)
 
 
+@pytest.mark.parametrize("uhf,ndet", [(False, 40), (True, 40), (False, 1)])
+def test_energy_alg0_matches_brute_force(uhf, ndet):
+    """energy_alg0 (one G and energy per unique configuration) against every determinant built
+    explicitly; ndet = 1 is the single-determinant energy."""
+    rng = np.random.default_rng(23 + uhf + ndet)
+    s = ph_system(rng, 9, 4, 3, 7, ndet, 3, uhf=uhf)
+    nw = 5
+    Phi_up, Phi_dn = rand_c(rng, nw, 9, 4), rand_c(rng, nw, 9, 3)
+    E0 = 1.25
+    E, logov = P.energy_alg0(s["struct"], s["A"], s["Lna"], s["haj"], E0, Phi_up, Phi_dn)
+    Eb, ovb = brute_force(s, Phi_up, Phi_dn, E0)
+    scale = np.abs(Eb).max()
+    np.testing.assert_allclose(E, Eb, rtol=1e-10, atol=1e-11 * scale)
+    np.testing.assert_allclose(np.exp(logov), ovb, rtol=1e-10)
+
+
+def test_energy_algorithms_agree():
+    """The reference algorithm and the Woodbury one are the same local energy: rank-1..4
+    excitations, 120 determinants, UHF orbitals."""
+    rng = np.random.default_rng(41)
+    s = ph_system(rng, 14, 5, 4, 9, 120, 4, uhf=True)
+    Phi_up, Phi_dn = rand_c(rng, 6, 14, 5), rand_c(rng, 6, 14, 4)
+    E0, ov0 = P.energy_alg0(s["struct"], s["A"], s["Lna"], s["haj"], 0.5, Phi_up, Phi_dn)
+    E1, ov1 = P.energy_alg1(s["struct"], s["A"], s["Lna"], s["haj"], 0.5, Phi_up, Phi_dn)
+    np.testing.assert_allclose(E0, E1, rtol=1e-10, atol=1e-11 * np.abs(E1).max())
+    d = ov0 - ov1
+    d.imag = (d.imag + np.pi) % (2 * np.pi) - np.pi
+    assert np.abs(d).max() < 1e-10
+
+
 def brute_force_density(sysd, Phi_up, Phi_dn):
     """The full multi-determinant mixed density matrix of each spin,
     sum_D conj(c_D) <D|phi> rho_D^T / sum_D conj(c_D) <D|phi> with rho_D = Phi (A_D Phi)^-1 A_D."""
