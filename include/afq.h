@@ -188,6 +188,24 @@ int afq_step(afq_ham ham, int W, double* d_wbuf, long ldw, const double* d_xi, d
 /* AFQMCBasePropagator::Orthogonalize (AFQMCBasePropagator.cpp:735-741): QR of every walker,
  * then OVLP recomputed with NOMSD::Log_Overlap. */
 int afq_orthogonalize(afq_ham ham, int W, double* d_wbuf, long ldw, void* stream);
+/* Back-propagation field history (WalkerSet::storeFields / advanceHistoryPos / getHistoryPos,
+ * used by AFQMCBasePropagator::Propagate at AFQMCBasePropagator.cpp:341-344, 387-390): with
+ * nfields > 0 every afq_step stores its assembled fields X [W][K] into slot (position % nfields) of
+ * the caller's ring d_fields [W][nfields][K] c128 (the reference's Fields(nwalk, steps, nCV)) and
+ * advances the position, which lives on the GPU (a captured step replays it). Resets the position
+ * to 0; nfields = 0 switches the history off. */
+int afq_set_field_history(afq_ham ham, int W, int nfields, double* d_fields, void* stream);
+/* The history position (steps stored since afq_set_field_history). Synchronises the stream. */
+int afq_history_position(afq_ham ham, int* pos, void* stream);
+/* AFQMCBasePropagator::BackPropagate (AFQMCBasePropagator.cpp:403-532), dense vHS: for the last
+ * nbpsteps stored field sets, newest first, vHS = sqrt(dt) L X (FP64) and every reference
+ * d_refs[w][nr] [M][N] (d_refs [W][nrefs][M][N] c128, the reference's Refs) propagated with the
+ * adjoint propagator P1^H exp(-i vHS^H) P1^H (Propagate<'H'>, order taylor_n); after the steps
+ * with ni % nstabilize == 0 and after the last one, QR (det_ops::orthogonalize) with log det R
+ * added to d_logdetR [W][nrefs] c128 (zeroed first). The walkers' own Slater matrices are not
+ * touched. nbpsteps <= nfields; W <= the ring's and the workspace's walker count. */
+int afq_back_propagate(afq_ham ham, int W, int nbpsteps, int nstabilize, double* d_refs,
+                       int nrefs, double* d_logdetR, void* stream);
 /* NOMSD::Energy(wset) (NOMSD.cpp:69-80) + seedPseudoEnergy: OVLP, E1, EXX, EJ and
  * PSEUDO_ELOC = E1+EXX+EJ of every walker (execute.cpp:172-184). */
 int afq_seed_properties(afq_ham ham, int W, double* d_wbuf, long ldw, void* stream);

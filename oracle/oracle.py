@@ -230,6 +230,38 @@ def orthogonalize(Phi):
     return Phi, ld
 
 
+def back_propagate(fields, pos, nbpsteps, nstab, Refs, P1, Likn, dt, order=6):
+    """AFQMCBasePropagator::BackPropagate (AFQMCBasePropagator.cpp:403-532), dense vHS, one Slater
+    matrix per reference: fields [W][nfields][K] (the ring), pos the history position (the slot
+    the next step would fill), Refs [W][nrefs][M][N]. Newest field set first; each reference
+    propagated by Propagate<'H'> (propagate.hpp:241-298 with apply_expM<'H'>, 41-70):
+    S <- P1^H S, T = S, T <- (-i/n) V^H T, S += T (n = 1..order), S <- P1^H S; QR
+    (orthogonalize) after the steps with ni % nstab == 0 and after the last one, log det R added
+    to logdetR. Returns (Refs, logdetR [W][nrefs])."""
+    fields = np.asarray(fields, np.complex128)
+    W, nfields, _ = fields.shape
+    Refs = np.array(Refs, np.complex128)
+    nrefs = Refs.shape[1]
+    logdetR = np.zeros((W, nrefs), np.complex128)
+    P1H = np.asarray(P1, np.complex128).conj().T
+    slot = pos % nfields
+    for ni in range(nbpsteps - 1, -1, -1):
+        slot = nfields - 1 if slot == 0 else slot - 1
+        VH = vhs(np.ascontiguousarray(fields[:, slot, :]), Likn, dt).conj().transpose(0, 2, 1)
+        for nr in range(nrefs):
+            S = np.einsum("ij,wjk->wik", P1H, Refs[:, nr])
+            T = S.copy()
+            for n in range(1, order + 1):
+                T = (-1j / n) * np.einsum("wij,wjk->wik", VH, T)
+                S = S + T
+            S = np.einsum("ij,wjk->wik", P1H, S)
+            if ni == 0 or ni % nstab == 0:
+                S, ld = orthogonalize(S)
+                logdetR[:, nr] += ld
+            Refs[:, nr] = S
+    return Refs, logdetR
+
+
 def construct_X(vbias_, xi, vMF, vbias_bound=50.0, project=False, free_projection=False):
     X = _c(vbias_).copy()
     W, K = X.shape

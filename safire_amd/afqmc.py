@@ -68,6 +68,9 @@ SIGNATURES = {
     "afq_walker_update": (_i, [_vp, _i, _vp, _l, _vp, _vp, _vp, _vp, _d, _i, _vp]),
     "afq_step": (_i, [_vp, _i, _vp, _l, _vp, _d, _vp]),
     "afq_orthogonalize": (_i, [_vp, _i, _vp, _l, _vp]),
+    "afq_set_field_history": (_i, [_vp, _i, _i, _vp, _vp]),
+    "afq_history_position": (_i, [_vp, _vp, _vp]),
+    "afq_back_propagate": (_i, [_vp, _i, _i, _i, _vp, _i, _vp, _vp]),
     "afq_seed_properties": (_i, [_vp, _i, _vp, _l, _vp]),
     "afq_weighted_sums": (_i, [_vp, _i, _vp, _l, _vp, _vp]),
     "afq_estimator_sums": (_i, [_vp, _i, _vp, _l, _vp, _vp, _vp, _vp]),
@@ -428,6 +431,28 @@ class WalkerSet:
         M, N = self.ham.M, self.ham.N
         return self.buf[:, :M * N].reshape(self.W, M, N)
 
+    def set_back_propagation(self, nfields: int, stream=None):
+        """The field history of back-propagation (WalkerSet NumBackProp / storeFields): a ring
+        Fields [W][nfields][K] c128 that every Propagate fills, position reset to 0;
+        nfields = 0 switches it off."""
+        torch = _torch()
+        self.fields = (torch.zeros((self.W, nfields, self.ham.K), dtype=torch.complex128,
+                                   device="cuda") if nfields > 0 else None)
+        check(lib().afq_set_field_history(self.ham.handle, self.W, int(nfields),
+                                          ptr(self.fields) if nfields > 0 else None,
+                                          stream_ptr(stream)))
+
+    def NumBackProp(self):
+        return 0 if getattr(self, "fields", None) is None else self.fields.shape[1]
+
+    def getFields(self):
+        return getattr(self, "fields", None)
+
+    def getHistoryPos(self, stream=None):
+        pos = C.c_int()
+        check(lib().afq_history_position(self.ham.handle, C.byref(pos), stream_ptr(stream)))
+        return pos.value
+
     def getProperty(self, p):
         return self.buf[:, self.ham.M * self.ham.N + p]
 
@@ -489,6 +514,18 @@ class Propagator:
     def Orthogonalize(self, wset: WalkerSet, stream=None):
         check(lib().afq_orthogonalize(self.ham.handle, wset.W, ptr(wset.buf), wset.ldw,
                                       stream_ptr(stream)))
+
+    def BackPropagate(self, nbpsteps: int, nStabilize: int, wset: WalkerSet, Refs, logdetR,
+                      stream=None):
+        """AFQMCBasePropagator::BackPropagate (AFQMCBasePropagator.cpp:403-532): Refs
+        [W][nrefs][M][N] c128 propagated in place with the adjoint propagator over the last
+        nbpsteps stored field sets (newest first); logdetR [W][nrefs] c128 receives the log
+        determinants of the orthogonalisations. Needs wset.set_back_propagation first."""
+        W, nrefs = Refs.shape[0], Refs.shape[1]
+        assert Refs.is_contiguous() and logdetR.is_contiguous()
+        assert tuple(logdetR.shape) == (W, nrefs)
+        check(lib().afq_back_propagate(self.ham.handle, W, int(nbpsteps), int(nStabilize),
+                                       ptr(Refs), nrefs, ptr(logdetR), stream_ptr(stream)))
 
     def population_control(self, wset: WalkerSet, uniform: float, stream=None):
         check(lib().afq_population_control(self.ham.handle, wset.W, ptr(wset.buf), wset.ldw,

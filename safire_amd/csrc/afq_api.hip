@@ -266,6 +266,9 @@ int afq_ham_destroy(afq_ham H) {
     dfree(H->Lnak_q);
     dfree(H->Lnak_u);
     dfree(H->stats);
+    dfree(H->hist_pos);
+    dfree(H->P1H);
+    dfree(H->bp_ldet);
     delete H;
   });
 }
@@ -625,6 +628,8 @@ int afq_step(afq_ham H, int W, double* wbuf, long ldw, const double* xi, double 
     check_ham(H);
     check_walkers(H, W, ldw);
     AFQ_CHECK(xi != nullptr, AFQ_EINVAL, "null fields");
+    AFQ_CHECK(H->nfields == 0 || W <= H->fields_W, AFQ_ECAPACITY,
+              "more walkers than the back-propagation field history holds");
     hipStream_t s = S(stream);
     const int M = H->M, N = H->N, K = H->K;
     const long NM = (long)N * M;
@@ -647,6 +652,9 @@ int afq_step(afq_ham H, int W, double* wbuf, long ldw, const double* xi, double 
       RegionScope r(tm, s, R_ASSEMBLE_X);
       launch_construct_X(s, W, K, p.vbias_bound, p.use_cp_constraint || p.project_force_bias,
                          p.free_projection, H->vMF, xi, ws.X, ws.HW, ws.mf);
+      // the fields of this step into the back-propagation ring (storeFields)
+      if (H->nfields > 0)
+        launch_store_fields(s, W, K, ws.X, H->fields, H->nfields, H->hist_pos);
     }
     // 3. vHS = sqrt(dt) L X           4. Phi <- P1 exp(iV) P1 Phi
     {
@@ -678,6 +686,83 @@ int afq_step(afq_ham H, int W, double* wbuf, long ldw, const double* xi, double 
       if (!p.hybrid) finalize_energy(H, s, W, ws.E);
       launch_walker_update(s, W, wbuf, ldw, (long)M * N, p, H->dt, Eshift, H->E0, ws.ov, ws.E,
                            ws.mf, ws.HW, p.hybrid, H->stats);
+    }
+    if (H->nfields > 0) launch_advance_history(s, H->hist_pos);   // advanceHistoryPos
+  });
+}
+
+int afq_set_field_history(afq_ham H, int W, int nfields, double* d_fields, void* stream) {
+  return guard([&] {
+    check_ham(H);
+    AFQ_CHECK(nfields >= 0 && W >= 0, AFQ_EBADSHAPE, "afq_set_field_history: negative size");
+    AFQ_CHECK(nfields == 0 || d_fields != nullptr, AFQ_EINVAL,
+              "afq_set_field_history: null field ring");
+    if (!H->hist_pos) dalloc(&H->hist_pos, 1);
+    AFQ_HIP(hipMemsetAsync(H->hist_pos, 0, sizeof(int), S(stream)));
+    AFQ_HIP(hipStreamSynchronize(S(stream)));
+    H->fields = nfields > 0 ? d_fields : nullptr;
+    H->nfields = nfields;
+    H->fields_W = nfields > 0 ? W : 0;
+  });
+}
+
+int afq_history_position(afq_ham H, int* pos, void* stream) {
+  return guard([&] {
+    check_ham(H);
+    AFQ_CHECK(pos != nullptr, AFQ_EINVAL, "afq_history_position: null output");
+    *pos = 0;
+    if (!H->hist_pos) return;
+    AFQ_HIP(hipMemcpyAsync(pos, H->hist_pos, sizeof(int), hipMemcpyDeviceToHost, S(stream)));
+    AFQ_HIP(hipStreamSynchronize(S(stream)));
+  });
+}
+
+int afq_back_propagate(afq_ham H, int W, int nbpsteps, int nstabilize, double* d_refs, int nrefs,
+                       double* d_logdetR, void* stream) {
+  return guard([&] {
+    check_ham(H);
+    AFQ_CHECK(H->nfields > 0, AFQ_EINVAL, "afq_back_propagate: no field history registered");
+    AFQ_CHECK(nbpsteps > 0, AFQ_EINVAL, "afq_back_propagate: nbpsteps must be positive");
+    AFQ_CHECK(nbpsteps <= H->nfields, AFQ_EBADSHAPE,
+              "afq_back_propagate: more steps than the field history holds");
+    AFQ_CHECK(nstabilize > 0, AFQ_EINVAL, "afq_back_propagate: nstabilize must be positive");
+    AFQ_CHECK(W >= 0 && W <= H->fields_W && W <= H->ws.cap, AFQ_ECAPACITY,
+              "afq_back_propagate: more walkers than the field history or workspace holds");
+    AFQ_CHECK(nrefs >= 1, AFQ_EBADSHAPE, "afq_back_propagate: no references");
+    AFQ_CHECK(d_refs && d_logdetR, AFQ_EINVAL, "afq_back_propagate: null operand");
+    if (W == 0) return;
+    hipStream_t s = S(stream);
+    const int M = H->M, N = H->N, K = H->K, MM = M * M, nf = H->nfields;
+    const long NM = (long)N * M;
+    if (!H->P1H) dalloc(&H->P1H, 2L * MM);
+    if (H->bp_cap < W) {
+      dfree(H->bp_ldet);
+      dalloc(&H->bp_ldet, 2L * W);
+      H->bp_cap = W;
+    }
+    int pos = 0;
+    AFQ_HIP(hipMemcpyAsync(&pos, H->hist_pos, sizeof(int), hipMemcpyDeviceToHost, s));
+    AFQ_HIP(hipStreamSynchronize(s));
+    launch_adjoint(s, M, H->P1, H->P1H);
+    launch_ph_fill(s, (long)W * nrefs, 0.0, 0.0, d_logdetR);
+    int slot = pos % nf;
+    for (int ni = nbpsteps - 1; ni >= 0; --ni) {
+      // newest entry first: the position already points at the next step's slot
+      slot = slot == 0 ? nf - 1 : slot - 1;
+      // vHS = sqrt(dt) L X of the stored fields (FP64), then V <- -V^H
+      launch_gemm_nt_z(s, AFQ_FP64, W, MM, K, std::sqrt(H->dt), H->fields + 2L * slot * K,
+                       (long)nf * K, H->Likn, K, false, H->ws.V, MM);
+      launch_neg_adjoint(s, W, M, H->ws.V);
+      for (int nr = 0; nr < nrefs; ++nr) {
+        double* Phi = d_refs + 2L * nr * NM;
+        const long ld = (long)nrefs * NM;
+        launch_propagate(s, W, M, N, Phi, ld, H->P1H, H->ws.V, H->params.taylor_n);
+        if (ni == 0 || ni % nstabilize == 0) {
+          launch_ph_fill(s, W, 0.0, 0.0, H->bp_ldet);
+          launch_orthogonalize(s, W, M, N, Phi, ld, H->bp_ldet);
+          launch_add_strided(s, W, H->bp_ldet, d_logdetR + 2L * nr, nrefs);
+        }
+      }
     }
   });
 }

@@ -175,6 +175,13 @@ struct afq_ham_s {
   double* Lnak_u = nullptr;  // f64 [K][N]   power-of-two unit of each Lnak row
   // [7]: eloc total/upper/lower, weight total/upper, population control calls/collapses
   unsigned long long* stats = nullptr;
+  // back-propagation: the caller's field ring [fields_W][nfields][K] c128 and its position
+  double* fields = nullptr;
+  int nfields = 0, fields_W = 0;
+  int* hist_pos = nullptr;   // device int
+  double* P1H = nullptr;     // c128 [M][M]
+  double* bp_ldet = nullptr; // c128 [bp_cap]
+  int bp_cap = 0;
   afq::Workspace ws;
   afq::Timers timers;
 };
@@ -233,6 +240,12 @@ void launch_propagate(hipStream_t s, int nb, int M, int N, double* Phi, long ldw
 void launch_orthogonalize(hipStream_t s, int nb, int M, int N, double* Phi, long ldw,
                           double* logdetR);
 // k_walker.hip
+void launch_store_fields(hipStream_t s, int W, int K, const double* X, double* ring, int nfields,
+                         const int* pos);
+void launch_advance_history(hipStream_t s, int* pos);
+void launch_neg_adjoint(hipStream_t s, int W, int M, double* V);
+void launch_adjoint(hipStream_t s, int M, const double* A, double* B);
+void launch_add_strided(hipStream_t s, int W, const double* x, double* out, long stride);
 void launch_construct_X(hipStream_t s, int W, int K, double vbias_bound, int project,
                         int free_projection, const double* vMF, const double* xi, double* X,
                         double* HW, double* mf);

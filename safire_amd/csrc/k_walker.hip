@@ -426,6 +426,55 @@ __global__ void k_probe_mfma(double* out) {
   for (int r = 0; r < 4; ++r) out[lane * 4 + r] = (double)acc[r];
 }
 
+// ---- back-propagation (AFQMCBasePropagator.cpp:336-344, 387-390, 403-532)
+
+// WalkerSet::storeFields: ring[w][pos % nfields][n] = X[w][n] (the history position lives on the
+// GPU so that a captured step replays it)
+__global__ void k_store_fields(int W, int K, const cd* __restrict__ X, cd* __restrict__ ring,
+                               int nfields, const int* __restrict__ pos) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)W * K) return;
+  const int slot = *pos % nfields;
+  const long w = e / K, n = e % K;
+  ring[(w * nfields + slot) * K + n] = X[e];
+}
+
+// WalkerSet::advanceHistoryPos
+__global__ void k_advance_history(int* pos) {
+  if (threadIdx.x == 0) *pos += 1;
+}
+
+// V_w <- -V_w^H in place, so that the forward propagator's exp(i V) series becomes the 'H'
+// series exp(-i V^H) of apply_expM<'H'> (propagate.hpp:57-58, 96-97). One thread per pair i <= j.
+__global__ void k_neg_adjoint(int M, cd* __restrict__ V) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int w = blockIdx.y;
+  if (e >= (long)M * M) return;
+  const int i = (int)(e / M), j = (int)(e % M);
+  if (i > j) return;
+  cd* Vw = V + (long)w * M * M;
+  const cd a = Vw[(long)i * M + j], b = Vw[(long)j * M + i];
+  Vw[(long)i * M + j] = cmk(-b.re, b.im);
+  Vw[(long)j * M + i] = cmk(-a.re, a.im);
+}
+
+// B = A^H for one [M][M] matrix (math::product<'H'>(P1, ...), propagate.hpp:282, 297)
+__global__ void k_adjoint(int M, const cd* __restrict__ A, cd* __restrict__ B) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)M * M) return;
+  const int i = (int)(e / M), j = (int)(e % M);
+  const cd a = A[(long)j * M + i];
+  B[e] = cmk(a.re, -a.im);
+}
+
+// out[w * stride] += x[w]
+__global__ void k_add_strided(int W, const cd* __restrict__ x, cd* __restrict__ out,
+                              long stride) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w < W) out[(long)w * stride] += x[w];
+}
+
+
 }  // namespace
 
 void launch_construct_X(hipStream_t s, int W, int K, double vbias_bound, int project,
@@ -538,6 +587,40 @@ void launch_probe_mfma(hipStream_t s, int kind, double* out) {
     hipLaunchKernelGGL(k_probe_mfma_bf16, dim3(1), dim3(64), 0, s, kind == 2 ? 0 : 31, out);
   else
     hipLaunchKernelGGL(k_probe_mfma_i8, dim3(1), dim3(64), 0, s, kind == 4 ? 0 : 63, out);
+  AFQ_HIP(hipGetLastError());
+}
+
+void launch_store_fields(hipStream_t s, int W, int K, const double* X, double* ring, int nfields,
+                         const int* pos) {
+  if (W <= 0 || K <= 0) return;
+  hipLaunchKernelGGL(k_store_fields, dim3((unsigned)(((long)W * K + 255) / 256)), dim3(256), 0, s,
+                     W, K, reinterpret_cast<const cd*>(X), reinterpret_cast<cd*>(ring), nfields,
+                     pos);
+  AFQ_HIP(hipGetLastError());
+}
+
+void launch_advance_history(hipStream_t s, int* pos) {
+  hipLaunchKernelGGL(k_advance_history, dim3(1), dim3(64), 0, s, pos);
+  AFQ_HIP(hipGetLastError());
+}
+
+void launch_neg_adjoint(hipStream_t s, int W, int M, double* V) {
+  if (W <= 0) return;
+  hipLaunchKernelGGL(k_neg_adjoint, dim3((unsigned)(((long)M * M + 255) / 256), W), dim3(256), 0,
+                     s, M, reinterpret_cast<cd*>(V));
+  AFQ_HIP(hipGetLastError());
+}
+
+void launch_adjoint(hipStream_t s, int M, const double* A, double* B) {
+  hipLaunchKernelGGL(k_adjoint, dim3((unsigned)(((long)M * M + 255) / 256)), dim3(256), 0, s, M,
+                     reinterpret_cast<const cd*>(A), reinterpret_cast<cd*>(B));
+  AFQ_HIP(hipGetLastError());
+}
+
+void launch_add_strided(hipStream_t s, int W, const double* x, double* out, long stride) {
+  if (W <= 0) return;
+  hipLaunchKernelGGL(k_add_strided, dim3((unsigned)((W + 255) / 256)), dim3(256), 0, s, W,
+                     reinterpret_cast<const cd*>(x), reinterpret_cast<cd*>(out), stride);
   AFQ_HIP(hipGetLastError());
 }
 
