@@ -213,6 +213,7 @@ def bench_phmsd(args, A, torch, synth):
     ph = A.PHMSD(occs, ci, orb, s.h, s.Likn, 0.0, walker_block=64)
     ph.prepare(s.dt, W)
     ph.set_params(hybrid=0)
+    ph.set_energy_algorithm(args.ph_energy)
     xi = torch.from_numpy(synth.fields(s.seed, warmup + steps, W, s.K)).to("cuda")
     uniforms = np.random.default_rng(s.seed).random(warmup + steps)
     print(f"[bench] c5: M={s.M} N={s.N}+{s.N} K={s.K} W={W}, {len(ci)} determinants "
@@ -238,8 +239,8 @@ def bench_phmsd(args, A, torch, synth):
         print(f"[bench] c5 step {t}: {ms:.1f} ms", file=sys.stderr, flush=True)
     ms = total / steps
     value = W / (ms / 1e3)
-    print(f"[bench] c5 fp64: {ms:.2f} ms/step, {value:.0f} walker-steps/s", file=sys.stderr,
-          flush=True)
+    print(f"[bench] c5 fp64 ({args.ph_energy} energy): {ms:.2f} ms/step, {value:.0f} "
+          f"walker-steps/s", file=sys.stderr, flush=True)
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "walker-steps/s", "n_gpus": 1,
         "steps": steps, "warmup": warmup, "ms_per_step": round(ms, 4),
@@ -248,7 +249,8 @@ def bench_phmsd(args, A, torch, synth):
         "config": {"workload": f"c5: PHMSD trial, NMO={s.M}, {s.N}+{s.N} electrons, "
                                f"Nchol={s.K}, {len(ci)} determinants of rank 1-4, {W} walkers, "
                                f"dt={s.dt}, local-energy step, QR every {ortho}",
-                   "form": "FP64, eager launches", "parallelism": "single GPU"},
+                   "form": "FP64, eager launches", "ph_energy_algorithm": args.ph_energy,
+                   "parallelism": "single GPU"},
         "roofline": None, "cpu_baseline": None,
     }
     ph.close()
@@ -341,6 +343,8 @@ def main(argv=None):
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--forms", default="fp64,fp32,bf16x3,int8")
+    ap.add_argument("--ph-energy", default="woodbury", choices=["woodbury", "reference"],
+                    help="c5: the PHMSD energy algorithm (the reference's `algorithm` option)")
     ap.add_argument("--no-hybrid", action="store_true",
                     help="skip the hybrid-weight (no per-step local energy) leg")
     ap.add_argument("--no-graph", action="store_true",
