@@ -315,7 +315,8 @@ void launch_ph0_products(hipStream_t s, int nw, int nmo, int nel, int nact, int 
                          const double* Lna, double* Phit, double* TAB, double* Hp, double* P);
 void launch_ph0_config(hipStream_t s, int nw, int nc, int nel, int nact, int nchol,
                        const int* conf, const double* TAB, const double* Hp, const double* P,
-                       double E0, double* ld, double* Emsd, double* KE, const double* KEr);
+                       double E0, double* ld, double* Emsd, double* KE, const double* KEr,
+                       double* Tinv);
 void launch_ph0_ratios(hipStream_t s, int nw, int nc, double* r, double* log_ov);
 void launch_ph0_assemble(hipStream_t s, int nw, int nu, int nd, const double* wgt0,
                          const double* Em0, const double* wgt1, const double* Em1,

@@ -173,6 +173,7 @@ struct afq_ph_t {
          *R = nullptr;
   double *Racc = nullptr, *Gblk = nullptr, *split = nullptr, *p2ws = nullptr;
   double* Emsd = nullptr;          // c128 [2][max nunique][wblock][4] (energy_alg0)
+  double* Tinv0 = nullptr;         // c128 [max nunique][wblock][nel][nel] (energy_alg0)
   long split_elems = 0, p2ws_doubles = 0;
   ~afq_ph_t() {
     for (void* p : owned) (void)hipFree(p);
@@ -269,7 +270,7 @@ void ph_energy_alg0(afq_ph_t* ph, hipStream_t s, int nw, const double* d_wbuf, l
       launch_ph0_config(s, nb, S[sp].nunique, S[sp].nel, na, K, S[sp].dconf, ph->QQ0[sp],
                         ph->Swia, ph->Twina, sp == 0 ? ph->E0 : 0.0, ph->ratios[sp],
                         ph->Emsd + sp * em1, sp == 0 ? ph->KEa : nullptr,
-                        sp == 1 ? ph->KEb : nullptr);
+                        sp == 1 ? ph->KEb : nullptr, ph->Tinv0);
       launch_ph0_ratios(s, nb, S[sp].nunique, ph->ratios[sp], ph->log_ov);
       if (sp == 0)
         launch_ph_couple_ke(s, nb, K, nd, S[1].cp, S[1].ci, S[1].cv, ph->ratios[0], ph->KEa,
@@ -712,6 +713,7 @@ int afq_ph_create(int nmo, int nup, int ndn, int nchol, int ndet, const int* occ
     P->Racc = dev_alloc<double>(own, 2 * nb * maxel * na);
     P->Gblk = dev_alloc<double>(own, 2 * nb * 2 * na * nmo);
     P->Emsd = dev_alloc<double>(own, 16L * maxu * nb);
+    if (maxel <= 32) P->Tinv0 = dev_alloc<double>(own, 2L * maxu * nb * maxel * maxel);
     P->p2ws_doubles = ph_2body_workspace((int)nb, (int)na, maxel, (int)K);
     if (P->p2ws_doubles > 0) P->p2ws = dev_alloc<double>(own, P->p2ws_doubles);
     P->split_elems = (long)kMaxSplits * nb * K;

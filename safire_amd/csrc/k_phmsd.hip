@@ -751,18 +751,16 @@ __global__ void k_ph0_transpose(int nmo, int nel, const cd* __restrict__ Phi, lo
   Phit[(long)w * nel * nmo + e] = Phi[(long)w * ldw + (long)k * nel + j];
 }
 
-// One workgroup per (configuration c, walker w), c fastest, so that the workgroups in flight share
-// a walker's P in L2. Gauss-Jordan on [T_c | I] with getrf's pivots (first max of |re|+|im| in
+// One workgroup per (configuration c, walker w). Gauss-Jordan on [T_c | I] with getrf's pivots (first max of |re|+|im| in
 // the column among the rows not yet eliminated; the rows at and below step k see the same
 // updates as in getrf), log det = sum_k log(+-U_kk + DBL_MIN), the sign flipped where rows were
 // swapped, imaginary part wrapped to [-pi, pi] (log_determinant_from_getrf). A configuration with
 // an exactly zero pivot gets a zero inverse (getri_or_zero) and log det = -inf, i.e. ratio 0.
-// Outputs: ld[c][w]; Emsd[c][w][4] = (E1, EXX, EJ of the same spin, sum_n KEr[c][w][n] KE[n]);
-// KE[c][w][n] when KE is given. Fixed-order sums throughout.
-__global__ __launch_bounds__(kPhThreads) void k_ph0_config(
-    int nw, int nc, int nel, int nact, int nchol, int nbv, const int* __restrict__ conf,
-    const cd* __restrict__ TAB, const cd* __restrict__ Hp, const cd* __restrict__ P, cd E0,
-    cd* __restrict__ ld, cd* __restrict__ Emsd, cd* __restrict__ KE, const cd* __restrict__ KEr) {
+// Outputs: ld[c][w], Emsd[c][w][0] = E1 and Tinv[c][w][j][b]. Fixed-order sums.
+__global__ __launch_bounds__(kPhThreads) void k_ph0_inverse(
+    int nw, int nc, int nel, int nact, const int* __restrict__ conf, const cd* __restrict__ TAB,
+    const cd* __restrict__ Hp, cd E0, cd* __restrict__ ld, cd* __restrict__ Emsd,
+    cd* __restrict__ Tinv) {
   extern __shared__ cd sm[];
   __shared__ cd red[kPhThreads / 64];
   __shared__ int sConf[kPhMaxElec];
@@ -773,8 +771,6 @@ __global__ __launch_bounds__(kPhThreads) void k_ph0_config(
   cd* sM = sm;                       // [nel][2 nel]: [T | I] -> [I | T^-1]
   cd* sRow = sM + nel * n2;          // [2 nel] the scaled pivot row
   cd* sF = sRow + n2;                // [nel] the pivot column before the update
-  cd* sPc = sF + nel;                // [nbv][nel][nel]: P[w][j][n0 + v][conf_a] as [v][a][j]
-  cd* sDg = sPc + (long)nbv * nel * nel;   // [nbv][nel]: Y^n[a][a]
   if (t < nel) sConf[t] = min(max(conf[(long)c * nel + t], 0), nact - 1);
   __syncthreads();
   const cd* TABw = TAB + (long)w * nel * nact;
@@ -857,90 +853,145 @@ __global__ __launch_bounds__(kPhThreads) void k_ph0_config(
     const int a = e / nel, j = e % nel;
     e1 += Ti[j * n2 + a] * Hw[(long)j * nact + sConf[a]];
   }
-  // Cholesky terms, nbv vectors at a time
-  const cd* Pw = P + (long)w * nel * nchol * nact;
-  const int npair = nel * (nel + 1) / 2;
-  cd exx = cmk(0.0, 0.0), ej = cmk(0.0, 0.0), cross = cmk(0.0, 0.0);
-  for (int n0 = 0; n0 < nchol; n0 += nbv) {
-    const int nv = min(nbv, nchol - n0);
-    __syncthreads();                 // the previous chunk's readers are done
-    for (int e = t; e < nv * nel * nel; e += kPhThreads) {
-      const int a = e % nel, j = (e / nel) % nel, v = e / (nel * nel);
-      sPc[(v * nel + a) * nel + j] = Pw[((long)j * nchol + n0 + v) * nact + sConf[a]];
-    }
-    __syncthreads();
-    for (int e = t; e < nv * npair; e += kPhThreads) {
-      const int v = e / npair;
-      int q = e % npair, a = 0;
-      while (q >= nel - a) {
-        q -= nel - a;
-        ++a;
-      }
-      const int b = a + q;
-      const cd* pa = sPc + (v * nel + a) * nel;
-      const cd* pb = sPc + (v * nel + b) * nel;
-      cd yab = cmk(0.0, 0.0), yba = cmk(0.0, 0.0);
-      for (int j = 0; j < nel; ++j) {
-        yab += pa[j] * Ti[j * n2 + b];
-        yba += pb[j] * Ti[j * n2 + a];
-      }
-      if (a == b) {
-        exx += yab * yab;
-        sDg[v * nel + a] = yab;
-      } else {
-        exx += 2.0 * (yab * yba);
-      }
-    }
-    __syncthreads();
-    if (t < nv) {
-      cd kv = cmk(0.0, 0.0);
-      for (int a = 0; a < nel; ++a) kv += sDg[t * nel + a];
-      ej += kv * kv;
-      const long o = ((long)c * nw + w) * nchol + n0 + t;
-      if (KE) KE[o] = kv;
-      if (KEr) cross += KEr[o] * kv;
-    }
-  }
+  cd* To = Tinv + ((long)c * nw + w) * nel * nel;
+  for (int e = t; e < nel * nel; e += kPhThreads) To[e] = Ti[(e / nel) * n2 + e % nel];
   const cd s1 = block_sum(e1, red);
-  const cd sx = block_sum(exx, red);
-  const cd sj = block_sum(ej, red);
-  const cd sc = block_sum(cross, red);
-  if (t == 0) {
+  if (t == 0) Emsd[((long)c * nw + w) * 4] = s1 + E0;
+}
+
+__device__ inline void ph0_wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ inline cd ph0_wave_sum(cd v) {
+  for (int off = 32; off > 0; off >>= 1) {
+    v.re += __shfl_xor(v.re, off, 64);
+    v.im += __shfl_xor(v.im, off, 64);
+  }
+  return v;
+}
+
+// The Cholesky terms of one (configuration c, walker w), one wavefront: lane (v, a) forms row a of
+// Y^n for vector n = n0 + v, y[b] = sum_j P[w][j][n][conf_a] Tinv[j][b], with Tinv in LDS (every
+// lane reads the same element: a broadcast) and P from memory (L2: the workgroups in flight,
+// configuration fastest, share the walker's P); the rows go to LDS once so that each lane also
+// reads column a for sum_b Y[a][b] Y[b][a], and lane (v, 0) sums the diagonal (KE[n]). Emsd[c][w]
+// [1..3] = (EXX, same-spin EJ, sum_n KEr[c][w][n] KE[n]); KE[c][w][n] when KE is given. Sums in a
+// fixed order.
+template <int NELB>
+__global__ __launch_bounds__(64) void k_ph0_chol(int nw, int nc, int nel, int nact, int nchol,
+                                                 int nvc, const int* __restrict__ conf,
+                                                 const cd* __restrict__ Tinv,
+                                                 const cd* __restrict__ P,
+                                                 cd* __restrict__ Emsd, cd* __restrict__ KE,
+                                                 const cd* __restrict__ KEr) {
+  extern __shared__ cd sm[];
+  const int lane = threadIdx.x;
+  const int c = blockIdx.x % nc, w = blockIdx.x / nc;
+  cd* sTi = sm;                      // [nel][nel]
+  cd* sY = sm + nel * nel;           // [nvc][nel][nel]
+  const cd* Tw = Tinv + ((long)c * nw + w) * nel * nel;
+  for (int e = lane; e < nel * nel; e += 64) sTi[e] = Tw[e];
+  const int v = lane / nel, a = lane % nel;
+  const bool act = lane < nvc * nel;
+  const int ca = act ? min(max(conf[(long)c * nel + a], 0), nact - 1) : 0;
+  ph0_wave_sync();
+  const cd* Pw = P + (long)w * nel * nchol * nact;
+  cd exx = cmk(0.0, 0.0), ej = cmk(0.0, 0.0), cross = cmk(0.0, 0.0);
+  for (int n0 = 0; n0 < nchol; n0 += nvc) {
+    const int n = n0 + v;
+    const bool on = act && n < nchol;
+    cd y[NELB];
+#pragma unroll
+    for (int b = 0; b < NELB; ++b) y[b] = cmk(0.0, 0.0);
+    if (on) {
+      const cd* pj = Pw + (long)n * nact + ca;
+#pragma unroll
+      for (int j = 0; j < NELB; ++j) {
+        if (j < nel) {
+          const cd p = pj[(long)j * nchol * nact];
+          const cd* ti = sTi + j * nel;
+#pragma unroll
+          for (int b = 0; b < NELB; ++b)
+            if (b < nel) y[b] += p * ti[b];
+        }
+      }
+#pragma unroll
+      for (int b = 0; b < NELB; ++b)
+        if (b < nel) sY[(v * nel + a) * nel + b] = y[b];
+    }
+    ph0_wave_sync();
+    if (on) {
+      cd sx = cmk(0.0, 0.0);
+#pragma unroll
+      for (int b = 0; b < NELB; ++b)
+        if (b < nel) sx += y[b] * sY[(v * nel + b) * nel + a];
+      exx += sx;
+      if (a == 0) {
+        cd kv = cmk(0.0, 0.0);
+        for (int d = 0; d < nel; ++d) kv += sY[(v * nel + d) * nel + d];
+        ej += kv * kv;
+        const long o = ((long)c * nw + w) * nchol + n;
+        if (KE) KE[o] = kv;
+        if (KEr) cross += KEr[o] * kv;
+      }
+    }
+    ph0_wave_sync();                 // every lane has read sY before the next pass writes it
+  }
+  exx = ph0_wave_sum(exx);
+  ej = ph0_wave_sum(ej);
+  cross = ph0_wave_sum(cross);
+  if (lane == 0) {
     cd* out = Emsd + ((long)c * nw + w) * 4;
-    out[0] = s1 + E0;
-    out[1] = -0.5 * sx;
-    out[2] = 0.5 * sj;
-    out[3] = sc;
+    out[1] = -0.5 * exx;
+    out[2] = 0.5 * ej;
+    out[3] = cross;
   }
 }
 
-// ld[c][w] -> ratio exp(ld[c][w] - ld[0][w]) in place (row 0 -> 1), log_ov[w] += ld[0][w]
-// (PHMSD.cpp:245-252, 364-369)
-__global__ void k_ph0_ratios(int nw, int nc, cd* __restrict__ r, cd* __restrict__ log_ov) {
+// ld[c][w] -> ratio exp(ld[c][w] - ld[0][w]) for c >= 1 (PHMSD.cpp:245-252); row 0 is read here and
+// replaced afterwards by k_ph0_ratio_ref
+__global__ void k_ph0_ratios(int nw, int nc, cd* __restrict__ r) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x + nw;
+  if (e >= (long)nc * nw) return;
+  r[e] = cexp_(r[e] - r[e % nw]);
+}
+
+// log_ov[w] += ld[0][w], row 0 -> 1 (PHMSD.cpp:245-248, 364-369)
+__global__ void k_ph0_ratio_ref(int nw, cd* __restrict__ r, cd* __restrict__ log_ov) {
   const int w = blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= nw) return;
-  const cd l0 = r[w];
-  for (int c = nc - 1; c >= 1; --c) r[(long)c * nw + w] = cexp_(r[(long)c * nw + w] - l0);
+  log_ov[w] += r[w];
   r[w] = cmk(1.0, 0.0);
-  log_ov[w] += l0;
 }
 
 // acc[w][k] = sum_u wgt0[u][w] Em0[u][w][k] + sum_d wgt1[d][w] Em1[d][w][k]
 //             (+ sum_d r1[d][w] Em1[d][w][3] for k = 2: the opposite-spin Coulomb term)
-// (PHMSD.cpp:301-359)
-__global__ void k_ph0_assemble(int nw, int nu, int nd, const cd* __restrict__ wgt0,
-                               const cd* __restrict__ Em0, const cd* __restrict__ wgt1,
-                               const cd* __restrict__ Em1, const cd* __restrict__ r1,
-                               cd* __restrict__ acc) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= 3 * nw) return;
-  const int w = e / 3, k = e % 3;
-  cd v = cmk(0.0, 0.0);
-  for (int u = 0; u < nu; ++u) v += wgt0[(long)u * nw + w] * Em0[((long)u * nw + w) * 4 + k];
-  for (int d = 0; d < nd; ++d) v += wgt1[(long)d * nw + w] * Em1[((long)d * nw + w) * 4 + k];
-  if (k == 2)
-    for (int d = 0; d < nd; ++d) v += r1[(long)d * nw + w] * Em1[((long)d * nw + w) * 4 + 3];
-  acc[e] = v;
+// (PHMSD.cpp:301-359); one workgroup per walker, threads over configurations, fixed-order sums.
+__global__ __launch_bounds__(kPhThreads) void k_ph0_assemble(
+    int nw, int nu, int nd, const cd* __restrict__ wgt0, const cd* __restrict__ Em0,
+    const cd* __restrict__ wgt1, const cd* __restrict__ Em1, const cd* __restrict__ r1,
+    cd* __restrict__ acc) {
+  __shared__ cd red[kPhThreads / 64];
+  const int w = blockIdx.x, t = threadIdx.x;
+  cd v[3] = {cmk(0.0, 0.0), cmk(0.0, 0.0), cmk(0.0, 0.0)};
+  for (int u = t; u < nu; u += kPhThreads) {
+    const cd g = wgt0[(long)u * nw + w];
+    const cd* e = Em0 + ((long)u * nw + w) * 4;
+    for (int k = 0; k < 3; ++k) v[k] += g * e[k];
+  }
+  for (int d = t; d < nd; d += kPhThreads) {
+    const cd g = wgt1[(long)d * nw + w];
+    const cd* e = Em1 + ((long)d * nw + w) * 4;
+    for (int k = 0; k < 3; ++k) v[k] += g * e[k];
+    v[2] += r1[(long)d * nw + w] * e[3];
+  }
+  for (int k = 0; k < 3; ++k) {
+    const cd sk = block_sum(v[k], red);
+    if (t == 0) acc[3L * w + k] = sk;
+  }
 }
 
 }  // namespace
@@ -1214,28 +1265,43 @@ void launch_ph0_products(hipStream_t s, int nw, int nmo, int nel, int nact, int 
                    (long)nchol * nact);
 }
 
-int ph0_vectors_per_chunk(int nel) { return std::max(1, std::min(16, 1536 / (nel * nel))); }
-
 void launch_ph0_config(hipStream_t s, int nw, int nc, int nel, int nact, int nchol,
                        const int* conf, const double* TAB, const double* Hp, const double* P,
-                       double E0, double* ld, double* Emsd, double* KE, const double* KEr) {
+                       double E0, double* ld, double* Emsd, double* KE, const double* KEr,
+                       double* Tinv) {
   if (nw <= 0 || nc <= 0) return;
   AFQ_CHECK(nel >= 1 && nel <= 32, AFQ_EUNSUPPORTED,
             "PHMSD reference energy: more than 32 electrons of one spin");
-  const int nbv = ph0_vectors_per_chunk(nel);
-  const size_t lds = sizeof(cd) * ((size_t)nel * 2 * nel + 2 * nel + nel +
-                                   (size_t)nbv * nel * nel + (size_t)nbv * nel);
-  hipLaunchKernelGGL(k_ph0_config, dim3((unsigned)((long)nc * nw)), dim3(kPhThreads), lds, s, nw,
-                     nc, nel, nact, nchol, nbv, conf, reinterpret_cast<const cd*>(TAB),
-                     reinterpret_cast<const cd*>(Hp), reinterpret_cast<const cd*>(P),
-                     cmk(E0, 0.0), reinterpret_cast<cd*>(ld), reinterpret_cast<cd*>(Emsd),
-                     reinterpret_cast<cd*>(KE), reinterpret_cast<const cd*>(KEr));
+  const long blocks = (long)nc * nw;
+  const size_t lds = sizeof(cd) * ((size_t)nel * 2 * nel + 3 * nel);
+  hipLaunchKernelGGL(k_ph0_inverse, dim3((unsigned)blocks), dim3(kPhThreads), lds, s, nw, nc, nel,
+                     nact, conf, reinterpret_cast<const cd*>(TAB),
+                     reinterpret_cast<const cd*>(Hp), cmk(E0, 0.0), reinterpret_cast<cd*>(ld),
+                     reinterpret_cast<cd*>(Emsd), reinterpret_cast<cd*>(Tinv));
+  AFQ_HIP(hipGetLastError());
+  const int nvc = 64 / nel;
+  const size_t lds2 = sizeof(cd) * ((size_t)nel * nel + (size_t)nvc * nel * nel);
+  auto args = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(64), lds2, s, nw, nc, nel, nact, nchol,
+                       nvc, conf, reinterpret_cast<const cd*>(Tinv),
+                       reinterpret_cast<const cd*>(P), reinterpret_cast<cd*>(Emsd),
+                       reinterpret_cast<cd*>(KE), reinterpret_cast<const cd*>(KEr));
+  };
+  if (nel <= 8) args(k_ph0_chol<8>);
+  else if (nel <= 12) args(k_ph0_chol<12>);
+  else if (nel <= 16) args(k_ph0_chol<16>);
+  else args(k_ph0_chol<32>);
   AFQ_HIP(hipGetLastError());
 }
 
 void launch_ph0_ratios(hipStream_t s, int nw, int nc, double* r, double* log_ov) {
   if (nw <= 0) return;
-  hipLaunchKernelGGL(k_ph0_ratios, ph_blocks(nw), dim3(256), 0, s, nw, nc,
+  if (nc > 1) {
+    hipLaunchKernelGGL(k_ph0_ratios, ph_blocks((long)(nc - 1) * nw), dim3(256), 0, s, nw, nc,
+                       reinterpret_cast<cd*>(r));
+    AFQ_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_ph0_ratio_ref, ph_blocks(nw), dim3(256), 0, s, nw,
                      reinterpret_cast<cd*>(r), reinterpret_cast<cd*>(log_ov));
   AFQ_HIP(hipGetLastError());
 }
@@ -1244,7 +1310,7 @@ void launch_ph0_assemble(hipStream_t s, int nw, int nu, int nd, const double* wg
                          const double* Em0, const double* wgt1, const double* Em1,
                          const double* r1, double* acc) {
   if (nw <= 0) return;
-  hipLaunchKernelGGL(k_ph0_assemble, ph_blocks(3L * nw), dim3(256), 0, s, nw, nu, nd,
+  hipLaunchKernelGGL(k_ph0_assemble, dim3(nw), dim3(kPhThreads), 0, s, nw, nu, nd,
                      reinterpret_cast<const cd*>(wgt0), reinterpret_cast<const cd*>(Em0),
                      reinterpret_cast<const cd*>(wgt1), reinterpret_cast<const cd*>(Em1),
                      reinterpret_cast<const cd*>(r1), reinterpret_cast<cd*>(acc));
