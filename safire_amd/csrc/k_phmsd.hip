@@ -874,8 +874,8 @@ __device__ inline cd ph0_wave_sum(cd v) {
 }
 
 // The Cholesky terms of one (configuration c, walker w), one wavefront: lane (v, a) forms row a of
-// Y^n for vector n = n0 + v, y[b] = sum_j P[w][j][n][conf_a] Tinv[j][b], with Tinv in LDS (every
-// lane reads the same element: a broadcast) and P from memory (L2: the workgroups in flight,
+// Y^n for vector n = n0 + v, y[b] = sum_j P[w][j][n][conf_a] Tinv[j][b], with Tinv read by scalar
+// loads (every lane needs the same element) and P from memory (L2: the workgroups in flight,
 // configuration fastest, share the walker's P); the rows go to LDS once so that each lane also
 // reads column a for sum_b Y[a][b] Y[b][a], and lane (v, 0) sums the diagonal (KE[n]). Emsd[c][w]
 // [1..3] = (EXX, same-spin EJ, sum_n KEr[c][w][n] KE[n]); KE[c][w][n] when KE is given. Sums in a
@@ -890,10 +890,9 @@ __global__ __launch_bounds__(64) void k_ph0_chol(int nw, int nc, int nel, int na
   extern __shared__ cd sm[];
   const int lane = threadIdx.x;
   const int c = blockIdx.x % nc, w = blockIdx.x / nc;
-  cd* sTi = sm;                      // [nel][nel]
-  cd* sY = sm + nel * nel;           // [nvc][nel][nel]
-  const cd* Tw = Tinv + ((long)c * nw + w) * nel * nel;
-  for (int e = lane; e < nel * nel; e += 64) sTi[e] = Tw[e];
+  cd* sY = sm;                       // [nvc][nel][nel]
+  // uniform across the wavefront and read-only: scalar loads, used as scalar FMA operands
+  const cd* __restrict__ Tw = Tinv + ((long)c * nw + w) * nel * nel;
   const int v = lane / nel, a = lane % nel;
   const bool act = lane < nvc * nel;
   const int ca = act ? min(max(conf[(long)c * nel + a], 0), nact - 1) : 0;
@@ -912,7 +911,7 @@ __global__ __launch_bounds__(64) void k_ph0_chol(int nw, int nc, int nel, int na
       for (int j = 0; j < NELB; ++j) {
         if (j < nel) {
           const cd p = pj[(long)j * nchol * nact];
-          const cd* ti = sTi + j * nel;
+          const cd* ti = Tw + j * nel;
 #pragma unroll
           for (int b = 0; b < NELB; ++b)
             if (b < nel) y[b] += p * ti[b];
@@ -1280,7 +1279,7 @@ void launch_ph0_config(hipStream_t s, int nw, int nc, int nel, int nact, int nch
                      reinterpret_cast<cd*>(Emsd), reinterpret_cast<cd*>(Tinv));
   AFQ_HIP(hipGetLastError());
   const int nvc = 64 / nel;
-  const size_t lds2 = sizeof(cd) * ((size_t)nel * nel + (size_t)nvc * nel * nel);
+  const size_t lds2 = sizeof(cd) * (size_t)nvc * nel * nel;
   auto args = [&](auto kern) {
     hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(64), lds2, s, nw, nc, nel, nact, nchol,
                        nvc, conf, reinterpret_cast<const cd*>(Tinv),
