@@ -751,18 +751,32 @@ __global__ void k_ph0_transpose(int nmo, int nel, const cd* __restrict__ Phi, lo
   Phit[(long)w * nel * nmo + e] = Phi[(long)w * ldw + (long)k * nel + j];
 }
 
-// One workgroup per (configuration c, walker w). Gauss-Jordan on [T_c | I] with getrf's pivots (first max of |re|+|im| in
-// the column among the rows not yet eliminated; the rows at and below step k see the same
-// updates as in getrf), log det = sum_k log(+-U_kk + DBL_MIN), the sign flipped where rows were
-// swapped, imaginary part wrapped to [-pi, pi] (log_determinant_from_getrf). A configuration with
-// an exactly zero pivot gets a zero inverse (getri_or_zero) and log det = -inf, i.e. ratio 0.
+__device__ inline void ph0_wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ inline cd ph0_wave_sum(cd v) {
+  for (int off = 32; off > 0; off >>= 1) {
+    v.re += __shfl_xor(v.re, off, 64);
+    v.im += __shfl_xor(v.im, off, 64);
+  }
+  return v;
+}
+
+// One wavefront per (configuration c, walker w). Gauss-Jordan on [T_c | I] with getrf's pivots
+// (first max of |re|+|im| in the column among the rows not yet eliminated; the rows at and below
+// step k see the same updates as in getrf), log det = sum_k log(+-U_kk + DBL_MIN), the sign
+// flipped where rows were swapped, imaginary part wrapped to [-pi, pi]
+// (log_determinant_from_getrf). A configuration with an exactly zero pivot gets a zero inverse
+// (getri_or_zero) and log det = -inf, i.e. ratio 0.
 // Outputs: ld[c][w], Emsd[c][w][0] = E1 and Tinv[c][w][j][b]. Fixed-order sums.
-__global__ __launch_bounds__(kPhThreads) void k_ph0_inverse(
+__global__ __launch_bounds__(64) void k_ph0_inverse(
     int nw, int nc, int nel, int nact, const int* __restrict__ conf, const cd* __restrict__ TAB,
     const cd* __restrict__ Hp, cd E0, cd* __restrict__ ld, cd* __restrict__ Emsd,
     cd* __restrict__ Tinv) {
   extern __shared__ cd sm[];
-  __shared__ cd red[kPhThreads / 64];
   __shared__ int sConf[kPhMaxElec];
   __shared__ int sPiv, sSing;
   const int t = threadIdx.x;
@@ -774,7 +788,7 @@ __global__ __launch_bounds__(kPhThreads) void k_ph0_inverse(
   if (t < nel) sConf[t] = min(max(conf[(long)c * nel + t], 0), nact - 1);
   __syncthreads();
   const cd* TABw = TAB + (long)w * nel * nact;
-  for (int e = t; e < nel * n2; e += kPhThreads) {
+  for (int e = t; e < nel * n2; e += 64) {
     const int i = e / n2, j = e % n2;
     sM[e] = j < nel ? TABw[(long)j * nact + sConf[i]] : cmk(j - nel == i ? 1.0 : 0.0, 0.0);
   }
@@ -782,7 +796,7 @@ __global__ __launch_bounds__(kPhThreads) void k_ph0_inverse(
   cd logd = cmk(0.0, 0.0);           // thread 0
   bool sing = false;                 // uniform: read from LDS after a barrier
   for (int k = 0; k < nel; ++k) {
-    if (t < 64) {
+    {
       double v = -1.0;
       int idx = nel;
       if (t >= k && t < nel) {
@@ -810,7 +824,7 @@ __global__ __launch_bounds__(kPhThreads) void k_ph0_inverse(
       break;
     }
     if (p != k)
-      for (int e = t; e < n2; e += kPhThreads) {
+      for (int e = t; e < n2; e += 64) {
         const cd x = sM[k * n2 + e];
         sM[k * n2 + e] = sM[p * n2 + e];
         sM[p * n2 + e] = x;
@@ -822,17 +836,17 @@ __global__ __launch_bounds__(kPhThreads) void k_ph0_inverse(
       logd += clog_(cmk(u.re + DBL_MIN, u.im));
     }
     const cd inv = cdiv(cmk(1.0, 0.0), piv);
-    for (int e = t; e < n2; e += kPhThreads) sRow[e] = sM[k * n2 + e] * inv;
-    for (int e = t; e < nel; e += kPhThreads) sF[e] = sM[e * n2 + k];
+    for (int e = t; e < n2; e += 64) sRow[e] = sM[k * n2 + e] * inv;
+    for (int e = t; e < nel; e += 64) sF[e] = sM[e * n2 + k];
     __syncthreads();
-    for (int e = t; e < nel * n2; e += kPhThreads) {
+    for (int e = t; e < nel * n2; e += 64) {
       const int r = e / n2, col = e % n2;
       sM[e] = r == k ? sRow[col] : sM[e] - sF[r] * sRow[col];
     }
     __syncthreads();
   }
   if (sing) {
-    for (int e = t; e < nel * nel; e += kPhThreads) sM[(e / nel) * n2 + nel + e % nel] = cmk(0, 0);
+    for (int e = t; e < nel * nel; e += 64) sM[(e / nel) * n2 + nel + e % nel] = cmk(0, 0);
     __syncthreads();
   }
   if (t == 0) {
@@ -849,28 +863,14 @@ __global__ __launch_bounds__(kPhThreads) void k_ph0_inverse(
   // one-body term
   cd e1 = cmk(0.0, 0.0);
   const cd* Hw = Hp + (long)w * nel * nact;
-  for (int e = t; e < nel * nel; e += kPhThreads) {
+  for (int e = t; e < nel * nel; e += 64) {
     const int a = e / nel, j = e % nel;
     e1 += Ti[j * n2 + a] * Hw[(long)j * nact + sConf[a]];
   }
   cd* To = Tinv + ((long)c * nw + w) * nel * nel;
-  for (int e = t; e < nel * nel; e += kPhThreads) To[e] = Ti[(e / nel) * n2 + e % nel];
-  const cd s1 = block_sum(e1, red);
+  for (int e = t; e < nel * nel; e += 64) To[e] = Ti[(e / nel) * n2 + e % nel];
+  const cd s1 = ph0_wave_sum(e1);
   if (t == 0) Emsd[((long)c * nw + w) * 4] = s1 + E0;
-}
-
-__device__ inline void ph0_wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-__device__ inline cd ph0_wave_sum(cd v) {
-  for (int off = 32; off > 0; off >>= 1) {
-    v.re += __shfl_xor(v.re, off, 64);
-    v.im += __shfl_xor(v.im, off, 64);
-  }
-  return v;
 }
 
 // The Cholesky terms of one (configuration c, walker w), one wavefront: lane (v, a) forms row a of
@@ -1273,7 +1273,7 @@ void launch_ph0_config(hipStream_t s, int nw, int nc, int nel, int nact, int nch
             "PHMSD reference energy: more than 32 electrons of one spin");
   const long blocks = (long)nc * nw;
   const size_t lds = sizeof(cd) * ((size_t)nel * 2 * nel + 3 * nel);
-  hipLaunchKernelGGL(k_ph0_inverse, dim3((unsigned)blocks), dim3(kPhThreads), lds, s, nw, nc, nel,
+  hipLaunchKernelGGL(k_ph0_inverse, dim3((unsigned)blocks), dim3(64), lds, s, nw, nc, nel,
                      nact, conf, reinterpret_cast<const cd*>(TAB),
                      reinterpret_cast<const cd*>(Hp), cmk(E0, 0.0), reinterpret_cast<cd*>(ld),
                      reinterpret_cast<cd*>(Emsd), reinterpret_cast<cd*>(Tinv));
