@@ -880,14 +880,15 @@ __global__ __launch_bounds__(64) void k_ph0_inverse(
 // reads column a for sum_b Y[a][b] Y[b][a], and lane (v, 0) sums the diagonal (KE[n]). Emsd[c][w]
 // [1..3] = (EXX, same-spin EJ, sum_n KEr[c][w][n] KE[n]); KE[c][w][n] when KE is given. Sums in a
 // fixed order.
-template <int NELB>
-__global__ __launch_bounds__(64) void k_ph0_chol(int nw, int nc, int nel, int nact, int nchol,
+template <int NELB, bool EXACT>
+__global__ __launch_bounds__(64) void k_ph0_chol(int nw, int nc, int nel_, int nact, int nchol,
                                                  int nvc, const int* __restrict__ conf,
                                                  const cd* __restrict__ Tinv,
                                                  const cd* __restrict__ P,
                                                  cd* __restrict__ Emsd, cd* __restrict__ KE,
                                                  const cd* __restrict__ KEr) {
   extern __shared__ cd sm[];
+  const int nel = EXACT ? NELB : nel_;   // a compile-time count when EXACT
   const int lane = threadIdx.x;
   const int c = blockIdx.x % nc, w = blockIdx.x / nc;
   cd* sY = sm;                       // [nvc][nel][nel]
@@ -1286,10 +1287,11 @@ void launch_ph0_config(hipStream_t s, int nw, int nc, int nel, int nact, int nch
                        reinterpret_cast<const cd*>(P), reinterpret_cast<cd*>(Emsd),
                        reinterpret_cast<cd*>(KE), reinterpret_cast<const cd*>(KEr));
   };
-  if (nel <= 8) args(k_ph0_chol<8>);
-  else if (nel <= 12) args(k_ph0_chol<12>);
-  else if (nel <= 16) args(k_ph0_chol<16>);
-  else args(k_ph0_chol<32>);
+  if (nel == 10) args(k_ph0_chol<10, true>);   // a compile-time count for the common case
+  else if (nel <= 8) args(k_ph0_chol<8, false>);
+  else if (nel <= 12) args(k_ph0_chol<12, false>);
+  else if (nel <= 16) args(k_ph0_chol<16, false>);
+  else args(k_ph0_chol<32, false>);
   AFQ_HIP(hipGetLastError());
 }
 
