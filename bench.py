@@ -1,10 +1,12 @@
 #!/usr/bin/env python3
 """Walker-step throughput on one MI355X: the phaseless-AFQMC step with a Cholesky Hamiltonian.
 
-Workload (default): NMO=64, 16 up + 16 down electrons, Nchol=320, 512 walkers, dt=0.005, the
-local-energy step (G -> vbias -> X -> vHS -> P1 exp(iV) P1 -> G, E1, EXX, EJ -> weights) with
-QR re-orthogonalisation every 5 steps. Synthetic seeded inputs; everything resident in HBM
-before the timed region.
+Workload (default): NMO=128, 32 up + 32 down electrons, Nchol=640 (Cholesky vectors decaying as
+exp(-n/64)), 1024 walkers, dt=0.005, the local-energy step (G -> vbias -> X -> vHS ->
+P1 exp(iV) P1 -> G, E1, EXX, EJ -> weights) with QR re-orthogonalisation every 5 steps: the
+largest single-system workload. The NMO=64 / 16+16 / Nchol=320 / 512-walker workload runs after
+it (--extra) and its results are added under its name. Synthetic seeded inputs; everything
+resident in HBM before the timed region.
 
 Every precision form of the same step is timed (--forms): the plain FP64 form and the mixed
 forms, which change only the Cholesky contractions (vbias and vHS in FP32; the exchange in FP32,
@@ -44,7 +46,8 @@ GATE_HA = 1e-5
 # scaled by a copy kernel of known size), as recorded in profiles/walker_step_c2_hbm_traffic.md;
 # null for a workload or form without such a measurement
 TRAFFIC_BYTES = {("c2", "int8"): 38.54e6, ("c2", "fp64"): 53.29e6}
-TRAFFIC_SOURCE = "profiles/walker_step_c2_hbm_traffic.md"
+TRAFFIC_SOURCE = {"c2": "profiles/walker_step_c2_hbm_traffic.md",
+                  "c3": "profiles/walker_step_c3_hbm_traffic.md"}
 
 
 def exchange_flops(W, M, N, K):
@@ -329,58 +332,36 @@ def bench_batch(args, A, torch, synth):
     return 0
 
 
-def main(argv=None):
-    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
-    ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
-    ap.add_argument("--streams", type=int, default=4,
-                    help="c4: capture streams the systems are spread over")
-    ap.add_argument("--tile-cache", default=os.path.join(tempfile.gettempdir(),
-                                                         "safire_amd_tiles.json"),
-                    help="c4: the GEMM autotuner's JSON cache (every system's tilings are tuned)")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--forms", default="fp64,fp32,bf16x3,int8")
-    ap.add_argument("--ph-energy", default="woodbury", choices=["woodbury", "reference"],
-                    help="c5: the PHMSD energy algorithm (the reference's `algorithm` option)")
-    ap.add_argument("--no-hybrid", action="store_true",
-                    help="skip the hybrid-weight (no per-step local energy) leg")
-    ap.add_argument("--no-graph", action="store_true",
-                    help="time the eager launches instead of one captured graph of the steps")
-    args = ap.parse_args(argv)
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU: the walker step runs on a single MI355X here and "
-              f"--gpus {args.gpus} is not supported (walkers would shard across GPUs with no "
-              f"communication inside a step).", file=sys.stderr)
-        return 2
+FORM_PRECISION = {"fp64": 0, "fp32": 1, "bf16x3": 2, "int8": 3}
 
+
+def bench_single(args, A, torch, synth, workload, with_cpu, with_hybrid):
+    """One single-system workload (c1, c2 or c3): every requested form timed, the accuracy gate,
+    the roofline of the exchange kernel of the headline form; returns the JSON line as a dict.
+    A form the shape does not support (AFQ_EUNSUPPORTED) is reported as such and skipped."""
     import numpy as np
-    import torch
-
-    from safire_amd import afqmc as A
-    from safire_amd import synth
-
-    if args.workload == "c4":
-        return bench_batch(args, A, torch, synth)
-    if args.workload == "c5":
-        return bench_phmsd(args, A, torch, synth)
-    s, W, _, ortho = synth.workload(args.workload)
+    s, W, _, ortho = synth.workload(workload)
     M, N, K = s.M, s.N, s.K
     nst = max(args.warmup + args.steps, 31)   # the accuracy gate replays 31 steps
     t0 = time.perf_counter()
     xi = synth.fields(s.seed, nst, W, K)
     xi_dev = torch.from_numpy(xi).to("cuda")
+    del xi
     print(f"[bench] {s.name}: M={M} N={N} K={K} W={W}, fields ready in "
           f"{time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
 
     uniforms = np.random.default_rng([s.seed, 31]).random(nst)
-    forms = {}
+    forms, unsupported = {}, {}
     for f in args.forms.split(","):
-        prec = {"fp64": A.FP64, "fp32": A.FP32, "bf16x3": A.BF16X3, "int8": A.INT8_SPLIT}[f]
-        forms[f] = run_form(A, torch, s, W, xi_dev, uniforms, args.warmup, args.steps, ortho,
-                            prec, hybrid=False, graph=not args.no_graph)
+        try:
+            forms[f] = run_form(A, torch, s, W, xi_dev, uniforms, args.warmup, args.steps, ortho,
+                                FORM_PRECISION[f], hybrid=False, graph=not args.no_graph)
+        except A.AfqError as e:
+            if e.code != 5:   # AFQ_EUNSUPPORTED: the form has no kernel for this shape
+                raise
+            unsupported[f] = str(e)
+            print(f"[bench] {f}: not supported at this shape ({e})", file=sys.stderr, flush=True)
+            continue
         r = forms[f]
         print(f"[bench] {f}: {r['ms'] / args.steps:.3f} ms/step "
               f"({'graph' if r['graph'] is not None else 'eager'}; eager "
@@ -428,11 +409,11 @@ def main(argv=None):
                   f"live walker states; relative error of vbias {cerr[f]['vbias']:.2e}, "
                   f"vHS {cerr[f]['vHS']:.2e} (G is FP64 in every form)", file=sys.stderr,
                   flush=True)
+        del states, run
     passing = [f for f in forms if f == "fp64" or gate.get(f, 1.0) < GATE_HA]
     # the headline: the fastest form that meets the accuracy gate
     mixed = min(passing, key=lambda f: forms[f]["ms"])
     fastest = min(forms, key=lambda f: forms[f]["ms"])
-    dE = gate.get(mixed, 0.0) if mixed != "fp64" else 0.0
 
     head = forms[mixed]
     ms_step = head["ms"] / args.steps
@@ -441,6 +422,7 @@ def main(argv=None):
     ex_avg = ex_ms / max(ex_n, 1)
     achieved = exchange_flops(W, M, N, K) / (ex_avg / 1e3) / 1e12
     peak = PEAK_TFLOPS[mixed]
+    traffic = TRAFFIC_BYTES.get((workload, mixed))
     out = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -460,10 +442,9 @@ def main(argv=None):
                    "parallelism": "single GPU"},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak,
                      "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
-                     "traffic": TRAFFIC_BYTES.get((args.workload, mixed)),
+                     "traffic": traffic,
                      "traffic_unit": "bytes per launch",
-                     "traffic_source": (TRAFFIC_SOURCE if (args.workload, mixed) in TRAFFIC_BYTES
-                                        else None),
+                     "traffic_source": TRAFFIC_SOURCE.get(workload) if traffic else None,
                      "kernel": f"k_exchange ({mixed} MFMA)",
                      "flop_per_launch": exchange_flops(W, M, N, K),
                      "mean_launch_ms": round(ex_avg, 5),
@@ -473,9 +454,11 @@ def main(argv=None):
         v64 = W * args.steps / (ref_form["ms"] / 1e3)
         ex64 = ref_form["timers"]["exchange"]
         ex64_avg = ex64[0] / max(ex64[1], 1)
-        out["fp64"] = {"value": round(v64, 1), "ms_per_step": round(ref_form["ms"] / args.steps, 4),
-                       "exchange_tflops": round(exchange_flops(W, M, N, K) / (ex64_avg / 1e3) / 1e12, 3),
-                       "exchange_frac_fp64_peak": round(exchange_flops(W, M, N, K) / (ex64_avg / 1e3) / 1e12 / PEAK_TFLOPS["fp64"], 4)}
+        tf64 = exchange_flops(W, M, N, K) / (ex64_avg / 1e3) / 1e12
+        out["fp64"] = {"value": round(v64, 1),
+                       "ms_per_step": round(ref_form["ms"] / args.steps, 4),
+                       "exchange_tflops": round(tf64, 3),
+                       "exchange_frac_fp64_peak": round(tf64 / PEAK_TFLOPS["fp64"], 4)}
         out["mixed_over_fp64"] = round(value / v64, 3)
     out["max_abs_dEloc_Ha"] = {f: gate[f] for f in gate}
     out["contraction_rel_err_vs_fp64"] = {f: {k: float(f"{e:.3g}") for k, e in c.items()}
@@ -483,6 +466,8 @@ def main(argv=None):
     out["headline_form"] = mixed
     out["forms_walker_steps_per_s"] = {f: round(W * args.steps / (r["ms"] / 1e3), 1)
                                        for f, r in forms.items()}
+    if unsupported:
+        out["forms_unsupported"] = unsupported
     if fastest != mixed:
         out["note"] = (f"{fastest} is faster but misses |dE_loc| < {GATE_HA} Ha "
                        f"({gate[fastest]:.2e} Ha); value is the fastest form within the gate")
@@ -494,23 +479,78 @@ def main(argv=None):
     out["eager_ms_per_step"] = round(head["eager_ms"] / args.steps, 4)
     if head["graph"] is not None:
         out["graph_matches_eager"] = {f: r["graph"] for f, r in forms.items()}
-    if not args.no_hybrid:
+    if with_hybrid:
         # the reference's default step: hybrid weights, no local energy per step
         hyb = {}
         for f in sorted({"fp64", mixed}):
-            prec = {"fp64": A.FP64, "fp32": A.FP32, "bf16x3": A.BF16X3, "int8": A.INT8_SPLIT}[f]
-            r = run_form(A, torch, s, W, xi_dev, uniforms, args.warmup, args.steps, ortho, prec,
-                         hybrid=True, graph=not args.no_graph)
+            r = run_form(A, torch, s, W, xi_dev, uniforms, args.warmup, args.steps, ortho,
+                         FORM_PRECISION[f], hybrid=True, graph=not args.no_graph)
             hyb[f] = {"walker_steps_per_s": round(W * args.steps / (r["ms"] / 1e3), 1),
                       "ms_per_step": round(r["ms"] / args.steps, 4)}
             r["ham"].close()
             print(f"[bench] hybrid {f}: {r['ms'] / args.steps:.3f} ms/step", file=sys.stderr,
                   flush=True)
         out["hybrid_form"] = hyb
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(s, W, ortho, args.cpu_seconds)
     for f in forms.values():
         f["ham"].close()
+    forms.clear()
+    del xi_dev
+    torch.cuda.empty_cache()
+    if with_cpu:
+        out["cpu_baseline"] = cpu_baseline(s, W, ortho, args.cpu_seconds)
+    return out
+
+
+# keys of a secondary workload's line kept under its name in the headline line
+EXTRA_KEYS = ("value", "ms_per_step", "config", "headline_form", "fp64", "mixed_over_fp64",
+              "max_abs_dEloc_Ha", "forms_walker_steps_per_s", "forms_unsupported", "roofline",
+              "regions_ms_per_step", "graph_matches_eager")
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("--extra", default="c2", choices=["none", "c1", "c2", "c3"],
+                    help="a second single-system workload whose results are added under its "
+                         "name (not timed for the CPU baseline or the hybrid form)")
+    ap.add_argument("--streams", type=int, default=4,
+                    help="c4: capture streams the systems are spread over")
+    ap.add_argument("--tile-cache", default=os.path.join(tempfile.gettempdir(),
+                                                         "safire_amd_tiles.json"),
+                    help="c4: the GEMM autotuner's JSON cache (every system's tilings are tuned)")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--forms", default="fp64,fp32,bf16x3,int8")
+    ap.add_argument("--ph-energy", default="woodbury", choices=["woodbury", "reference"],
+                    help="c5: the PHMSD energy algorithm (the reference's `algorithm` option)")
+    ap.add_argument("--no-hybrid", action="store_true",
+                    help="skip the hybrid-weight (no per-step local energy) leg")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="time the eager launches instead of one captured graph of the steps")
+    args = ap.parse_args(argv)
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: the walker step runs on a single MI355X here and "
+              f"--gpus {args.gpus} is not supported (walkers would shard across GPUs with no "
+              f"communication inside a step).", file=sys.stderr)
+        return 2
+
+    import torch
+
+    from safire_amd import afqmc as A
+    from safire_amd import synth
+
+    if args.workload == "c4":
+        return bench_batch(args, A, torch, synth)
+    if args.workload == "c5":
+        return bench_phmsd(args, A, torch, synth)
+    out = bench_single(args, A, torch, synth, args.workload, with_cpu=not args.no_cpu,
+                       with_hybrid=not args.no_hybrid)
+    if args.extra not in ("none", args.workload):
+        ex = bench_single(args, A, torch, synth, args.extra, with_cpu=False, with_hybrid=False)
+        out[args.extra] = {k: ex[k] for k in EXTRA_KEYS if k in ex}
     print(json.dumps(out), flush=True)
     return 0
 
