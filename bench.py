@@ -45,7 +45,8 @@ GATE_HA = 1e-5
 # HBM bytes per launch of each form's exchange kernel (FETCH_SIZE and WRITE_SIZE counter runs
 # scaled by a copy kernel of known size), as recorded in profiles/walker_step_c2_hbm_traffic.md;
 # null for a workload or form without such a measurement
-TRAFFIC_BYTES = {("c2", "int8"): 38.54e6, ("c2", "fp64"): 53.29e6}
+TRAFFIC_BYTES = {("c2", "int8"): 38.54e6, ("c2", "fp64"): 53.29e6,
+                 ("c3", "int8"): 5946.18e6, ("c3", "fp64"): 21822.38e6}
 TRAFFIC_SOURCE = {"c2": "profiles/walker_step_c2_hbm_traffic.md",
                   "c3": "profiles/walker_step_c3_hbm_traffic.md"}
 
@@ -370,10 +371,11 @@ def bench_single(args, A, torch, synth, workload, with_cpu, with_hybrid):
               + ", ".join(f"{k}={v[0]:.2f}" for k, v in r["timers"].items() if v[1]),
               file=sys.stderr, flush=True)
 
-    # accuracy of each reduced-precision form on identical walker states: the seeded trial state
-    # and the states right after population control at steps 10, 20 and 30 of an (untimed) FP64
-    # run, over the walkers whose weight is not zero (a walker killed by the phaseless
-    # projection keeps weight 0 and enters no estimator)
+    # accuracy of each reduced-precision form on identical walker states: the seeded trial state,
+    # the states after steps 1 and 3 and those right after population control at steps 10, 20
+    # and 30 of an (untimed) FP64 run, over the walkers whose weight is not zero (a walker killed
+    # by the phaseless projection keeps weight 0 and enters no estimator; on the NMO=128
+    # Hamiltonian most walkers are killed within a few steps, hence the early states)
     ref_form = forms.get("fp64")
     gate = {}
     cerr = {}
@@ -386,7 +388,7 @@ def bench_single(args, A, torch, synth, workload, with_cpu, with_hybrid):
         prop.seed(run)
         for t in range(31):
             run_loop_step(prop, run, xi_dev[t], ref_form["Eshift"], t, ortho, uniforms)
-            if t in (10, 20, 30):
+            if t in (1, 3, 10, 20, 30):
                 states.append(A.WalkerSet(ham64, W, run.host()))
         for f in forms:
             if f == "fp64":

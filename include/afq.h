@@ -192,10 +192,26 @@ int afq_orthogonalize(afq_ham ham, int W, double* d_wbuf, long ldw, void* stream
  * used by AFQMCBasePropagator::Propagate at AFQMCBasePropagator.cpp:341-344, 387-390): with
  * nfields > 0 every afq_step stores its assembled fields X [W][K] into slot (position % nfields) of
  * the caller's ring d_fields [W][nfields][K] c128 (the reference's Fields(nwalk, steps, nCV)) and
- * advances the position, which lives on the GPU (a captured step replays it). Resets the position
- * to 0; nfields = 0 switches the history off. */
+ * advances the position modulo the history length 3 nfields (WalkerSetBase.h:158,
+ * WalkerSetBase.cpp:340); the position lives on the GPU (a captured step replays it).
+ * afq_population_control moves each walker's ring row with the walker (WalkerSet::copy_walker,
+ * WalkerSetBase.h:413-415). Resets the position to 0 and unregisters the weight rings;
+ * nfields = 0 switches the history off. Sizes afq_back_propagate's scratch for W walkers (the
+ * only allocation of back-propagation). */
 int afq_set_field_history(afq_ham ham, int W, int nfields, double* d_fields, void* stream);
-/* The history position (steps stored since afq_set_field_history). Synchronises the stream. */
+/* The weight rings of back-propagation (WalkerSet getWeightFactors / getWeightHistory,
+ * WalkerSetBase.h:521-534), caller-owned, each [W][3 nfields] c128 for the W and nfields of the
+ * registered field history; set to 1 (factors) and 0 (history) here, as resize_bp does. Every
+ * afq_step then writes at the current position each walker's weight factor
+ * exp(-dt (E_eff - Eshift)) / |.| / scale (0 for a walker dropped by the phaseless projection or a
+ * non-finite energy) and its updated weight before the weight bound (WalkerSetUpdate.hpp:103,
+ * 128-147, 184-190); afq_population_control moves both rows with the walker and writes w/|w|
+ * into the last step's history entry (population_control.hpp:249-254). NULL, NULL unregisters
+ * them. These are what a BackPropEstimator's path restoration reads
+ * (BackPropEstimator.hpp:215-228). */
+int afq_set_weight_history(afq_ham ham, double* d_factors, double* d_history, void* stream);
+/* The history position (steps stored since afq_set_field_history, modulo 3 nfields).
+ * Synchronises the stream. */
 int afq_history_position(afq_ham ham, int* pos, void* stream);
 /* AFQMCBasePropagator::BackPropagate (AFQMCBasePropagator.cpp:403-532), dense vHS: for the last
  * nbpsteps stored field sets, newest first, vHS = sqrt(dt) L X (FP64) and every reference

@@ -596,12 +596,16 @@ enum { P_WEIGHT = 0, P_PHASE, P_PSEUDO_ELOC, P_E1, P_EXX, P_EJ, P_OVLP, P_LOGSCL
        P_IS_UNITARY, P_THETA, P_COUNT };
 
 // walker_update (WalkerSetUpdate.hpp:69-191) + bound_local_energy (46-58). stats[0..2] =
-// (total, upper, lower) local-energy bound hits; accumulated.
+// (total, upper, lower) local-energy bound hits; accumulated. With back-propagation the
+// reference also records each walker's weight factor (WalkerSetUpdate.hpp:103, 128-147) and its
+// updated weight (WalkerSetUpdate.hpp:184-190) in the history rings: wfac_out[W] and
+// weight_out[W] receive them when not null.
 void orc_walker_update(int W, cd* wbuf, long ldw, long poff, int hybrid, int free_projection,
                        int use_cp, double dt, double Eshift, double E0, const cd* new_ov,
                        const cd* new_E, const cd* mf, const cd* HW, double lower_scale,
-                       double upper_scale, long* stats) {
+                       double upper_scale, long* stats, cd* wfac_out, cd* weight_out) {
   for (int i = 0; i < W; ++i) {
+    cd wfac = 0.0;
     cd* p = wbuf + (long)i * ldw + poff;
     cd old_ov = p[P_OVLP], old_eloc = p[P_PSEUDO_ELOC];
     double scale = 1.0, dtheta = 0.0;
@@ -616,7 +620,7 @@ void orc_walker_update(int W, cd* wbuf, long ldw, long poff, int hybrid, int fre
     if (!std::isfinite(eloc.real())) {
       p[P_WEIGHT] = 0.0;
       p[P_PHASE] = 0.0;
-      eloc = old_eloc;
+      eloc = old_eloc;   // wfac stays 0
     } else {
       if (!free_projection) {
         double hi = Eshift + upper_scale * std::sqrt(2.0 / dt);
@@ -628,9 +632,12 @@ void orc_walker_update(int W, cd* wbuf, long ldw, long poff, int hybrid, int fre
       }
       cd eeff = hybrid ? eloc : 0.5 * (eloc + old_eloc);
       cd wf = std::exp(-dt * (eeff - Eshift));
+      if (std::abs(scale) > std::numeric_limits<double>::min()) wfac = wf / std::abs(wf) / scale;
       p[P_WEIGHT] *= free_projection ? scale * wf : scale * std::abs(wf);
       p[P_PHASE] *= wf / std::abs(wf);
     }
+    if (wfac_out) wfac_out[i] = wfac;
+    if (weight_out) weight_out[i] = p[P_WEIGHT];
     p[P_PSEUDO_ELOC] = eloc;
     p[P_OVLP] = new_ov[i];
     if (!hybrid) {
@@ -822,7 +829,8 @@ void orc_system_seed_properties(const orc_system* s, int W, cd* wbuf, long ldw) 
 // continuous charge fields, dense vHS, with the N(0,1) fields xi[W][K] given. Optional outputs
 // (may be null) expose the intermediates for step-level parity: vbias, X, V, new overlaps/energies.
 void orc_system_step(orc_system* s, int W, cd* wbuf, long ldw, const double* xi, double Eshift,
-                     int hybrid, cd* out_vbias, cd* out_X, cd* out_V, cd* out_ov, cd* out_E) {
+                     int hybrid, cd* out_vbias, cd* out_X, cd* out_V, cd* out_ov, cd* out_E,
+                     cd* out_wfac, cd* out_weight) {
   const int M = s->M, N = s->N, K = s->K;
   const double dt = s->dt;
   const long poff = (long)M * N;
@@ -849,7 +857,8 @@ void orc_system_step(orc_system* s, int W, cd* wbuf, long ldw, const double* xi,
   // 6. walker_update, 7. bound_walker_weights (target population = W on one rank)
   orc_walker_update(W, wbuf, ldw, poff, hybrid, s->free_projection, s->use_cp_constraint, dt,
                     Eshift, s->E0, new_ov.data(), new_E.data(), mf.data(), HW.data(),
-                    s->lower_cutoff_scale, s->upper_cutoff_scale, s->eloc_stats);
+                    s->lower_cutoff_scale, s->upper_cutoff_scale, s->eloc_stats, out_wfac,
+                    out_weight);
   orc_bound_weights(W, wbuf, ldw, poff, s->weight_bound_floor, s->weight_bound_fraction,
                     double(W), s->weight_stats);
 }

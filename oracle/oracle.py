@@ -65,7 +65,7 @@ def _declare(L):
         "orc_construct_X": (None, [_i, _i, _d, _i, _i, _vp, _vp, _vp, _vp]),
         "orc_meanfield_factor": (None, [_i, _i, _vp, _vp, _vp]),
         "orc_walker_update": (None, [_i, _vp, _l, _l, _i, _i, _i, _d, _d, _d, _vp, _vp, _vp,
-                                     _vp, _d, _d, _vp]),
+                                     _vp, _d, _d, _vp, _vp, _vp]),
         "orc_bound_weights": (None, [_i, _vp, _l, _l, _d, _d, _d, _vp]),
         "orc_comb": (_i, [_i, _vp, _d, _i, _vp]),
         "orc_population_control": (_i, [_i, _vp, _l, _l, _d, _vp]),
@@ -85,7 +85,8 @@ def _declare(L):
         "orc_system_log_overlap": (None, [_vp, _i, _vp, _l, _vp]),
         "orc_system_energy": (None, [_vp, _i, _vp, _l, _vp, _vp]),
         "orc_system_seed_properties": (None, [_vp, _i, _vp, _l]),
-        "orc_system_step": (None, [_vp, _i, _vp, _l, _vp, _d, _i, _vp, _vp, _vp, _vp, _vp]),
+        "orc_system_step": (None, [_vp, _i, _vp, _l, _vp, _d, _i, _vp, _vp, _vp, _vp, _vp,
+                                   _vp, _vp]),
         "orc_system_orthogonalize": (None, [_vp, _i, _vp, _l]),
         "orc_system_stats": (None, [_vp, _vp, _vp]),
         "orc_num_threads": (_i, []),
@@ -294,15 +295,21 @@ def walker_update(buf, poff, hybrid, dt, Eshift, E0, new_ov, new_E, mf, HW, free
                   use_cp=False, lower_scale=1.0, upper_scale=10.0, weight_floor=100.0,
                   weight_fraction=0.1):
     """walker_update + bound_local_energy (WalkerSetUpdate.hpp:46-191), then the weight bound
-    of the step (target population = W), in place on buf [W][ldw]."""
+    of the step (target population = W), in place on buf [W][ldw]. Returns the weight factors
+    and the weights after the update (before the bound) that back-propagation records in its
+    history rings (WalkerSetUpdate.hpp:184-190)."""
     W, ldw = buf.shape
     stats = np.zeros(3, np.int64)
+    wfac = np.zeros(W, np.complex128)
+    wnew = np.zeros(W, np.complex128)
     lib().orc_walker_update(W, _p(buf), ldw, poff, int(hybrid), int(free_projection),
                             int(use_cp), dt, Eshift, E0, _p(_c(new_ov)), _p(_c(new_E)),
-                            _p(_c(mf)), _p(_c(HW)), lower_scale, upper_scale, _p(stats))
+                            _p(_c(mf)), _p(_c(HW)), lower_scale, upper_scale, _p(stats),
+                            _p(wfac), _p(wnew))
     wst = np.zeros(2, np.int64)
     lib().orc_bound_weights(W, _p(buf), ldw, poff, weight_floor, weight_fraction, float(W),
                             _p(wst))
+    return wfac, wnew
 
 
 def comb(weights, r, num_target=None):
@@ -319,6 +326,28 @@ def population_control(buf, poff, r):
     off = np.zeros(W, np.int32)
     rc = lib().orc_population_control(W, _p(buf), buf.shape[1], poff, float(r), _p(off))
     return off, rc
+
+
+def copy_plan(off):
+    """The slot placement of population control on one rank (swap_walkers_async,
+    population_control.hpp:143-209) as a source map: src[slot] = the walker copied into slot,
+    -1 where the slot keeps its walker. A walker with k offspring keeps its slot; its k-1 copies
+    fill the slots of zero-offspring walkers in increasing order. The same map moves every
+    per-walker array (WalkerSet::copy_walker copies the back-propagation row too,
+    WalkerSetBase.h:413-415)."""
+    off = np.asarray(off)
+    src = np.full(len(off), -1, np.int64)
+    free = iter(np.flatnonzero(off == 0))
+    for i, k in enumerate(off):
+        for _ in range(1, int(k)):
+            src[next(free)] = i
+    return src
+
+
+def apply_copy_plan(rows, src):
+    """rows[slot] <- rows[src[slot]] for every slot with a source (in place)."""
+    for slot in np.flatnonzero(src >= 0):
+        rows[slot] = rows[src[slot]]
 
 
 def ph_excited_2body(iexcit, refc, Twina, R, wgt):
@@ -611,10 +640,11 @@ class System:
             M, N, K = self.M, self.N, self.K
             out = dict(vbias=np.zeros((W, K), np.complex128), X=np.zeros((W, K), np.complex128),
                        V=np.zeros((W, M, M), np.complex128), ov=np.zeros(W, np.complex128),
-                       E=np.zeros((W, 3), np.complex128))
-            args = [_p(out[k]) for k in ("vbias", "X", "V", "ov", "E")]
+                       E=np.zeros((W, 3), np.complex128), wfac=np.zeros(W, np.complex128),
+                       weight=np.zeros(W, np.complex128))
+            args = [_p(out[k]) for k in ("vbias", "X", "V", "ov", "E", "wfac", "weight")]
         else:
-            args = [None] * 5
+            args = [None] * 7
         lib().orc_system_step(self.ptr, W, _p(buf), buf.shape[1], _p(xi), Eshift, int(hybrid),
                               *args)
         return out

@@ -70,6 +70,7 @@ SIGNATURES = {
     "afq_orthogonalize": (_i, [_vp, _i, _vp, _l, _vp]),
     "afq_set_field_history": (_i, [_vp, _i, _i, _vp, _vp]),
     "afq_history_position": (_i, [_vp, _vp, _vp]),
+    "afq_set_weight_history": (_i, [_vp, _vp, _vp, _vp]),
     "afq_back_propagate": (_i, [_vp, _i, _i, _i, _vp, _i, _vp, _vp]),
     "afq_seed_properties": (_i, [_vp, _i, _vp, _l, _vp]),
     "afq_weighted_sums": (_i, [_vp, _i, _vp, _l, _vp, _vp]),
@@ -436,11 +437,32 @@ class WalkerSet:
         Fields [W][nfields][K] c128 that every Propagate fills, position reset to 0;
         nfields = 0 switches it off."""
         torch = _torch()
+        self.weight_factors = self.weight_history = None   # unregistered with the old ring
         self.fields = (torch.zeros((self.W, nfields, self.ham.K), dtype=torch.complex128,
                                    device="cuda") if nfields > 0 else None)
         check(lib().afq_set_field_history(self.ham.handle, self.W, int(nfields),
                                           ptr(self.fields) if nfields > 0 else None,
                                           stream_ptr(stream)))
+
+    def set_weight_history(self, stream=None):
+        """The weight-factor and weight-history rings [W][3 nfields] c128 of back-propagation
+        (getWeightFactors / getWeightHistory), filled by every Propagate and moved by
+        population control; needs set_back_propagation first."""
+        torch = _torch()
+        n = 3 * self.NumBackProp()
+        self.weight_factors = torch.empty((self.W, n), dtype=torch.complex128, device="cuda")
+        self.weight_history = torch.empty((self.W, n), dtype=torch.complex128, device="cuda")
+        check(lib().afq_set_weight_history(self.ham.handle, ptr(self.weight_factors),
+                                           ptr(self.weight_history), stream_ptr(stream)))
+
+    def HistoryBufferLength(self):
+        return 3 * self.NumBackProp()
+
+    def getWeightFactors(self):
+        return getattr(self, "weight_factors", None)
+
+    def getWeightHistory(self):
+        return getattr(self, "weight_history", None)
 
     def NumBackProp(self):
         return 0 if getattr(self, "fields", None) is None else self.fields.shape[1]

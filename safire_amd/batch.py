@@ -65,9 +65,11 @@ class SystemBatch:
         for it in self.items:
             self._system_steps(it, 0, self.steps)
 
-    def capture(self, streams: int = 4):
-        """One HIP graph of all steps of all systems; system i on capture stream i % streams.
-        Systems are assigned largest first, so the streams carry similar work."""
+    def capture(self, streams: int = 4, t0: int = 0, t1: int | None = None):
+        """One HIP graph of steps t0 .. t1-1 (all steps by default) of all systems, the systems
+        spread over `streams` capture streams: largest first, each to the least loaded stream,
+        so the streams carry similar work."""
+        t1 = self.steps if t1 is None else t1
         torch = A._torch()
         order = sorted(range(len(self.items)),
                        key=lambda i: -self.items[i]["W"] * self.items[i]["sys"].M ** 2 *
@@ -87,7 +89,7 @@ class SystemBatch:
                 st.wait_stream(main)
                 with torch.cuda.stream(st):
                     for it in lanes[j]:
-                        self._system_steps(it, 0, self.steps)
+                        self._system_steps(it, t0, t1)
             for st in side:
                 main.wait_stream(st)
 

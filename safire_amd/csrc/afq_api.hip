@@ -3,6 +3,7 @@
 // failures into an afq_status; the message is kept per thread for afq_last_error().
 #include "afq_internal.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <cctype>
@@ -493,9 +494,20 @@ int afq_population_control(afq_ham H, int W, double* wbuf, long ldw, double unif
     check_ham(H);
     check_walkers(H, W, ldw);
     AFQ_CHECK(uniform >= 0.0 && uniform < 1.0, AFQ_EINVAL, "uniform must be in [0, 1)");
+    AFQ_CHECK(H->nfields == 0 || W <= H->fields_W, AFQ_ECAPACITY,
+              "more walkers than the back-propagation field history holds");
     Workspace& ws = H->ws;
+    BackPropRows bp;
+    if (H->nfields > 0) {
+      bp.fields = H->fields;
+      bp.fields_ld = (long)H->nfields * H->K;
+      bp.wfac = H->wfac;
+      bp.whis = H->whis;
+      bp.nhist = 3 * H->nfields;
+      bp.pos = H->hist_pos;
+    }
     launch_population_control(S(stream), W, wbuf, ldw, (long)H->M * H->N, uniform, ws.cum,
-                              ws.off, ws.src, ws.ok, H->stats);
+                              ws.off, ws.src, ws.ok, H->stats, H->nfields > 0 ? &bp : nullptr);
   });
 }
 
@@ -685,9 +697,11 @@ int afq_step(afq_ham H, int W, double* wbuf, long ldw, const double* xi, double 
       RegionScope r(tm, s, R_WALKER_UPDATE);
       if (!p.hybrid) finalize_energy(H, s, W, ws.E);
       launch_walker_update(s, W, wbuf, ldw, (long)M * N, p, H->dt, Eshift, H->E0, ws.ov, ws.E,
-                           ws.mf, ws.HW, p.hybrid, H->stats);
+                           ws.mf, ws.HW, p.hybrid, H->stats, H->wfac, H->whis,
+                           3 * H->nfields, H->hist_pos);
     }
-    if (H->nfields > 0) launch_advance_history(s, H->hist_pos);   // advanceHistoryPos
+    if (H->nfields > 0)   // advanceHistoryPos: wraps at the history length 3 nfields
+      launch_advance_history(s, H->hist_pos, 3 * H->nfields);
   });
 }
 
@@ -698,11 +712,41 @@ int afq_set_field_history(afq_ham H, int W, int nfields, double* d_fields, void*
     AFQ_CHECK(nfields == 0 || d_fields != nullptr, AFQ_EINVAL,
               "afq_set_field_history: null field ring");
     if (!H->hist_pos) dalloc(&H->hist_pos, 1);
+    // back-propagation's own device memory is sized here, so that neither the step nor
+    // afq_back_propagate allocates
+    if (nfields > 0) {
+      if (!H->P1H) dalloc(&H->P1H, 2L * H->M * H->M);
+      if (H->bp_cap < W) {
+        dfree(H->bp_ldet);
+        dalloc(&H->bp_ldet, 2L * std::max(W, 1));
+        H->bp_cap = W;
+      }
+    }
     AFQ_HIP(hipMemsetAsync(H->hist_pos, 0, sizeof(int), S(stream)));
     AFQ_HIP(hipStreamSynchronize(S(stream)));
     H->fields = nfields > 0 ? d_fields : nullptr;
     H->nfields = nfields;
     H->fields_W = nfields > 0 ? W : 0;
+    H->wfac = H->whis = nullptr;   // the weight rings belong to the previous field ring
+  });
+}
+
+int afq_set_weight_history(afq_ham H, double* d_factors, double* d_history, void* stream) {
+  return guard([&] {
+    check_ham(H);
+    AFQ_CHECK(H->nfields > 0, AFQ_EINVAL,
+              "afq_set_weight_history: register the field history first");
+    AFQ_CHECK((d_factors == nullptr) == (d_history == nullptr), AFQ_EINVAL,
+              "afq_set_weight_history: give both rings or neither");
+    if (d_factors != nullptr) {
+      // WalkerSetBase::resize_bp: factors start at 1, the weight history at 0
+      const long n = (long)H->fields_W * 3 * H->nfields;
+      launch_ph_fill(S(stream), n, 1.0, 0.0, d_factors);
+      launch_ph_fill(S(stream), n, 0.0, 0.0, d_history);
+      AFQ_HIP(hipStreamSynchronize(S(stream)));
+    }
+    H->wfac = d_factors;
+    H->whis = d_history;
   });
 }
 
@@ -734,12 +778,8 @@ int afq_back_propagate(afq_ham H, int W, int nbpsteps, int nstabilize, double* d
     hipStream_t s = S(stream);
     const int M = H->M, N = H->N, K = H->K, MM = M * M, nf = H->nfields;
     const long NM = (long)N * M;
-    if (!H->P1H) dalloc(&H->P1H, 2L * MM);
-    if (H->bp_cap < W) {
-      dfree(H->bp_ldet);
-      dalloc(&H->bp_ldet, 2L * W);
-      H->bp_cap = W;
-    }
+    AFQ_CHECK(H->P1H != nullptr && H->bp_cap >= W, AFQ_ECAPACITY,
+              "afq_back_propagate: more walkers than afq_set_field_history sized it for");
     int pos = 0;
     AFQ_HIP(hipMemcpyAsync(&pos, H->hist_pos, sizeof(int), hipMemcpyDeviceToHost, s));
     AFQ_HIP(hipStreamSynchronize(s));

@@ -178,6 +178,9 @@ struct afq_ham_s {
   // back-propagation: the caller's field ring [fields_W][nfields][K] c128 and its position
   double* fields = nullptr;
   int nfields = 0, fields_W = 0;
+  // weight factors and weight history [fields_W][3 nfields] c128 (caller-owned; null = none)
+  double* wfac = nullptr;
+  double* whis = nullptr;
   int* hist_pos = nullptr;   // device int
   double* P1H = nullptr;     // c128 [M][M]
   double* bp_ldet = nullptr; // c128 [bp_cap]
@@ -242,7 +245,7 @@ void launch_orthogonalize(hipStream_t s, int nb, int M, int N, double* Phi, long
 // k_walker.hip
 void launch_store_fields(hipStream_t s, int W, int K, const double* X, double* ring, int nfields,
                          const int* pos);
-void launch_advance_history(hipStream_t s, int* pos);
+void launch_advance_history(hipStream_t s, int* pos, int nhist);
 void launch_neg_adjoint(hipStream_t s, int W, int M, double* V);
 void launch_adjoint(hipStream_t s, int M, const double* A, double* B);
 void launch_add_strided(hipStream_t s, int W, const double* x, double* out, long stride);
@@ -252,7 +255,9 @@ void launch_construct_X(hipStream_t s, int W, int K, double vbias_bound, int pro
 void launch_walker_update(hipStream_t s, int W, double* wbuf, long ldw, long poff,
                           const afq_step_params& p, double dt, double Eshift, double E0,
                           const double* ov, const double* E, const double* mf,
-                          const double* HW, int hybrid, unsigned long long* stats);
+                          const double* HW, int hybrid, unsigned long long* stats,
+                          double* wfac = nullptr, double* whis = nullptr, int nhist = 0,
+                          const int* pos = nullptr);
 void launch_set_properties(hipStream_t s, int W, double* wbuf, long ldw, long poff,
                            const double* ov, const double* E, int seed_pseudo);
 void launch_weighted_sums(hipStream_t s, int W, const double* wbuf, long ldw, long poff,
@@ -260,9 +265,20 @@ void launch_weighted_sums(hipStream_t s, int W, const double* wbuf, long ldw, lo
 void launch_estimator_sums(hipStream_t s, int W, const double* wbuf, long ldw, long poff,
                            double dt, const double* E, const double* ov, double* out);
 void launch_probe_mfma(hipStream_t s, int kind, double* out);
+// the per-walker back-propagation rows that population control moves with the walkers: the
+// field ring [W][fields_ld] c128 and the weight-factor / weight-history rings [W][nhist] c128
+// (null when absent), and the history position on the device
+struct BackPropRows {
+  double* fields = nullptr;
+  long fields_ld = 0;
+  double* wfac = nullptr;
+  double* whis = nullptr;
+  int nhist = 0;
+  const int* pos = nullptr;
+};
 void launch_population_control(hipStream_t s, int W, double* wbuf, long ldw, long poff,
                                double r, double* cum, int* off, int* src, int* ok,
-                               unsigned long long* stats);
+                               unsigned long long* stats, const BackPropRows* bp = nullptr);
 // k_setup.hip
 void setup_hamiltonian(afq_ham_s* H, hipStream_t s);
 void launch_v0(hipStream_t s, int M, int K, const double* Likn, double* v0);

@@ -67,9 +67,11 @@ __global__ void k_walker_update(int W, cd* __restrict__ wbuf, long ldw, long pof
                                 const cd* __restrict__ E, const cd* __restrict__ mf,
                                 const cd* __restrict__ HW, double lower_scale,
                                 double upper_scale, double max_weight,
-                                unsigned long long* __restrict__ stats) {
+                                unsigned long long* __restrict__ stats, cd* __restrict__ wfac,
+                                cd* __restrict__ whis, int nhist, const int* __restrict__ pos) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= W) return;
+  cd factor = cmk(0.0, 0.0);   // back-propagation's weight factor (0 for a dropped walker)
   cd* p = wbuf + (long)i * ldw + poff;
   const cd old_ov = p[AFQ_P_OVLP], old_eloc = p[AFQ_P_PSEUDO_ELOC];
   double scale = 1.0, dtheta = 0.0;
@@ -105,6 +107,14 @@ __global__ void k_walker_update(int W, cd* __restrict__ wbuf, long ldw, long pof
     const cd wt = p[AFQ_P_WEIGHT];
     p[AFQ_P_WEIGHT] = free_projection ? (wt * (scale * wf)) : (scale * aw) * wt;
     p[AFQ_P_PHASE] = p[AFQ_P_PHASE] * cmk(wf.re / aw, wf.im / aw);
+    if (fabs(scale) > 2.2250738585072014e-308) factor = cmk(wf.re / aw / scale, wf.im / aw / scale);
+  }
+  // with back-propagation the factor and the updated weight (before the bound below) go into
+  // the history rings at the current position (WalkerSetUpdate.hpp:184-190)
+  if (wfac != nullptr) {
+    const long slot = (long)i * nhist + *pos;
+    wfac[slot] = factor;
+    whis[slot] = p[AFQ_P_WEIGHT];
   }
   p[AFQ_P_PSEUDO_ELOC] = eloc;
   p[AFQ_P_OVLP] = new_ov[i];
@@ -439,9 +449,32 @@ __global__ void k_store_fields(int W, int K, const cd* __restrict__ X, cd* __res
   ring[(w * nfields + slot) * K + n] = X[e];
 }
 
-// WalkerSet::advanceHistoryPos
-__global__ void k_advance_history(int* pos) {
-  if (threadIdx.x == 0) *pos += 1;
+// WalkerSet::advanceHistoryPos: the position wraps at the history length, 3 x the field ring
+// (WalkerSetBase.h:158, WalkerSetBase.cpp:340)
+__global__ void k_advance_history(int* pos, int nhist) {
+  if (threadIdx.x == 0) *pos = (*pos + 1) % nhist;
+}
+
+// rows[slot] <- rows[src[slot]] for the slots population control refills (src >= 0); a source
+// keeps its own slot, so no row is both read and written
+__global__ void k_copy_rows(int W, cd* __restrict__ rows, long ld, long ncols,
+                            const int* __restrict__ src) {
+  const int slot = blockIdx.x;
+  const int from = src[slot];
+  if (from < 0) return;
+  for (long e = threadIdx.x; e < ncols; e += blockDim.x)
+    rows[(long)slot * ld + e] = rows[(long)from * ld + e];
+}
+
+// after the comb the last completed step's weight-history entry holds the normalised weight
+// (population_control.hpp:249-254)
+__global__ void k_history_fixup(int W, const cd* __restrict__ wbuf, long ldw, long poff,
+                                cd* __restrict__ whis, int nhist, const int* __restrict__ pos,
+                                const int* __restrict__ ok) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= W || ok[0] == 0) return;
+  const int slot = (*pos + nhist - 1) % nhist;
+  whis[(long)i * nhist + slot] = wbuf[(long)i * ldw + poff + AFQ_P_WEIGHT];
 }
 
 // V_w <- -V_w^H in place, so that the forward propagator's exp(i V) series becomes the 'H'
@@ -491,15 +524,20 @@ void launch_construct_X(hipStream_t s, int W, int K, double vbias_bound, int pro
 void launch_walker_update(hipStream_t s, int W, double* wbuf, long ldw, long poff,
                           const afq_step_params& p, double dt, double Eshift, double E0,
                           const double* ov, const double* E, const double* mf,
-                          const double* HW, int hybrid, unsigned long long* stats) {
+                          const double* HW, int hybrid, unsigned long long* stats,
+                          double* wfac, double* whis, int nhist, const int* pos) {
   if (W <= 0) return;
   const double maxw = fmax(p.weight_bound_floor, p.weight_bound_fraction * (double)W);
+  const bool hist = wfac != nullptr && whis != nullptr && nhist > 0 && pos != nullptr;
   hipLaunchKernelGGL(k_walker_update, dim3((W + 127) / 128), dim3(128), 0, s, W,
                      reinterpret_cast<cd*>(wbuf), ldw, poff, hybrid, p.free_projection,
                      p.use_cp_constraint, dt, Eshift, E0, reinterpret_cast<const cd*>(ov),
                      reinterpret_cast<const cd*>(E), reinterpret_cast<const cd*>(mf),
                      reinterpret_cast<const cd*>(HW), p.lower_cutoff_scale,
-                     p.upper_cutoff_scale, maxw, stats);
+                     p.upper_cutoff_scale, maxw, stats,
+                     hist ? reinterpret_cast<cd*>(wfac) : nullptr,
+                     hist ? reinterpret_cast<cd*>(whis) : nullptr, hist ? nhist : 0,
+                     hist ? pos : nullptr);
   AFQ_HIP(hipGetLastError());
 }
 
@@ -531,7 +569,7 @@ void launch_estimator_sums(hipStream_t s, int W, const double* wbuf, long ldw, l
 
 void launch_population_control(hipStream_t s, int W, double* wbuf, long ldw, long poff,
                                double r, double* cum, int* off, int* src, int* ok,
-                               unsigned long long* stats) {
+                               unsigned long long* stats, const BackPropRows* bp) {
   if (W <= 0) return;
   const size_t lds = (size_t)W * (sizeof(double) + 3 * sizeof(int));
   if (lds <= 120 * 1024) {
@@ -544,8 +582,25 @@ void launch_population_control(hipStream_t s, int W, double* wbuf, long ldw, lon
   }
   hipLaunchKernelGGL(k_copy_walkers, dim3(W), dim3(256), 0, s, W, reinterpret_cast<cd*>(wbuf),
                      ldw, (const int*)src);
+  if (bp != nullptr && bp->fields != nullptr) {
+    // WalkerSet::copy_walker moves the back-propagation row with the walker
+    // (WalkerSetBase.h:377-382, 413-415): the field ring, and the weight rings when present
+    hipLaunchKernelGGL(k_copy_rows, dim3(W), dim3(256), 0, s, W,
+                       reinterpret_cast<cd*>(bp->fields), bp->fields_ld, bp->fields_ld,
+                       (const int*)src);
+    if (bp->wfac != nullptr) {
+      hipLaunchKernelGGL(k_copy_rows, dim3(W), dim3(64), 0, s, W, reinterpret_cast<cd*>(bp->wfac),
+                         (long)bp->nhist, (long)bp->nhist, (const int*)src);
+      hipLaunchKernelGGL(k_copy_rows, dim3(W), dim3(64), 0, s, W, reinterpret_cast<cd*>(bp->whis),
+                         (long)bp->nhist, (long)bp->nhist, (const int*)src);
+    }
+  }
   hipLaunchKernelGGL(k_normalize_weights, dim3((W + 127) / 128), dim3(128), 0, s, W,
                      reinterpret_cast<cd*>(wbuf), ldw, poff, (const int*)ok);
+  if (bp != nullptr && bp->whis != nullptr)
+    hipLaunchKernelGGL(k_history_fixup, dim3((W + 127) / 128), dim3(128), 0, s, W,
+                       reinterpret_cast<const cd*>(wbuf), ldw, poff,
+                       reinterpret_cast<cd*>(bp->whis), bp->nhist, bp->pos, (const int*)ok);
   AFQ_HIP(hipGetLastError());
 }
 
@@ -599,8 +654,8 @@ void launch_store_fields(hipStream_t s, int W, int K, const double* X, double* r
   AFQ_HIP(hipGetLastError());
 }
 
-void launch_advance_history(hipStream_t s, int* pos) {
-  hipLaunchKernelGGL(k_advance_history, dim3(1), dim3(64), 0, s, pos);
+void launch_advance_history(hipStream_t s, int* pos, int nhist) {
+  hipLaunchKernelGGL(k_advance_history, dim3(1), dim3(64), 0, s, pos, nhist);
   AFQ_HIP(hipGetLastError());
 }
 

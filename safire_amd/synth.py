@@ -128,16 +128,3 @@ def fields(seed: int, steps: int, W: int, K: int, first_step: int = 0) -> np.nda
         rng = np.random.default_rng([seed, 7919, first_step + s])
         out[s] = rng.standard_normal((W, K))
     return out
-
-
-def uneven_batch(seed: int = SEEDS["c4"], nsys: int = 32):
-    """32 independent systems: M uniform in {16,24,...,96}, N = M // 4, K = 5 M, W in [32,256]."""
-    rng = np.random.default_rng(seed)
-    Ms = rng.choice(np.arange(16, 97, 8), size=nsys)
-    Ws = rng.integers(32, 257, size=nsys)
-    systems = []
-    for i, (M, W) in enumerate(zip(Ms, Ws)):
-        M = int(M)
-        systems.append((make_system(M, max(1, M // 4), 5 * M, seed * 1000 + i, name=f"c4.{i}"),
-                        int(W)))
-    return systems

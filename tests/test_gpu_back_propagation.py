@@ -72,3 +72,66 @@ def test_back_propagation_matches_oracle(nsteps, nfields, nbp, nstab, nrefs):
     with pytest.raises(A.AfqError):
         prop.BackPropagate(nfields + 1, nstab, wset, dR, ldR)
     ham.close()
+
+
+def test_weight_history_and_population_control_match_oracle():
+    """The history rings of back-propagation on c1 (M 24, N 6, K 96), 16 walkers, a field ring of
+    2 (history length 6): 9 steps (the position wraps at 6), QR every 5 steps, population control
+    after steps 3 and 7. Each step starts from the oracle's walker state; the GPU's field,
+    weight-factor and weight-history rings are compared with rings built from the oracle's
+    assembled fields, weight factors and updated weights (WalkerSetUpdate.hpp:184-190), moved by
+    the oracle's population-control plan with the last entry set to w/|w|
+    (population_control.hpp:249-254)."""
+    s, _, _, _ = synth.workload("c1")
+    W, nf, steps = 16, 2, 9
+    nh = 3 * nf
+    ham = A.HamiltonianOperations(s.h, s.Likn, s.PsiT, 0.0, s.dt)
+    ham.set_params(hybrid=0)
+    prop = A.Propagator(ham)
+    ref = O.System(s.h, s.Likn, s.PsiT, 0.0, s.dt)
+    buf = ref.new_walkers(W)
+    ref.seed_properties(buf)
+    Eshift = float(ref.energy(buf)[0].sum(1).real.mean())
+    wset = A.WalkerSet(ham, W, buf.copy())
+    wset.set_back_propagation(nf)
+    wset.set_weight_history()
+    assert wset.HistoryBufferLength() == nh
+    assert np.all(wset.getWeightFactors().cpu().numpy() == 1.0)
+    assert np.all(wset.getWeightHistory().cpu().numpy() == 0.0)
+    F = np.zeros((W, nf, s.K), complex)
+    WF = np.ones((W, nh), complex)
+    WH = np.zeros((W, nh), complex)
+    xi = synth.fields(s.seed, steps, W, s.K)
+    uniforms = np.random.default_rng(3).random(steps)
+    P = s.M * s.N
+    pos = 0
+    for t in range(steps):
+        wset.buf.copy_(torch.from_numpy(buf))
+        prop.Propagate(wset, dev(xi[t]), Eshift)
+        out = ref.step(buf, xi[t], Eshift, hybrid=False, capture=True)
+        F[:, pos % nf] = out["X"]
+        WF[:, pos] = out["wfac"]
+        WH[:, pos] = out["weight"]
+        pos = (pos + 1) % nh
+        if t % 5 == 0:
+            prop.Orthogonalize(wset)
+            ref.orthogonalize(buf)
+        if t in (3, 7):
+            # the same walker state on both sides, so the combs see identical weights
+            wset.buf.copy_(torch.from_numpy(buf))
+            prop.population_control(wset, float(uniforms[t]))
+            off, rc = O.population_control(buf, P, float(uniforms[t]))
+            assert rc == 0 and np.any(off != 1)
+            src = O.copy_plan(off)
+            for ring in (F, WF, WH):
+                O.apply_copy_plan(ring, src)
+            WH[:, (pos + nh - 1) % nh] = buf[:, P + A.P_WEIGHT]
+        assert wset.getHistoryPos() == pos, t
+        gF = wset.getFields().cpu().numpy()
+        gWF = wset.getWeightFactors().cpu().numpy()
+        gWH = wset.getWeightHistory().cpu().numpy()
+        assert np.abs(gF - F).max() <= 1e-11 * np.abs(F).max(), t
+        for g, r, name in ((gWF, WF, "factors"), (gWH, WH, "history")):
+            err = np.abs(g - r) / (1.0 + np.abs(r))
+            assert err.max() < 1e-9, (t, name, float(err.max()))
+    ham.close()

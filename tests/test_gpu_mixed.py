@@ -130,3 +130,64 @@ def test_int8_split_edge_rows():
     for w in (0, 3, 4, 5):
         scale = 1.0 + np.abs(e64[w]).sum()
         assert np.abs(e8[w] - e64[w]).max() < 1e-9 * scale, w
+
+
+def test_int8_energy_gate_c3():
+    """The int8 form at the full NMO=128 / 32+32 / Nchol=640 shape (Cholesky vectors decaying as
+    exp(-n/64)) against the FP64 oracle on walker states of an FP64 run: the seeded trial state
+    and the states after steps 0, 1, 2 and 4 (QR and population control after step 0; on this
+    Hamiltonian the phaseless projection removes most walkers within a few steps, so the states
+    are taken early). Every live walker's E_loc = E1 + EXX + EJ within 1e-5 Ha of the oracle's
+    on the oracle's G; the FP64 form within 1e-10 of the energy scale."""
+    s, _, _, ortho = synth.workload("c3")
+    W = 48
+    ham = A.HamiltonianOperations(s.h, s.Likn, s.PsiT, 0.0, s.dt)
+    ham.set_params(hybrid=0)
+    wset = A.WalkerSet(ham, W)
+    prop = A.Propagator(ham)
+    prop.seed(wset)
+    sums = prop.weighted_sums(wset).cpu().numpy()
+    Eshift = float((sums[0] / sums[1]).real)
+    xi = A.to_device(synth.fields(s.seed, 5, W, s.K))
+    uniforms = np.random.default_rng([s.seed, 31]).random(5)
+    states = [wset.host()]
+    for t in range(5):
+        prop.Propagate(wset, xi[t], Eshift)
+        if t % ortho == 0:
+            prop.Orthogonalize(wset)
+        if t % 10 == 0:
+            prop.population_control(wset, float(uniforms[t]))
+        if t in (0, 1, 2, 4):
+            states.append(wset.host())
+    ham8 = A.HamiltonianOperations(s.h, s.Likn, s.PsiT, 0.0, s.dt)
+    ham8.reserve(W)
+    ham8.set_precision(A.INT8_SPLIT)
+    R = O.System(s.h, s.Likn, s.PsiT, 0.0, s.dt).tensors()
+    P = s.M * s.N
+    worst, nlive = 0.0, 0
+    for k, st in enumerate(states):
+        live = np.abs(st[:, P + A.P_WEIGHT]) > 0
+        if not np.any(live):
+            continue
+        nlive += int(live.sum())
+        Phi = st[:, :P].reshape(W, s.M, s.N)
+        G, _ = O.mixed_density_matrix(s.PsiT, Phi)
+        Eref = O.energy(G, R["haj"], R["Lnak"], E0=0.0)
+        Gd = A.to_device(G)
+        out = {}
+        for key, h in (("fp64", ham), ("int8", ham8)):
+            E = torch.zeros((W, 3), dtype=torch.complex128, device="cuda")
+            h.energy(Gd, E)
+            torch.cuda.synchronize()
+            out[key] = E.cpu().numpy()
+        scale = 1.0 + np.abs(Eref).sum(1)
+        e64 = np.abs(out["fp64"] - Eref).max(1) / scale
+        assert np.all(e64[live] < 1e-10), (k, float(e64[live].max()))
+        dE = np.abs(out["int8"].sum(1) - Eref.sum(1))
+        worst = max(worst, float(dE[live].max()))
+        assert np.all(dE[live] < 1e-5), (k, float(dE[live].max()), float(scale[live].max()))
+    assert nlive >= W
+    print(f"int8 form at NMO=128/N=32/K=640: max |dE_loc| {worst:.3e} Ha over {nlive} live "
+          f"walker states")
+    ham8.close()
+    ham.close()

@@ -163,7 +163,7 @@ def test_orthogonalize_parity(M, N, nb):
 
 
 @pytest.mark.parametrize("shape", ["c1", (16, 5, 40, 3), (40, 10, 200, 7), (64, 16, 320, 9),
-                                   (96, 24, 480, 5), (128, 32, 640, 2)])
+                                   (96, 24, 480, 5), (128, 32, 640, 2), "c3"])
 def test_setup_and_contractions_parity(shape):
     s, W = make(shape, seed=3)
     W = min(W, 40)
@@ -194,12 +194,14 @@ def test_setup_and_contractions_parity(shape):
     ham.close()
 
 
-def run_resynced(shape, steps, hybrid, ortho=5, seed=9):
+def run_resynced(shape, steps, hybrid, ortho=5, seed=9, walkers=None):
     """Runs the oracle trajectory; before every step the GPU gets the oracle's walker buffer, so
     each step is compared on identical inputs. Returns per step and walker the relative error of
     the Slater matrix, the error of the properties relative to (1 + |value|), and kappa(PsiT Phi)
-    of the walker before the step (the conditioning that amplifies rounding through G)."""
+    of the walker before the step (the conditioning that amplifies rounding through G).
+    walkers: the first `walkers` walkers of the workload (all by default)."""
     s, W = make(shape, seed=seed)
+    W = W if walkers is None else walkers
     ham = A.HamiltonianOperations(s.h, s.Likn, s.PsiT, 0.0, s.dt)
     ham.set_params(hybrid=int(hybrid))
     prop = A.Propagator(ham)
@@ -276,6 +278,16 @@ def test_walker_steps_parity_c2(hybrid):
     """c2 (M=64, N=16, K=320), 512 walkers, 12 steps. Rounding is amplified by the conditioning
     of each walker's overlap matrix; the bound is 1e-10 + 1e-13 kappa_w per walker."""
     for t, (e_sm, e_pr, kappa) in enumerate(run_resynced("c2", 12, hybrid)):
+        bound = TOL + 1e-13 * kappa
+        assert np.all(e_sm < bound), (t, worst(e_sm / bound, kappa))
+        assert np.all(e_pr < bound), (t, worst(e_pr / bound, kappa))
+
+
+@pytest.mark.parametrize("hybrid", [True, False])
+def test_walker_steps_parity_c3(hybrid):
+    """c3 (M=128, N=32, K=640, Cholesky vectors decaying as exp(-n/64)): 16 walkers, 5 steps
+    with QR after step 0, each step from the oracle's state, within 1e-10 + 1e-13 kappa_w."""
+    for t, (e_sm, e_pr, kappa) in enumerate(run_resynced("c3", 5, hybrid, walkers=16)):
         bound = TOL + 1e-13 * kappa
         assert np.all(e_sm < bound), (t, worst(e_sm / bound, kappa))
         assert np.all(e_pr < bound), (t, worst(e_pr / bound, kappa))

@@ -180,9 +180,11 @@ def test_walker_update_phaseless_and_bounds():
     HW = np.array([0.01 + 0.0j])
     E = np.zeros((1, 3), complex)
     st = np.zeros(3, np.int64)
+    wf = np.zeros(1, complex)      # back-propagation's weight factor of the step
+    wn = np.zeros(1, complex)      # and the weight it records (after the update)
     L.orc_walker_update(1, buf.ctypes.data, buf.shape[1], p, 1, 0, 0, dt, Es, E0,
                         new_ov.ctypes.data, E.ctypes.data, mf.ctypes.data, HW.ctypes.data,
-                        1.0, 10.0, st.ctypes.data)
+                        1.0, 10.0, st.ctypes.data, wf.ctypes.data, wn.ctypes.data)
     dth = np.pi / 3
     eloc = (mf[0] - HW[0] - (new_ov[0] - (0.2 + 0.1j))) / dt + E0
     np.testing.assert_allclose(buf[0, p + O.P_THETA], dth, rtol=1e-14)
@@ -190,13 +192,16 @@ def test_walker_update_phaseless_and_bounds():
     w = np.exp(-dt * (eloc - Es))
     np.testing.assert_allclose(buf[0, p + O.P_WEIGHT], np.cos(dth) * abs(w), rtol=1e-13)
     np.testing.assert_allclose(buf[0, p + O.P_PHASE], w / abs(w), rtol=1e-13)
+    np.testing.assert_allclose(wf[0], w / abs(w) / np.cos(dth), rtol=1e-13)
+    assert wn[0] == buf[0, p + O.P_WEIGHT]
     # |dtheta| > pi/2 kills the walker (max(0, cos))
     buf2, p = _one_walker_buf()
     new_ov = np.array([2.0j])
     L.orc_walker_update(1, buf2.ctypes.data, buf2.shape[1], p, 1, 0, 0, dt, Es, E0,
                         new_ov.ctypes.data, E.ctypes.data, np.zeros(1, complex).ctypes.data,
-                        np.zeros(1, complex).ctypes.data, 1.0, 10.0, st.ctypes.data)
+                        np.zeros(1, complex).ctypes.data, 1.0, 10.0, st.ctypes.data, wf.ctypes.data, wn.ctypes.data)
     assert buf2[0, p + O.P_WEIGHT] == 0
+    assert wf[0] == 0 and wn[0] == 0       # scale 0: no factor
     # local-energy form: midpoint rule and the lower clamp at Eshift - sqrt(2/dt)
     buf3, p = _one_walker_buf()
     buf3[0, p + O.P_PSEUDO_ELOC] = -2.0
@@ -205,7 +210,7 @@ def test_walker_update_phaseless_and_bounds():
     L.orc_walker_update(1, buf3.ctypes.data, buf3.shape[1], p, 0, 0, 0, dt, Es, E0,
                         np.zeros(1, complex).ctypes.data, E.ctypes.data,
                         np.zeros(1, complex).ctypes.data, np.zeros(1, complex).ctypes.data,
-                        1.0, 10.0, st.ctypes.data)
+                        1.0, 10.0, st.ctypes.data, wf.ctypes.data, wn.ctypes.data)
     lo = Es - np.sqrt(2 / dt)
     np.testing.assert_allclose(buf3[0, p + O.P_PSEUDO_ELOC], lo, rtol=1e-14)
     np.testing.assert_allclose(buf3[0, p + O.P_WEIGHT], np.exp(-dt * (0.5 * (lo - 2.0) - Es)),
@@ -219,8 +224,34 @@ def test_walker_update_phaseless_and_bounds():
     L.orc_walker_update(1, buf4.ctypes.data, buf4.shape[1], p, 0, 0, 0, dt, Es, E0,
                         np.zeros(1, complex).ctypes.data, E.ctypes.data,
                         np.zeros(1, complex).ctypes.data, np.zeros(1, complex).ctypes.data,
-                        1.0, 10.0, st.ctypes.data)
+                        1.0, 10.0, st.ctypes.data, wf.ctypes.data, wn.ctypes.data)
     assert buf4[0, p + O.P_WEIGHT] == 0 and buf4[0, p + O.P_PSEUDO_ELOC] == -7.0
+    assert wf[0] == 0 and wn[0] == 0
+
+
+def test_copy_plan_matches_population_control():
+    """The source map of oracle.copy_plan moves rows exactly as the oracle's population control
+    moves walker rows (the slot placement of swap_walkers_async)."""
+    rng = np.random.default_rng(5)
+    for W in (1, 7, 64, 301):
+        buf = rng.standard_normal((W, 4 + O.P_COUNT)) + 1j * rng.standard_normal((W, 4 + O.P_COUNT))
+        w = np.abs(rng.standard_normal(W))
+        w[rng.random(W) < 0.3] = 0.0
+        w[0] = max(w[0], 0.5)
+        buf[:, 4 + O.P_WEIGHT] = w
+        tag = np.arange(W, dtype=float)
+        ref = buf.copy()
+        off, rc = O.population_control(ref, 4, 0.37)
+        assert rc == 0
+        src = O.copy_plan(off)
+        rows = buf.copy()
+        O.apply_copy_plan(rows, src)
+        O.apply_copy_plan(tag, src)
+        keep = np.ones(buf.shape[1], bool)
+        keep[4 + O.P_WEIGHT] = False
+        assert np.array_equal(rows[:, keep], ref[:, keep])
+        # every walker with k offspring appears k times
+        assert np.array_equal(np.bincount(tag.astype(int), minlength=W), off)
 
 
 def test_bound_weights():
