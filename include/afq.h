@@ -276,6 +276,22 @@ int afq_ph_excited_2body_energy(int nw, int ndet, int nex, int nact, int nelec, 
                                 const int* d_iexcit, const int* d_refc, const double* d_Twina,
                                 const double* d_R, const double* d_wgt, double* d_EX,
                                 double* d_EJ, double* d_KE, void* stream);
+/* The same computation as afq_ph_excited_2body_energy, allocating nothing. It has no counterpart
+ * in the reference: it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). afq_ph_excited_2body_energy
+ * allocates a temporary for its group partials on every call (and waits for the stream before
+ * freeing it); this variant keeps them in a static device buffer of
+ * afq_ph_noalloc_scratch_doubles() doubles (2^20, 8 MiB) that each GPU holds until the process
+ * ends, and returns AFQ_ECAPACITY when 4 * nw * ceil(nchol / 4) exceeds it. Every call on a GPU
+ * shares that GPU's buffer: calls that overlap in time on one GPU (different streams, threads
+ * or processes' graphs) race and must be serialised by the caller. Results equal
+ * afq_ph_excited_2body_energy's bit for bit. */
+int afq_ph_excited_2body_energy_noalloc(int nw, int ndet, int nex, int nact, int nelec,
+                                        int nchol, const int* d_iexcit, const int* d_refc,
+                                        const double* d_Twina, const double* d_R,
+                                        const double* d_wgt, double* d_EX, double* d_EJ,
+                                        double* d_KE, void* stream);
+long afq_ph_noalloc_scratch_doubles(void);
 /* ph_excited_1body_energy (phmsd_impl.hpp:149-183): S [nw][nelec][nact] c128;
  * d_E[w] += sum_d wgt sum_e (S[p_e] . R_e + S[p_e][v_e] - S[p_e][refc p_e]). */
 int afq_ph_excited_1body_energy(int nw, int ndet, int nex, int nact, int nelec,

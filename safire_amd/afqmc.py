@@ -81,6 +81,9 @@ SIGNATURES = {
     "afq_probe_mfma": (_i, [_i, _vp, _vp]),
     "afq_ph_excited_2body_energy": (_i, [_i, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp,
                                          _vp, _vp, _vp]),
+    "afq_ph_excited_2body_energy_noalloc": (_i, [_i, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp,
+                                                 _vp, _vp, _vp, _vp, _vp]),
+    "afq_ph_noalloc_scratch_doubles": (_l, []),
     "afq_ph_excited_1body_energy": (_i, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "afq_ph_mixed_density_matrix_woodbury": (_i, [_i, _i, _i, _i, _vp, _vp, _vp, _l, _vp, _vp,
                                                   _vp, _vp, _vp]),

@@ -916,6 +916,61 @@ int afq_ph_excited_2body_energy(int nw, int ndet, int nex, int nact, int nelec, 
   });
 }
 
+}  // extern "C"
+
+// Scratch memory of afq_ph_excited_2body_energy_noalloc: the group partials of the staged
+// 2-body kernel, 4 doubles per (walker, group of Cholesky vectors). Each device holds its own
+// copy for the life of the process. Host code never passes `scratch` itself (its host-side
+// name is not a device address): the device address comes from hipGetSymbolAddress, once per
+// device, and only that address reaches kernels.
+static constexpr long kNoAllocScratchDoubles = 1L << 20;   // 8 MiB per device
+static __device__ __align__(256) double scratch[kNoAllocScratchDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's address of `scratch`, resolved on the first call on that device.
+double* device_scratch() {
+  int dev = 0;
+  AFQ_HIP(hipGetDevice(&dev));
+  AFQ_CHECK(dev >= 0 && dev < kMaxDevices, AFQ_EUNSUPPORTED, "device ordinal beyond 63");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    AFQ_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    AFQ_CHECK(p != nullptr, AFQ_EHIP, "hipGetSymbolAddress returned no address for scratch");
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  return g_scratch_ptr[dev];
+}
+}  // namespace
+
+extern "C" {
+
+long afq_ph_noalloc_scratch_doubles(void) { return kNoAllocScratchDoubles; }
+
+int afq_ph_excited_2body_energy_noalloc(int nw, int ndet, int nex, int nact, int nelec,
+                                        int nchol, const int* iexcit, const int* refc,
+                                        const double* Twina, const double* R, const double* wgt,
+                                        double* EX, double* EJ, double* KE, void* stream) {
+  return guard([&] {
+    AFQ_CHECK(nw >= 0 && ndet >= 0 && nact > 0 && nelec > 0 && nelec <= nact && nchol >= 0,
+              AFQ_EBADSHAPE, "ph 2-body energy: bad sizes");
+    const long need = ph_2body_workspace(nw, nact, nelec, nchol);
+    AFQ_CHECK(need <= kNoAllocScratchDoubles, AFQ_ECAPACITY,
+              "ph 2-body energy (no allocation): the group partials need " +
+                  std::to_string(need) + " doubles, more than the " +
+                  std::to_string(kNoAllocScratchDoubles) +
+                  " of the static scratch buffer (4 * walkers * ceil(nchol / 4) <= 2^20)");
+    // every group partial is written before k_ph_2body_finish reads it: nothing to clear
+    double* ws = (need > 0 && nw > 0 && ndet > 0) ? device_scratch() : nullptr;
+    launch_ph_2body(S(stream), nw, ndet, nex, nact, nelec, nchol, iexcit, refc, Twina, R, wgt,
+                    EX, EJ, KE, ws, ws ? kNoAllocScratchDoubles : 0);
+  });
+}
+
 int afq_ph_excited_1body_energy(int nw, int ndet, int nex, int nact, int nelec,
                                 const int* iexcit, const int* refc, const double* Smat,
                                 const double* R, const double* wgt, double* E, void* stream) {
