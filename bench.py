@@ -40,7 +40,9 @@ METRIC = "walker-steps/s mixed vs FP64 at |ΔE_loc|<1e-5 Ha; % MFMA/HBM peak"
 # dense MFMA peak of the arithmetic each form's exchange kernel runs on
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0, "int8": 5000.0}
 # executed MFMA flops per algorithmic flop of the exchange contraction
-EXECUTED_PER_ALGO = {"fp64": 1.0, "fp32": 1.0, "bf16x3": 6.0, "int8": 15.0}
+# (BF16X3: 3 real products per complex product x 3 hi/lo terms = 18 flops per complex
+# multiply-add against the 8 counted; int8: 15 slice products)
+EXECUTED_PER_ALGO = {"fp64": 1.0, "fp32": 1.0, "bf16x3": 18.0 / 8.0, "int8": 15.0}
 GATE_HA = 1e-5
 # HBM bytes per launch of each form's exchange kernel (FETCH_SIZE and WRITE_SIZE counter runs
 # scaled by a copy kernel of known size), as recorded in profiles/walker_step_c2_hbm_traffic.md;
@@ -468,6 +470,16 @@ def bench_single(args, A, torch, synth, workload, with_cpu, with_hybrid):
     out["headline_form"] = mixed
     out["forms_walker_steps_per_s"] = {f: round(W * args.steps / (r["ms"] / 1e3), 1)
                                        for f, r in forms.items()}
+    # MFMA utilisation of each form's exchange kernel: executed MFMA operations per launch (the
+    # algorithmic count times the form's executed-per-algorithmic ratio) over its mean launch
+    # time, against the dense peak of the MFMA type it runs on
+    util = {}
+    for f, r in forms.items():
+        ms_, n_ = r["timers"]["exchange"]
+        if n_:
+            ex_rate = exchange_flops(W, M, N, K) / (ms_ / n_ / 1e3) / 1e12
+            util[f] = round(ex_rate * EXECUTED_PER_ALGO[f] / PEAK_TFLOPS[f], 4)
+    out["exchange_mfma_utilisation"] = util
     if unsupported:
         out["forms_unsupported"] = unsupported
     if fastest != mixed:
@@ -506,6 +518,7 @@ def bench_single(args, A, torch, synth, workload, with_cpu, with_hybrid):
 # keys of a secondary workload's line kept under its name in the headline line
 EXTRA_KEYS = ("value", "ms_per_step", "config", "headline_form", "fp64", "mixed_over_fp64",
               "max_abs_dEloc_Ha", "forms_walker_steps_per_s", "forms_unsupported", "roofline",
+              "exchange_mfma_utilisation", "contraction_rel_err_vs_fp64",
               "regions_ms_per_step", "graph_matches_eager")
 
 

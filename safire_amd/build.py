@@ -8,7 +8,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libsafire_afq.so")
-SOURCES = ["afq_api.hip", "afq_phmsd.hip", "afq_rng.hip", "k_green.hip", "k_gemm.hip", "k_energy.hip", "k_exchange_bf16.hip",
+SOURCES = ["afq_api.hip", "afq_phmsd.hip", "afq_rng.hip", "k_green.hip", "k_gemm.hip", "k_gemm_bf16.hip", "k_energy.hip", "k_exchange_bf16.hip",
            "k_exchange_i8.hip", "k_phmsd.hip", "k_propagate.hip", "k_qr.hip", "k_walker.hip",
            "k_setup.hip"]
 # k_green.hip unrolls the 32 Gauss-Jordan steps of its N <= 32 kernel in full (each step indexes

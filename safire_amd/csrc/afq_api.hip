@@ -108,18 +108,41 @@ const void* L_for(afq_ham H) { return H->precision == AFQ_FP64 ? (const void*)H-
 const void* Likn_for(afq_ham H) { return H->precision == AFQ_FP64 ? (const void*)H->Likn : (const void*)H->Likn_f; }
 double scl(afq_ham H) { return H->walker_type == AFQ_CLOSED ? 2.0 : 1.0; }
 
+// vbias = 2 sqrt(dt) G Lnak^T and vHS = sqrt(dt) L X in the handle's precision (the BF16X3 form
+// on its own GEMM, the others on the FP64/FP32 MFMA GEMM with tiling cfg)
+void run_vbias(afq_ham H, hipStream_t s, int W, const double* G, double* v, int cfg) {
+  const long NM = (long)H->N * H->M;
+  const double alpha = std::sqrt(H->dt) * scl(H);
+  if (H->precision == AFQ_BF16X3)
+    launch_gemm_bf16(s, W, H->K, (int)NM, alpha, G, NM, H->Lnak_g, true, v, H->K, H->ws.split,
+                     H->ws.split_elems);
+  else
+    launch_gemm_nt_z(s, H->precision, W, H->K, (int)NM, alpha, G, NM, L_for(H), NM, true, v,
+                     H->K, H->ws.split, H->ws.split_elems, cfg);
+}
+
+void run_vhs(afq_ham H, hipStream_t s, int W, const double* X, double* V, int cfg) {
+  const int MM = H->M * H->M;
+  if (H->precision == AFQ_BF16X3)
+    launch_gemm_bf16(s, W, MM, H->K, std::sqrt(H->dt), X, H->K, H->Likn_g, false, V, MM);
+  else
+    launch_gemm_nt_z(s, H->precision, W, MM, H->K, std::sqrt(H->dt), X, H->K, Likn_for(H),
+                     H->K, false, V, MM, nullptr, 0, cfg);
+}
+
 // the exchange contraction in the handle's precision: Kl and per-chunk tr(T^2) partials
 void run_exchange(afq_ham H, hipStream_t s, int W, const double* G) {
   if (H->precision == AFQ_BF16X3)
     launch_exchange_bf16(s, W, H->M, H->N, H->K, G, H->Lnak_b, H->ws.Kl, H->ws.exx);
   else if (H->precision == AFQ_INT8_SPLIT)
-    launch_exchange_i8(s, W, H->M, H->N, H->K, G, H->Lnak_q, H->Lnak_u, H->ws.Kl, H->ws.exx);
+    launch_exchange_i8(s, W, H->M, H->N, H->K, G, H->Lnak_q, H->Lnak_u, H->Lnak_ord, H->ws.Kl,
+                       H->ws.exx);
   else
     launch_exchange(s, H->precision, W, H->M, H->N, H->K, G, L_for(H), H->ws.Kl, H->ws.exx);
 }
 
 int exchange_chunks(afq_ham H, int W) {
-  if (H->precision == AFQ_BF16X3) return exchange_bf16_chunks(W, H->K);
+  if (H->precision == AFQ_BF16X3) return exchange_bf16_chunks(W, H->N, H->K);
   if (H->precision == AFQ_INT8_SPLIT) return exchange_i8_chunks(W, H->N, H->K);
   return exx_nchunks(W, H->N, H->K);
 }
@@ -230,13 +253,21 @@ int afq_ham_create(int M, int N, int K, const double* h_h, const double* h_Likn,
       AFQ_HIP(hipMemset(H->stats, 0, 7 * sizeof(unsigned long long)));
       setup_hamiltonian(H, nullptr);
       if (exchange_bf16_supported(M, N)) {
-        dalloc(&H->Lnak_b, 6 * NM * K);
+        dalloc(&H->Lnak_b, exchange_bf16_elems(M, N, K));
         split_lnak_bf16(nullptr, M, N, K, H->Lnak, H->Lnak_b);
+        dalloc(&H->Lnak_g, 6 * (size_t)K * bf16_row((int)NM));
+        split_planes_bf16(nullptr, K, (int)NM, true, H->Lnak, H->Lnak_g);
+        dalloc(&H->Likn_g, 2 * MM * bf16_row(K));
+        split_planes_bf16(nullptr, (long)MM, K, false, H->Likn, H->Likn_g);
       }
       if (exchange_i8_supported(M, N)) {
         dalloc(&H->Lnak_q, exchange_i8_bytes(M, N, K));
         dalloc(&H->Lnak_u, (size_t)K * N);
         split_lnak_i8(nullptr, M, N, K, H->Lnak, H->Lnak_q, H->Lnak_u);
+        if (N > 16) {
+          dalloc(&H->Lnak_ord, (size_t)K);
+          exchange_i8_orders(nullptr, M, N, K, H->Lnak, H->Lnak_ord);
+        }
       }
       AFQ_HIP(hipDeviceSynchronize());
     } catch (...) {
@@ -263,7 +294,10 @@ int afq_ham_destroy(afq_ham H) {
     dfree(H->H1);
     dfree(H->Lnak_f);
     dfree(H->Likn_f);
+    dfree(H->Lnak_ord);
     dfree(H->Lnak_b);
+    dfree(H->Lnak_g);
+    dfree(H->Likn_g);
     dfree(H->Lnak_q);
     dfree(H->Lnak_u);
     dfree(H->stats);
@@ -309,7 +343,7 @@ int afq_ham_set_precision(afq_ham H, int precision) {
     AFQ_CHECK(precision >= AFQ_FP64 && precision <= AFQ_INT8_SPLIT, AFQ_EINVAL,
               "unknown precision");
     AFQ_CHECK(precision != AFQ_BF16X3 || H->Lnak_b != nullptr, AFQ_EUNSUPPORTED,
-              "BF16X3 exchange needs N <= 16 and M <= 96 with M % 8 == 0");
+              "BF16X3 form needs N <= 32 and M <= 128");
     AFQ_CHECK(precision != AFQ_INT8_SPLIT || H->Lnak_q != nullptr, AFQ_EUNSUPPORTED,
               "INT8_SPLIT exchange needs N <= 32 and M <= 128");
     H->precision = precision;
@@ -471,14 +505,10 @@ int afq_autotune(afq_ham H, int W, const char* cache_path, int* cfg_out, void* s
       cache[k] = best;
       return best;
     };
-    H->cfg_vbias = tune(key("vbias", K, NM), true, [&](int c) {
-      launch_gemm_nt_z(s, p, W, K, (int)NM, std::sqrt(H->dt) * scl(H), ws.G, NM, L_for(H), NM,
-                       true, ws.X, K, ws.split, ws.split_elems, c);
-    });
-    H->cfg_vhs = tune(key("vhs", (long)M * M, K), false, [&](int c) {
-      launch_gemm_nt_z(s, p, W, M * M, K, std::sqrt(H->dt), ws.X, K, Likn_for(H), K, false, ws.V,
-                       (long)M * M, nullptr, 0, c);
-    });
+    H->cfg_vbias = tune(key("vbias", K, NM), true,
+                        [&](int c) { run_vbias(H, s, W, ws.G, ws.X, c); });
+    H->cfg_vhs = tune(key("vhs", (long)M * M, K), false,
+                      [&](int c) { run_vhs(H, s, W, ws.X, ws.V, c); });
     AFQ_HIP(hipStreamSynchronize(s));
     write_tile_cache(cache_path, cache);
     if (cfg_out) {
@@ -565,11 +595,8 @@ int afq_vbias(afq_ham H, int W, const double* G, double* v, void* stream) {
   return guard([&] {
     check_ham(H);
     AFQ_CHECK(W >= 0, AFQ_EBADSHAPE, "negative walker count");
-    const long NM = (long)H->N * H->M;
     AFQ_CHECK(W <= H->ws.cap, AFQ_ECAPACITY, "more walkers than reserved");
-    launch_gemm_nt_z(S(stream), H->precision, W, H->K, (int)NM, std::sqrt(H->dt) * scl(H), G,
-                     NM, L_for(H), NM, true, v, H->K, H->ws.split, H->ws.split_elems,
-                     H->cfg_vbias);
+    run_vbias(H, S(stream), W, G, v, H->cfg_vbias);
   });
 }
 
@@ -577,9 +604,7 @@ int afq_vhs(afq_ham H, int W, const double* X, double* V, void* stream) {
   return guard([&] {
     check_ham(H);
     AFQ_CHECK(W >= 0, AFQ_EBADSHAPE, "negative walker count");
-    const int MM = H->M * H->M;
-    launch_gemm_nt_z(S(stream), H->precision, W, MM, H->K, std::sqrt(H->dt), X, H->K,
-                     Likn_for(H), H->K, false, V, MM, nullptr, 0, H->cfg_vhs);
+    run_vhs(H, S(stream), W, X, V, H->cfg_vhs);
   });
 }
 
@@ -656,8 +681,7 @@ int afq_step(afq_ham H, int W, double* wbuf, long ldw, const double* xi, double 
     }
     {
       RegionScope r(tm, s, R_VBIAS);
-      launch_gemm_nt_z(s, H->precision, W, K, (int)NM, std::sqrt(H->dt) * scl(H), ws.G, NM,
-                       L_for(H), NM, true, ws.X, K, ws.split, ws.split_elems, H->cfg_vbias);
+      run_vbias(H, s, W, ws.G, ws.X, H->cfg_vbias);
     }
     // 2. X, HW, mean-field factor     (assemble_X)
     {
@@ -671,8 +695,7 @@ int afq_step(afq_ham H, int W, double* wbuf, long ldw, const double* xi, double 
     // 3. vHS = sqrt(dt) L X           4. Phi <- P1 exp(iV) P1 Phi
     {
       RegionScope r(tm, s, R_VHS);
-      launch_gemm_nt_z(s, H->precision, W, M * M, K, std::sqrt(H->dt), ws.X, K, Likn_for(H),
-                       K, false, ws.V, (long)M * M, nullptr, 0, H->cfg_vhs);
+      run_vhs(H, s, W, ws.X, ws.V, H->cfg_vhs);
     }
     {
       RegionScope r(tm, s, R_PROPAGATE);

@@ -170,9 +170,12 @@ struct afq_ham_s {
   double* H1 = nullptr;      // c128 [M][M]
   float* Lnak_f = nullptr;   // c64 [K][N][M]   (FP32 contraction copies)
   float* Likn_f = nullptr;   // f32 [M*M][K]
-  __bf16* Lnak_b = nullptr;  // bf16 [3 slices][re,im][K][N][M]   (BF16X3 exchange)
-  int8_t* Lnak_q = nullptr;  // i8 [4 slices][re,im][K][N][Mp]     (INT8_SPLIT exchange)
+  __bf16* Lnak_b = nullptr;  // bf16 [6 planes][K][16 NT][32 KC]  (BF16X3 exchange)
+  __bf16* Lnak_g = nullptr;  // bf16 [6 planes][K][N*M padded to 32]   (BF16X3 vbias)
+  __bf16* Likn_g = nullptr;  // bf16 [2 planes][M*M][K padded to 32]   (BF16X3 vHS)
+  int8_t* Lnak_q = nullptr;  // i8 [5 or 6 slices][re,im][K][N][Mp] (INT8_SPLIT exchange)
   double* Lnak_u = nullptr;  // f64 [K][N]   power-of-two unit of each Lnak row
+  int* Lnak_ord = nullptr;   // i32 [K]      slice-product order of each vector (N > 16)
   // [7]: eloc total/upper/lower, weight total/upper, population control calls/collapses
   unsigned long long* stats = nullptr;
   // back-propagation: the caller's field ring [fields_W][nfields][K] c128 and its position
@@ -214,10 +217,22 @@ void launch_gemm_nt_z(hipStream_t s, int precision, int m, int n, int k, double 
 // k_energy.hip
 int device_cus();                        // compute units of the current device (cached)
 int exx_nchunks(int W, int N, int K);   // chunks of Cholesky vectors the exchange kernel uses
-// k_exchange_bf16.hip: the exchange on 3-slice BF16 MFMA (N <= 16, M <= 96, M % 8 == 0)
+void launch_split_reduce(hipStream_t s, int m, int n, int splits, double alpha,
+                         const double* split_ws, double* C, long ldc);
+// k_gemm_bf16.hip: the BF16X3 form of the Cholesky GEMMs (three-product complex multiplication,
+// hi/lo BF16 splits, FP32 accumulation); B as BF16 planes from split_planes_bf16 (complex: 6
+// planes, real: 2, rows padded to bf16_row(cols))
+int bf16_row(int cols);
+void split_planes_bf16(hipStream_t s, long rows, int cols, bool complex_src, const double* src,
+                       __bf16* dst);
+void launch_gemm_bf16(hipStream_t s, int m, int n, int k, double alpha, const double* A, long lda,
+                      const __bf16* Bp, bool b_complex, double* C, long ldc,
+                      double* split_ws = nullptr, long split_ws_elems = 0);
+// k_exchange_bf16.hip: the exchange in the BF16X3 form (N <= 32, M <= 128)
 bool exchange_bf16_supported(int M, int N);
-int exchange_bf16_chunk(int W, int K);
-int exchange_bf16_chunks(int W, int K);
+size_t exchange_bf16_elems(int M, int N, int K);   // bf16 elements of the exchange planes
+int exchange_bf16_chunk(int W, int N, int K);
+int exchange_bf16_chunks(int W, int N, int K);
 void split_lnak_bf16(hipStream_t s, int M, int N, int K, const double* Lnak, __bf16* Lb);
 void launch_exchange_bf16(hipStream_t s, int W, int M, int N, int K, const double* G,
                           const __bf16* Lb, double* Kl, double* exx_part);
@@ -228,8 +243,12 @@ size_t exchange_i8_bytes(int M, int N, int K);     // size of the Lnak slices
 int exchange_i8_chunks(int W, int N, int K);   // tr(T^2) partials per walker
 void split_lnak_i8(hipStream_t s, int M, int N, int K, const double* Lnak, int8_t* Lq,
                    double* Lu);
+int exchange_i8_slices(int N);                     // slices of each Lnak row
+// per Cholesky vector the slice-product order of the 17..32-electron kernel (set-up, synchronous)
+void exchange_i8_orders(hipStream_t s, int M, int N, int K, const double* Lnak, int* ord);
 void launch_exchange_i8(hipStream_t s, int W, int M, int N, int K, const double* G,
-                        const int8_t* Lq, const double* Lu, double* Kl, double* exx_part);
+                        const int8_t* Lq, const double* Lu, const int* ord, double* Kl,
+                        double* exx_part);
 void launch_exchange(hipStream_t s, int precision, int W, int M, int N, int K, const double* G,
                      const void* Lnak, double* Kl, double* exx_part);
 void launch_energy_finalize(hipStream_t s, int W, int K, int nchunk, int walker_type,

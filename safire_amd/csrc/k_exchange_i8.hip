@@ -24,6 +24,10 @@
 // N <= 16 (one 16x16 tile per block), M <= 128.
 #include "afq_internal.hpp"
 
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
 namespace afq {
 
 namespace {
@@ -112,6 +116,14 @@ __device__ inline void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// workgroup barrier after this wavefront's LDS reads and writes have completed; global loads and
+// copies into LDS still in flight are not waited for (a fence would wait for them: the copies
+// write LDS). Every access to the copies' destination waits for them explicitly (vmcnt) before
+// the next full barrier. The memory clobber keeps the compiler from moving LDS accesses across.
+__device__ inline void lds_barrier() {
+  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
 template <int KC, int NV>
@@ -393,20 +405,36 @@ __global__ __launch_bounds__(64 * kWPB, kWgPerCu * kWPB / 4) void k_exchange_i8(
 
 // N in 17..32: T_wn is 2 x 2 tiles of 16 x 16. An 8-wavefront workgroup holds 4 walkers; the
 // two wavefronts of a walker own its row tiles ta = 0, 1 (G rows 16 ta .. 16 ta + 15 sliced into
-// registers) and compute both column tiles of their rows. Each vector's T goes to a per-walker
-// 32 x 32 LDS block; after a barrier every wavefront takes tr(T^2) over its rows and wavefront 0
-// of the walker takes tr(T) from the diagonal. The two row tiles write separate tr(T^2) partials
-// (slots 2c and 2c + 1 of the walker).
+// registers) and compute both column tiles of their rows.
 //
-// Staging: the int8 slices of one vector (10 slice-parts x 32 rows x 64 KC bytes, 20 or 40 KB)
-// are copied from global memory straight into LDS (global_load_lds_dwordx4, no registers), double
-// buffered, so the next vector's copy runs under this vector's MFMAs and epilogue. Rows are not
-// padded; the 16-byte chunk kv of row b sits at position kv ^ f(b) of its row (XOR swizzle), which
-// spreads the 16 rows an MFMA fragment reads over distinct banks. Rows b >= N and chunks past the
-// end of a global row copy clamped (in-bounds) data: their column unit is 0 and the matching G
-// entries are 0, so they contribute nothing. LDS: 2 x 40 KB + 4 x 16 KB at M = 128 (one
-// workgroup, two wavefronts per SIMD).
+// Slices: 6 of G and 6 of each Lnak row (base 254), and per Cholesky vector a slice-product order
+// o_n in 2..5 (all products s + t <= o_n kept; 21, 15, 10 or 6 products) fixed at set-up from
+// the vector's size r_n = max|L_n| / max_m |L_m|: o_n = ceil(5.5 + 2 log_254 r_n). The error a
+// vector adds to EXX scales as r_n^2 254^-(o_n + 1) (both its T block and the truncation of its
+// product series are proportional to r_n), so every vector then contributes at most
+// 254^-6.5 of the largest one's scale: on Cholesky vectors whose sizes decay, the large vectors
+// take all 21 products and the small ones few, at about the 15 products per vector of a uniform
+// order 4 and ~100x less error. The real part keeps Re G Re L and Im G Im L in separate int32
+// accumulators (subtracted exactly before the FP64 assembly), so no negated slices are needed.
+//
+// tr(T^2) = tr(T00^2) + tr(T11^2) + 2 tr(T01 T10): wavefront ta = 0 takes the first and the cross
+// term (it holds T00 and T01 in registers and reads T00^T and T10 from LDS), wavefront 1 the
+// second (T11^T from LDS); so only T00, T10 and T11 go to LDS, and wavefront 0 of the walker
+// takes tr(T) from their diagonals. The two row tiles write separate tr(T^2) partials (slots
+// 2c and 2c + 1 of the walker).
+//
+// Staging: the int8 slices of one vector (12 slice-parts x 32 rows x 64 KC bytes, 24 or 48 KB)
+// are copied from global memory straight into LDS (global_load_lds_dwordx4, no registers),
+// double-buffered in two LDS objects with the vector loop unrolled by two: every access then
+// names its buffer statically, so the compiler does not wait for the next vector's copies before
+// reading the current one. Rows are not padded; the 16-byte chunk kv of row b sits at position
+// kv ^ f(b) of its row (XOR swizzle), which spreads the 16 rows an MFMA fragment reads over
+// distinct banks. Rows b >= N and chunks past the end of a global row copy clamped (in-bounds)
+// data: their column unit is 0 and the matching G entries are 0, so they contribute nothing.
+// LDS: 2 x 48 KB + 4 x 12 KB at M = 128 (one workgroup, two wavefronts per SIMD).
 constexpr int kT2Walkers = 4;
+constexpr int kS6 = 6;          // slices of G and of Lnak (N > 16)
+constexpr int kO6 = 6;          // accumulator orders 0..5
 
 template <int KC>
 __device__ inline int t2_swz(int b) {
@@ -416,21 +444,23 @@ __device__ inline int t2_swz(int b) {
 template <int KC>
 __global__ __launch_bounds__(128 * kT2Walkers, 2) void k_exchange_i8_t2(
     int W, int M, int N, int K, int Mp, int kChunk, const double* __restrict__ G,
-    const int8_t* __restrict__ Lq, const double* __restrict__ Lu, double* __restrict__ Kl,
-    double* __restrict__ exx_part, int nslots) {
+    const int8_t* __restrict__ Lq, const double* __restrict__ Lu, const int* __restrict__ ord,
+    double* __restrict__ Kl, double* __restrict__ exx_part, int nslots) {
   constexpr int kRows = 32;
   constexpr int ROWB = 64 * KC;                  // staged row (bytes), unpadded
   constexpr int kPlane = kRows * ROWB;           // one slice-part of one vector
-  constexpr int kVecBytes = kSL * 2 * kPlane;
+  constexpr int kVecBytes = kS6 * 2 * kPlane;
   constexpr int kWindows = kVecBytes / 1024;     // 1 KB copied per wavefront instruction
   constexpr int kWaves = 2 * kT2Walkers;
+  constexpr int kTile = 16 * 16 * 2;             // doubles of one 16 x 16 complex tile
   static_assert(kVecBytes % 1024 == 0, "a vector's slices are whole 1 KB windows");
-  __shared__ __attribute__((aligned(16))) int8_t sB[2][kVecBytes];
-  __shared__ double sU[2][kRows];
-  // per walker T [a][b] (re, im); entry (x, y) sits in column y ^ (x % 16) of row x, so that 16
-  // lanes reading down a column (the transposed read) hit 16 different banks
-  __shared__ double sT[kT2Walkers][kRows * kRows * 2];
-  auto tcol = [](int x, int y) { return y ^ (x & 15); };
+  __shared__ __attribute__((aligned(16))) int8_t sB0[kVecBytes];
+  __shared__ __attribute__((aligned(16))) int8_t sB1[kVecBytes];
+  __shared__ double sU0[kRows];
+  __shared__ double sU1[kRows];
+  // per walker the tiles T00, T10, T11 [x][y] (re, im); entry (x, y) sits in column y ^ x of row
+  // x, so that 16 lanes reading down a column hit 16 different banks
+  __shared__ double sT[kT2Walkers][3 * kTile];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int g = lane >> 4, r16 = lane & 15;
@@ -444,7 +474,7 @@ __global__ __launch_bounds__(128 * kT2Walkers, 2) void k_exchange_i8_t2(
   const long KNMp = (long)K * N * Mp;
   const int arow = ta * 16 + r16;                   // this lane's row of G
 
-  i4 ar[KC][kSG], ai[KC][kSG], nai[KC][kSG];
+  i4 ar[KC][kS6], ai[KC][kS6];
   double uG;
   {
     const double* Gw = G + 2 * ((long)w * N + arow) * M;
@@ -469,7 +499,7 @@ __global__ __launch_bounds__(128 * kT2Walkers, 2) void k_exchange_i8_t2(
     for (int kc = 0; kc < KC; ++kc) {
 #pragma unroll
       for (int d = 0; d < 4; ++d) {
-        int qr[4][kSG], qi[4][kSG];
+        int qr[4][kS6], qi[4][kS6];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
           const int k = 64 * kc + 16 * g + 4 * d + j;
@@ -478,14 +508,13 @@ __global__ __launch_bounds__(128 * kT2Walkers, 2) void k_exchange_i8_t2(
             xr = Gw[2 * k] * inv;
             xi = Gw[2 * k + 1] * inv;
           }
-          slice<kSG>(xr, qr[j]);
-          slice<kSG>(xi, qi[j]);
+          slice<kS6>(xr, qr[j]);
+          slice<kS6>(xi, qi[j]);
         }
 #pragma unroll
-        for (int s = 0; s < kSG; ++s) {
+        for (int s = 0; s < kS6; ++s) {
           ar[kc][s][d] = (int)pack4(qr[0][s], qr[1][s], qr[2][s], qr[3][s]);
           ai[kc][s][d] = (int)pack4(qi[0][s], qi[1][s], qi[2][s], qi[3][s]);
-          nai[kc][s][d] = (int)pack4(-qi[0][s], -qi[1][s], -qi[2][s], -qi[3][s]);
         }
       }
     }
@@ -496,10 +525,13 @@ __global__ __launch_bounds__(128 * kT2Walkers, 2) void k_exchange_i8_t2(
 
   double exx_re = 0.0, exx_im = 0.0;
   double* T = sT[wl];
+  // LDS tile of (row tile x, column tile y): T00 -> 0, T10 -> 1, T11 -> 2 (T01 stays in registers)
+  auto tslot = [](int x, int y) { return x == 0 ? 0 : 1 + y; };
+  auto tpos = [](int x, int y) { return 2 * (x * 16 + (y ^ x)); };   // within a tile
 
-  // copy vector n's slices into sB[buf]: window win of the vector is LDS bytes
+  // copy vector n's slices into dst: window win of the vector is LDS bytes
   // [1024 win, 1024 win + 1024), lane l writes bytes 16 l .. 16 l + 15 of it
-  auto stage = [&](int n, int buf) {
+  auto stage = [&](int n, int8_t* dst) {
     for (int win = wave; win < kWindows; win += kWaves) {
       const int o = win * 1024 + 16 * lane;
       const int tp = o / kPlane;
@@ -510,7 +542,7 @@ __global__ __launch_bounds__(128 * kT2Walkers, 2) void k_exchange_i8_t2(
       const int8_t* src = Lq + tp * KNMp + ((long)n * N + bg) * Mp + kg;
       __builtin_amdgcn_global_load_lds(
           (const void __attribute__((address_space(1)))*)src,
-          (void __attribute__((address_space(3)))*)(sB[buf] + win * 1024), 16, 0, 0);
+          (void __attribute__((address_space(3)))*)(dst + win * 1024), 16, 0, 0);
     }
   };
   double preU = 0.0;
@@ -519,90 +551,104 @@ __global__ __launch_bounds__(128 * kT2Walkers, 2) void k_exchange_i8_t2(
   };
 
   if (nbeg < nend) {
-    stage(nbeg, 0);
     load_units(nbeg);
-    if (threadIdx.x < kRows) sU[0][threadIdx.x] = preU;
+    stage(nbeg, sB0);
   }
-  for (int n = nbeg, buf = 0; n < nend; ++n, buf ^= 1) {
-    // vector n is in sB[buf] (this thread's copies done, then everyone's); every wavefront is
-    // past iteration n - 1, so sB[buf ^ 1], sU[buf ^ 1] and the T blocks are free again
+  // one vector: its slices are in sBb (or in flight to it) and its column units in preU; they
+  // go to uB here, and the next vector's slices to sBn
+  auto vector = [&](int n, const int8_t* sBb, double* uB, int8_t* sBn) {
+    // vector n is in sBb (this thread's copies done, then everyone's); every wavefront is past
+    // vector n - 1, so sBn, uB and the T tiles are free again
     __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0)
+    if (threadIdx.x < kRows) uB[threadIdx.x] = preU;
     __syncthreads();
-    const bool more = n + 1 < nend;
-    if (more) {
-      stage(n + 1, buf ^ 1);
-      load_units(n + 1);
+    const int on = __builtin_amdgcn_readfirstlane(ord[n]);   // this vector's slice order
+    if (n + 1 < nend) {
+      load_units(n + 1);   // before the copies: waiting for it leaves them in flight
+      stage(n + 1, sBn);
     }
-    const int8_t* sBb = sB[buf];
-
-    // the two column tiles one after the other: each tile's T entries (rows 16 ta + 4g + r,
-    // columns 16 tb + r16) are assembled in FP64 and written to the walker's T block
+    double kre[4], kim[4];   // T01 (rows of wavefront 0, column tile 1) stays in registers
 #pragma unroll
     for (int tb = 0; tb < 2; ++tb) {
-      i4 accR[kOrders], accI[kOrders];
+      i4 acR[kO6], acQ[kO6], acI[kO6];   // Re G Re L, Im G Im L, Re G Im L + Im G Re L by order
 #pragma unroll
-      for (int o = 0; o < kOrders; ++o) {
-        accR[o] = i4{0, 0, 0, 0};
-        accI[o] = i4{0, 0, 0, 0};
+      for (int o = 0; o < kO6; ++o) {
+        acR[o] = i4{0, 0, 0, 0};
+        acQ[o] = i4{0, 0, 0, 0};
+        acI[o] = i4{0, 0, 0, 0};
       }
       const int row = tb * 16 + r16;
 #pragma unroll
       for (int kc = 0; kc < KC; ++kc) {
         const int chunk = (((4 * kc + g) ^ t2_swz<KC>(row)) << 4);
 #pragma unroll
-        for (int t = 0; t < kSL; ++t) {
+        for (int t = 0; t < kS6; ++t) {
+          if (t > on) break;   // uniform over the workgroup
           const int8_t* base = sBb + ((t * 2) * kRows + row) * ROWB + chunk;
           const i4 br = *reinterpret_cast<const i4*>(base);
           const i4 bi = *reinterpret_cast<const i4*>(base + kPlane);
 #pragma unroll
-          for (int s = 0; s < kSG; ++s)
-            if (kept(s, t)) {
-              accR[s + t] = mfma_i8(ar[kc][s], br, accR[s + t]);
-              accI[s + t] = mfma_i8(ar[kc][s], bi, accI[s + t]);
-            }
-#pragma unroll
-          for (int s = 0; s < kSG; ++s)
-            if (kept(s, t)) {
-              accR[s + t] = mfma_i8(nai[kc][s], bi, accR[s + t]);
-              accI[s + t] = mfma_i8(ai[kc][s], br, accI[s + t]);
+          for (int s = 0; s < kS6; ++s)
+            if (s + t < kO6 && s + t <= on) {
+              acR[s + t] = mfma_i8(ar[kc][s], br, acR[s + t]);
+              acI[s + t] = mfma_i8(ar[kc][s], bi, acI[s + t]);
+              acQ[s + t] = mfma_i8(ai[kc][s], bi, acQ[s + t]);
+              acI[s + t] = mfma_i8(ai[kc][s], br, acI[s + t]);
             }
         }
       }
-      const double uL = sU[buf][row];
+      const double uL = uB[row];
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        double xr = (double)accR[kOrders - 1][r], xi = (double)accI[kOrders - 1][r];
+        double xr = (double)(acR[kO6 - 1][r] - acQ[kO6 - 1][r]);
+        double xi = (double)acI[kO6 - 1][r];
 #pragma unroll
-        for (int o = kOrders - 2; o >= 0; --o) {
-          xr = fma(xr, 1.0 / kBase, (double)accR[o][r]);
-          xi = fma(xi, 1.0 / kBase, (double)accI[o][r]);
+        for (int o = kO6 - 2; o >= 0; --o) {
+          xr = fma(xr, 1.0 / kBase, (double)(acR[o][r] - acQ[o][r]));
+          xi = fma(xi, 1.0 / kBase, (double)acI[o][r]);
         }
         const double u = uRow[r] * uL;
-        const int a = ta * 16 + 4 * g + r;
-        T[2 * (a * kRows + tcol(a, row))] = xr * u;
-        T[2 * (a * kRows + tcol(a, row)) + 1] = xi * u;
+        if (ta == 0 && tb == 1) {
+          kre[r] = xr * u;
+          kim[r] = xi * u;
+        } else {
+          double* tile = T + tslot(ta, tb) * kTile;
+          const int x = 4 * g + r, y = r16;
+          tile[tpos(x, y)] = xr * u;
+          tile[tpos(x, y) + 1] = xi * u;
+        }
       }
     }
-    if (more && threadIdx.x < kRows) sU[buf ^ 1][threadIdx.x] = preU;
-    __syncthreads();   // both row tiles of T are in LDS
-#pragma unroll
-    for (int tb = 0; tb < 2; ++tb)
+    lds_barrier();   // T00, T10, T11 of every walker are in LDS
+    // ta = 0: sum_{a,b in tile 0} T_ab T_ba + 2 sum_{a in 0, b in 1} T_ab T_ba;
+    // ta = 1: sum_{a,b in tile 1} T_ab T_ba. For (a, b) = (4 g + r, r16) within the tiles, T_ab
+    // is entry (4 g + r, r16) of its tile and T_ba entry (r16, 4 g + r) of the tile (row tile of
+    // b, column tile of a).
+    {
+      const double* diag = T + (ta == 0 ? 0 : 2) * kTile;   // T00 or T11
+      const double* cross = T + 1 * kTile;                 // T10
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int a = ta * 16 + 4 * g + r, b = tb * 16 + r16;
-        const double xr = T[2 * (a * kRows + tcol(a, b))];
-        const double xi = T[2 * (a * kRows + tcol(a, b)) + 1];
-        const double yr = T[2 * (b * kRows + tcol(b, a))];
-        const double yi = T[2 * (b * kRows + tcol(b, a)) + 1];
+        const int x = 4 * g + r, y = r16;
+        const double xr = diag[tpos(x, y)], xi = diag[tpos(x, y) + 1];
+        const double yr = diag[tpos(y, x)], yi = diag[tpos(y, x) + 1];
         exx_re += xr * yr - xi * yi;
         exx_im += xr * yi + xi * yr;
+        if (ta == 0) {
+          const double zr = cross[tpos(y, x)], zi = cross[tpos(y, x) + 1];
+          exx_re += 2.0 * (kre[r] * zr - kim[r] * zi);
+          exx_im += 2.0 * (kre[r] * zi + kim[r] * zr);
+        }
       }
+    }
     if (ta == 0) {
-      // tr(T): diagonal entries a < N in lanes 0..31, summed by rows of 16 and combined
+      // tr(T): diagonals of T00 (lanes 0..15) and T11 (lanes 16..31), rows a < N
       double dr = 0.0, di = 0.0;
       if (lane < N && lane < kRows) {
-        dr = T[2 * (lane * kRows + tcol(lane, lane))];
-        di = T[2 * (lane * kRows + tcol(lane, lane)) + 1];
+        const double* tile = T + (lane < 16 ? 0 : 2) * kTile;
+        const int x = lane & 15;
+        dr = tile[tpos(x, x)];
+        di = tile[tpos(x, x) + 1];
       }
       dr = row_sum16(dr);
       di = row_sum16(di);
@@ -613,6 +659,10 @@ __global__ __launch_bounds__(128 * kT2Walkers, 2) void k_exchange_i8_t2(
         Kl[2 * ((long)w * K + n) + 1] = ki;
       }
     }
+  };
+  for (int n = nbeg; n < nend; n += 2) {
+    vector(n, sB0, sU0, sB1);
+    if (n + 1 < nend) vector(n + 1, sB1, sU1, sB0);
   }
 
   for (int off = 32; off > 0; off >>= 1) {
@@ -625,8 +675,21 @@ __global__ __launch_bounds__(128 * kT2Walkers, 2) void k_exchange_i8_t2(
   }
 }
 
-// Lq[t][p][n][b][0..Mp) = slice t of the real (p = 0) or imaginary (p = 1) part of
+// per Cholesky vector n: vmax[n] = max |component| of Lnak_n (one workgroup per vector)
+__global__ void k_vector_max(long per, const double* __restrict__ L, double* __restrict__ vmax) {
+  const long n = blockIdx.x;
+  double m = 0.0;
+  for (long e = threadIdx.x; e < 2 * per; e += blockDim.x) m = fmax(m, fabs(L[2 * n * per + e]));
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+  __shared__ double red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) vmax[n] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+}
+
+// Lq[t][p][n][b][0..Mp) = slice t < S of the real (p = 0) or imaginary (p = 1) part of
 // Lnak[n][b][:] / Lu[n][b] (zero for k >= M); one thread per row (n, b)
+template <int S>
 __global__ void k_split_lnak_i8(int rows, int M, int Mp, long KNMp, const double* __restrict__ L,
                                 int8_t* __restrict__ Lq, double* __restrict__ Lu) {
   const int row = blockIdx.x * blockDim.x + threadIdx.x;
@@ -645,10 +708,10 @@ __global__ void k_split_lnak_i8(int rows, int M, int Mp, long KNMp, const double
   for (int k = 0; k < Mp; ++k) {
 #pragma unroll
     for (int p = 0; p < 2; ++p) {
-      int q[kSL];
-      slice<kSL>(k < M ? x[2 * k + p] * inv : 0.0, q);
+      int q[S];
+      slice<S>(k < M ? x[2 * k + p] * inv : 0.0, q);
 #pragma unroll
-      for (int t = 0; t < kSL; ++t) Lq[(t * 2 + p) * KNMp + (long)row * Mp + k] = (int8_t)q[t];
+      for (int t = 0; t < S; ++t) Lq[(t * 2 + p) * KNMp + (long)row * Mp + k] = (int8_t)q[t];
     }
   }
 }
@@ -659,8 +722,38 @@ bool exchange_i8_supported(int M, int N) { return N <= 32 && M <= 64 * kMaxKC; }
 
 int exchange_i8_row(int M) { return (M + 15) / 16 * 16; }
 
+// slices of each Lnak row: 5 for up to 16 electrons, 6 for 17..32 (per-vector orders)
+int exchange_i8_slices(int N) { return N > 16 ? kS6 : kSL; }
+
 size_t exchange_i8_bytes(int M, int N, int K) {
-  return (size_t)kSL * 2 * K * N * exchange_i8_row(M);
+  return (size_t)exchange_i8_slices(N) * 2 * K * N * exchange_i8_row(M);
+}
+
+void exchange_i8_orders(hipStream_t s, int M, int N, int K, const double* Lnak, int* ord) {
+  double* vmax = nullptr;
+  AFQ_HIP(hipMalloc(&vmax, sizeof(double) * std::max(K, 1)));
+  std::vector<double> h(K);
+  try {
+    hipLaunchKernelGGL(k_vector_max, dim3(K), dim3(256), 0, s, (long)N * M, Lnak, vmax);
+    AFQ_HIP(hipGetLastError());
+    AFQ_HIP(hipMemcpyAsync(h.data(), vmax, sizeof(double) * K, hipMemcpyDeviceToHost, s));
+    AFQ_HIP(hipStreamSynchronize(s));
+  } catch (...) {
+    (void)hipFree(vmax);
+    throw;
+  }
+  AFQ_HIP(hipFree(vmax));
+  double top = 0.0;
+  for (double v : h) top = std::max(top, v);
+  std::vector<int> o(K, kO6 - 1);
+  for (int n = 0; n < K; ++n) {
+    if (!(top > 0.0) || !(h[n] <= top)) continue;   // zero or non-finite: keep every product
+    const double r = h[n] / top;
+    int on = r > 0.0 ? (int)std::ceil(5.5 + 2.0 * std::log(r) / std::log(kBase)) : 2;
+    o[n] = std::min(kO6 - 1, std::max(2, on));
+  }
+  AFQ_HIP(hipMemcpyAsync(ord, o.data(), sizeof(int) * K, hipMemcpyHostToDevice, s));
+  AFQ_HIP(hipStreamSynchronize(s));
 }
 
 // whole rounds of the resident workgroups (two per CU), in multiples of the staging group; for
@@ -696,18 +789,24 @@ void split_lnak_i8(hipStream_t s, int M, int N, int K, const double* Lnak, int8_
                    double* Lu) {
   const int rows = K * N;
   const int Mp = exchange_i8_row(M);
-  hipLaunchKernelGGL(k_split_lnak_i8, dim3((unsigned)((rows + 127) / 128)), dim3(128), 0, s, rows,
-                     M, Mp, (long)K * N * Mp, Lnak, Lq, Lu);
+  if (N > 16)
+    hipLaunchKernelGGL(k_split_lnak_i8<kS6>, dim3((unsigned)((rows + 127) / 128)), dim3(128), 0, s,
+                       rows, M, Mp, (long)K * N * Mp, Lnak, Lq, Lu);
+  else
+    hipLaunchKernelGGL(k_split_lnak_i8<kSL>, dim3((unsigned)((rows + 127) / 128)), dim3(128), 0, s,
+                       rows, M, Mp, (long)K * N * Mp, Lnak, Lq, Lu);
   AFQ_HIP(hipGetLastError());
 }
 
 void launch_exchange_i8(hipStream_t s, int W, int M, int N, int K, const double* G,
-                        const int8_t* Lq, const double* Lu, double* Kl, double* exx_part) {
+                        const int8_t* Lq, const double* Lu, const int* ord, double* Kl,
+                        double* exx_part) {
   if (W <= 0 || K <= 0) return;
   AFQ_CHECK(exchange_i8_supported(M, N), AFQ_EUNSUPPORTED,
             "int8 exchange: needs N <= 32 and M <= 128");
-  AFQ_CHECK((long)kSL * 2 * K * N * exchange_i8_row(M) < (1L << 31), AFQ_EUNSUPPORTED,
+  AFQ_CHECK(exchange_i8_bytes(M, N, K) < (1UL << 31), AFQ_EUNSUPPORTED,
             "int8 exchange: the Cholesky slices exceed 2 GiB");
+  AFQ_CHECK(N <= 16 || ord != nullptr, AFQ_EINVAL, "int8 exchange: no slice orders");
   const int chunk = exchange_i8_chunk(W, N, K);
   const int nchunk = (K + chunk - 1) / chunk;
   const int Mp = exchange_i8_row(M);
@@ -716,10 +815,10 @@ void launch_exchange_i8(hipStream_t s, int W, int M, int N, int K, const double*
     const int nslots = 2 * nchunk;
     if (M <= 64)
       hipLaunchKernelGGL((k_exchange_i8_t2<1>), grid, dim3(128 * kT2Walkers), 0, s, W, M, N, K,
-                         Mp, chunk, G, Lq, Lu, Kl, exx_part, nslots);
+                         Mp, chunk, G, Lq, Lu, ord, Kl, exx_part, nslots);
     else
       hipLaunchKernelGGL((k_exchange_i8_t2<2>), grid, dim3(128 * kT2Walkers), 0, s, W, M, N, K,
-                         Mp, chunk, G, Lq, Lu, Kl, exx_part, nslots);
+                         Mp, chunk, G, Lq, Lu, ord, Kl, exx_part, nslots);
   } else {
     dim3 grid((W + kWPB - 1) / kWPB, nchunk);
     if (M <= 64)

@@ -189,6 +189,15 @@ __global__ void k_split_reduce(int m, int n, int splits, double alpha,
 
 }  // namespace
 
+void launch_split_reduce(hipStream_t s, int m, int n, int splits, double alpha,
+                         const double* split_ws, double* C, long ldc) {
+  const long tot = (long)m * n;
+  hipLaunchKernelGGL(k_split_reduce, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, m, n,
+                     splits, alpha, reinterpret_cast<const double2*>(split_ws),
+                     reinterpret_cast<double2*>(C), ldc);
+  AFQ_HIP(hipGetLastError());
+}
+
 int gemm_splits(int m, int n, int k) {
   const int tiles = ((m + kTile - 1) / kTile) * ((n + kTile - 1) / kTile);
   int s = (512 + tiles - 1) / tiles;
@@ -199,7 +208,7 @@ int gemm_splits(int m, int n, int k) {
 int gemm_tile_configs(int precision, bool b_complex) {
   // FP32: 16- or 32-wide k slices, and for real B also 128-wide column tiles; FP64: one tiling
   // (a 64 x 128 FP64 tile or 32-wide FP64 slices would not fit the register or LDS budget)
-  if (precision == AFQ_FP64) return 1;
+  if (precision == AFQ_FP64 || precision == AFQ_BF16X3) return 1;
   return b_complex ? 2 : 4;
 }
 
@@ -266,13 +275,7 @@ void launch_gemm_nt_z(hipStream_t s, int precision, int m, int n, int k, double 
     }
   }
   AFQ_HIP(hipGetLastError());
-  if (splits > 1) {
-    const long tot = (long)m * n;
-    hipLaunchKernelGGL(k_split_reduce, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, m,
-                       n, splits, alpha, reinterpret_cast<const double2*>(split_ws),
-                       reinterpret_cast<double2*>(C), ldc);
-    AFQ_HIP(hipGetLastError());
-  }
+  if (splits > 1) launch_split_reduce(s, m, n, splits, alpha, split_ws, C, ldc);
 }
 
 }  // namespace afq

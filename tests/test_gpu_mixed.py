@@ -7,7 +7,12 @@ allows, on the same seeded walker states as the oracle:
 * FP32: operands rounded to FP32, FP32 MFMA accumulation. The rounding of G is shared by all
   Cholesky vectors, so the energy error grows with the energy scale: bounded here by
   5e-6 * (1 + |E1| + |EXX| + |EJ|) per walker (1.3e-6 measured).
-* BF16X3: both operands as three BF16 slices, FP32 accumulation: the same order as FP32.
+* BF16X3: three real products per complex product (Re Re, Im Im, (Re + Im)(Re + Im)), each
+  operand split into hi + lo BF16 (~2^-16 relative), each real product hi hi' + hi lo' + lo hi'
+  (the dropped lo lo' term is ~2^-16 of it), FP32 accumulation. The FP32 form's energy error is
+  ~22 u (u = 2^-24) of the energy scale; with u = 2^-16 and the Gauss recombination
+  Im = P3 - P1 - P2 adding up to ~3x, the bound here is 2e-3 * (1 + |E1| + |EXX| + |EJ|), and
+  5e-4 of the largest vbias / vHS element for the contractions.
 * INT8_SPLIT: row-scaled int8 slices (5 of G, 5 of Lnak, base 254) with exact int32
   accumulation: the truncation of the slices leaves ~1e-3 of the FP32 error, bounded here by
   1e-9 * (1 + |E1| + |EXX| + |EJ|) per walker.
@@ -28,7 +33,9 @@ if not torch.cuda.is_available():  # pragma: no cover
 from safire_amd import afqmc as A  # noqa: E402
 
 # relative-to-energy-scale bound of each reduced form
-FORMS = {"fp32": (A.FP32, 5e-6), "bf16x3": (A.BF16X3, 5e-6), "int8": (A.INT8_SPLIT, 1e-9)}
+FORMS = {"fp32": (A.FP32, 5e-6), "bf16x3": (A.BF16X3, 2e-3), "int8": (A.INT8_SPLIT, 1e-9)}
+# relative bound of vbias / vHS (largest element) of each reduced form
+CONTRACTION_BOUND = {"fp32": 1e-5, "bf16x3": 5e-4, "int8": 1e-5}
 
 
 def states(s, W, amp, seed):
@@ -43,8 +50,6 @@ def states(s, W, amp, seed):
 def test_reduced_form_energy(form, shape):
     prec, bound = FORMS[form]
     M, N, K = shape
-    if form == "bf16x3" and N > 16:
-        pytest.skip("the BF16X3 exchange covers N <= 16")
     s = synth.make_system(M, N, K, seed=M + N)
     W = 48
     ham = A.HamiltonianOperations(s.h, s.Likn, s.PsiT, 0.0, s.dt)
@@ -82,20 +87,22 @@ def test_reduced_form_contractions(form):
     ham.vbias(A.to_device(G), v)
     torch.cuda.synchronize()
     vref = O.vbias(G, R["Lnak"], s.dt)
-    # FP32 operands: error ~ 2^-24 sqrt(N M) relative to the row magnitude
-    assert np.abs(v.cpu().numpy() - vref).max() <= 1e-5 * np.abs(vref).max()
+    # FP32 operands: error ~ 2^-24 sqrt(N M) relative to the row magnitude (int8 and FP32 forms:
+    # FP32 GEMMs); BF16 hi/lo operands ~2^-16
+    cb = CONTRACTION_BOUND[form]
+    assert np.abs(v.cpu().numpy() - vref).max() <= cb * np.abs(vref).max()
     X = rand_c(rng, W, s.K)
     V = torch.zeros((W, s.M, s.M), dtype=torch.complex128, device="cuda")
     ham.vHS(A.to_device(X), V)
     torch.cuda.synchronize()
     Vref = O.vhs(X, s.Likn, s.dt)
-    assert np.abs(V.cpu().numpy() - Vref).max() <= 1e-5 * np.abs(Vref).max()
+    assert np.abs(V.cpu().numpy() - Vref).max() <= cb * np.abs(Vref).max()
     ham.close()
 
 
-@pytest.mark.parametrize("prec,M,N", [(A.BF16X3, 40, 20), (A.INT8_SPLIT, 136, 20)])
+@pytest.mark.parametrize("prec,M,N", [(A.BF16X3, 136, 20), (A.INT8_SPLIT, 136, 20)])
 def test_split_forms_reject_unsupported_shapes(prec, M, N):
-    s = synth.make_system(M, N, 60, seed=2)        # N beyond 16 (BF16X3), M beyond 128 (int8)
+    s = synth.make_system(M, N, 60, seed=2)        # M beyond 128
     ham = A.HamiltonianOperations(s.h, s.Likn, s.PsiT, 0.0, s.dt)
     with pytest.raises(A.AfqError):
         ham.set_precision(prec)
