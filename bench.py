@@ -48,7 +48,8 @@ GATE_HA = 1e-5
 # scaled by a copy kernel of known size), as recorded in profiles/walker_step_c2_hbm_traffic.md;
 # null for a workload or form without such a measurement
 TRAFFIC_BYTES = {("c2", "int8"): 38.54e6, ("c2", "fp64"): 53.29e6,
-                 ("c3", "int8"): 5946.18e6, ("c3", "fp64"): 21822.38e6}
+                 ("c3", "int8"): 2939.46e6, ("c3", "fp64"): 21822.41e6,
+                 ("c3", "bf16x3"): 1741.51e6}
 TRAFFIC_SOURCE = {"c2": "profiles/walker_step_c2_hbm_traffic.md",
                   "c3": "profiles/walker_step_c3_hbm_traffic.md"}
 
@@ -247,6 +248,23 @@ def bench_phmsd(args, A, torch, synth):
     value = W / (ms / 1e3)
     print(f"[bench] c5 fp64 ({args.ph_energy} energy): {ms:.2f} ms/step, {value:.0f} "
           f"walker-steps/s", file=sys.stderr, flush=True)
+    # the dominant part of the step: the PHMSD local energy, timed alone (HIP events on the
+    # launch stream, 3 calls) on the walkers of the last step
+    eflops = ph_energy_flops(occs, s.N, s.M, s.K, W, args.ph_energy)
+    ph.energy(b)
+    torch.cuda.synchronize()
+    start.record()
+    for _ in range(3):
+        ph.energy(b)
+    stop.record()
+    torch.cuda.synchronize()
+    e_ms = start.elapsed_time(stop) / 3
+    e_tf = eflops / (e_ms / 1e3) / 1e12
+    print(f"[bench] c5 energy ({args.ph_energy}): {e_ms:.2f} ms per call, {e_tf:.2f} TFLOP/s",
+          file=sys.stderr, flush=True)
+    cpu = None
+    if not args.no_cpu:
+        cpu = ph_cpu_baseline(s, occs, ci, orb, b, args.ph_energy, args.cpu_seconds)
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "walker-steps/s", "n_gpus": 1,
         "steps": steps, "warmup": warmup, "ms_per_step": round(ms, 4),
@@ -257,11 +275,100 @@ def bench_phmsd(args, A, torch, synth):
                                f"dt={s.dt}, local-energy step, QR every {ortho}",
                    "form": "FP64, eager launches", "ph_energy_algorithm": args.ph_energy,
                    "parallelism": "single GPU"},
-        "roofline": None, "cpu_baseline": None,
+        "roofline": {"bound": "fp64 (vector ALU; no MFMA form)", "achieved": round(e_tf, 3),
+                     "peak": PEAK_TFLOPS["fp64"], "unit": "TFLOP/s",
+                     "frac": round(e_tf / PEAK_TFLOPS["fp64"], 4), "traffic": None,
+                     "kernel": f"PHMSD local energy ({args.ph_energy}: k_ph_2body_lds "
+                               f"and companions, one call)" if args.ph_energy == "woodbury"
+                               else "PHMSD local energy (reference: k_ph0_inverse + k_ph0_chol, "
+                                    "one call)",
+                     "flop_per_launch": eflops, "mean_launch_ms": round(e_ms, 4)},
+        "cpu_baseline": cpu,
     }
     ph.close()
     print(json.dumps(out))
     return 0
+
+
+def ph_energy_flops(occs, N, M, K, W, algorithm):
+    """Algorithmic FLOP of one PHMSD local energy over W walkers (complex MAC = 8), counted from
+    the expansion: per spin, every unique excited configuration of rank nex contributes
+    K (nelec nex nact + nelec^2) complex MACs to the excited-determinant 2-body energy (Woodbury,
+    phmsd_impl.hpp:29-147), or, for the reference algorithm (PHMSD.cpp:122-370), K nelec^3 for
+    its Y = P T^-1 rows and tr(Y Y) (nelec = N, nact = M)."""
+    import numpy as np
+    total = 0.0
+    for spin in range(2):
+        part = occs[:, spin * N:(spin + 1) * N] - spin * M
+        uniq = np.unique(np.sort(part, axis=1), axis=0)
+        ranks = (uniq >= N).sum(1)
+        ranks = ranks[ranks > 0]
+        if algorithm == "woodbury":
+            total += float(np.sum(K * (N * ranks * M + N * N)))
+        else:
+            total += float(len(ranks) * K * (N ** 3 + N * N))
+    return 8.0 * W * total
+
+
+def ph_cpu_baseline(s, occs, ci, orb, b, algorithm, seconds):
+    """The oracle's PHMSD local energy (numpy restatement) on a bounded sample of the c5
+    walkers: the step's dominant part; walker-energies/s reported as walker-steps/s, the rest of
+    the step (propagation, vbias, QR) excluded, so an upper bound on the CPU step rate."""
+    import numpy as np
+
+    from oracle import ph_energy as P
+    buf = b.cpu().numpy()
+    M, N = s.M, s.N
+    Pu = buf[:, :M * N].reshape(-1, M, N)
+    Pd = buf[:, M * N:2 * M * N].reshape(-1, M, N)
+    # orbital rows of both spins, half-rotated Cholesky vectors and one-body term, as the
+    # oracle's tests build them
+    A = [np.ascontiguousarray(orb[0]), np.ascontiguousarray(orb[-1])]
+    L = np.ascontiguousarray(s.Likn.transpose(2, 0, 1))
+    Lna = [np.ascontiguousarray(np.einsum("al,nlk->nak", A[sp], L)) for sp in range(2)]
+    haj = [np.ascontiguousarray(A[sp] @ s.h) for sp in range(2)]
+    struct = P.build_ph_struct(ci, occs, M, N, N)
+    fn = P.energy_alg1 if algorithm == "woodbury" else P.energy_alg0
+    done, el = 0, 0.0
+    while done < len(Pu) and el < seconds:
+        t0 = time.perf_counter()
+        fn(struct, A, Lna, haj, 0.0, Pu[done:done + 1], Pd[done:done + 1])
+        el += time.perf_counter() - t0
+        done += 1
+    return {"value": round(done / el, 3), "unit": "walker-steps/s", "cores": 1, "kind": "port",
+            "sample": (f"c5 system, PHMSD local energy ({algorithm}) of {done} walkers, numpy "
+                       f"restatement (oracle/ph_energy.py), {el:.1f} s; the rest of the step is "
+                       f"not included (an upper bound on the CPU step rate)")}
+
+
+def batch_cpu_baseline(systems, ortho, seconds):
+    """The FP64 oracle (C++, OpenMP over walkers) stepping the systems one after another, each
+    with all its walkers, until the time bound; walker-steps/s over the systems reached."""
+    import numpy as np
+
+    from oracle import oracle as O
+    from safire_amd import synth
+    done, ws, el = 0, 0, 0.0
+    for s, W in systems:
+        if el >= seconds:
+            break
+        ref = O.System(s.h, s.Likn, s.PsiT, 0.0, s.dt)
+        buf = ref.new_walkers(W)
+        ref.seed_properties(buf)
+        Eshift = float(ref.energy(buf)[0].sum(1).real.mean())
+        xi = synth.fields(s.seed, 1, W, s.K)[0]
+        t0 = time.perf_counter()
+        ref.step(buf, xi, Eshift, hybrid=False)
+        ref.orthogonalize(buf)
+        O.population_control(buf, s.M * s.N, float(np.random.default_rng(s.seed).random()))
+        el += time.perf_counter() - t0
+        done += 1
+        ws += W
+    return {"value": round(ws / el, 2), "unit": "walker-steps/s", "cores": O.num_threads(),
+            "kind": "port",
+            "sample": (f"first {done} of {len(systems)} systems, one local-energy step each with "
+                       f"QR and population control, all their walkers ({ws}), C++ FP64 oracle, "
+                       f"OpenMP over walkers, {el:.1f} s")}
 
 
 def bench_batch(args, A, torch, synth):
@@ -285,6 +392,19 @@ def bench_batch(args, A, torch, synth):
         b.run_sequential()
         torch.cuda.synchronize()
         seq = time.perf_counter() - t0
+        # exchange kernels of every system, timed in a second one-at-a-time run (region timers)
+        b.reset()
+        for it in b.items:
+            it["ham"].enable_timers(16 * steps + 16)
+        b.run_sequential()
+        torch.cuda.synchronize()
+        ex_flop, ex_ms = 0.0, 0.0
+        for it in b.items:
+            tm = it["ham"].read_timers()
+            it["ham"].enable_timers(0)
+            sy = it["sys"]
+            ex_flop += tm["exchange"][1] * exchange_flops(it["W"], sy.M, sy.N, sy.K)
+            ex_ms += tm["exchange"][0]
         ref = [it["wset"].buf.clone() for it in b.items]
         b.reset()
         g = b.capture(streams=args.streams)
@@ -299,7 +419,10 @@ def bench_batch(args, A, torch, synth):
         bat = start.elapsed_time(stop) / 1e3
         same = all(same_bits(torch, it["wset"].buf, r) for it, r in zip(b.items, ref))
         ws = b.walker_steps()
-        res[name] = {"tilings_autotuned": len(b.tilings),
+        ex_tf = ex_flop / (ex_ms / 1e3) / 1e12 if ex_ms > 0 else 0.0
+        res[name] = {"exchange_tflops": round(ex_tf, 3),
+                     "exchange_ms_per_step": round(ex_ms / steps, 4),
+                     "tilings_autotuned": len(b.tilings),
                      "batched_walker_steps_per_s": round(ws / bat, 1),
                      "one_at_a_time_walker_steps_per_s": round(ws / seq, 1),
                      "batched_ms_per_step": round(bat * 1e3 / steps, 4),
@@ -329,7 +452,16 @@ def bench_batch(args, A, torch, synth):
                                             res[head]["one_at_a_time_walker_steps_per_s"], 3),
         "mixed_over_fp64": round(res["int8"]["batched_walker_steps_per_s"] /
                                  res["fp64"]["batched_walker_steps_per_s"], 3),
-        "roofline": None, "cpu_baseline": None,
+        "roofline": {"bound": "mfma", "achieved": res[head]["exchange_tflops"],
+                     "peak": PEAK_TFLOPS[head], "unit": "TFLOP/s",
+                     "frac": round(res[head]["exchange_tflops"] / PEAK_TFLOPS[head], 4),
+                     "traffic": None,
+                     "kernel": f"k_exchange ({head}) over all systems: total algorithmic FLOP / "
+                               f"total launch time, one-at-a-time timed run",
+                     "executed_frac": round(res[head]["exchange_tflops"] *
+                                            EXECUTED_PER_ALGO[head] / PEAK_TFLOPS[head], 4)},
+        "cpu_baseline": None if args.no_cpu else batch_cpu_baseline(systems, ortho,
+                                                                    args.cpu_seconds),
     }
     print(json.dumps(out))
     return 0

@@ -441,6 +441,25 @@ void orc_exp_hermitian(int n, const cd* A, cd* expA) {
     }
 }
 
+// P1 as the reference's host path applies it (denseP1 = false, parameters.hpp:120): to_csr
+// keeps entries with |x| > 1e-8 (csr_utils.hpp:43-67, AFQMCBasePropagator.cpp:165-166); when
+// every kept imaginary part is within 1e-14 of the largest |x|, the real parts are applied in
+// real arithmetic, again cut at |x| > 1e-8 (AFQMCBasePropagator.cpp:174-211).
+void orc_sparsify_p1(int M, cd* P) {
+  const long n = (long)M * M;
+  double vmax = 0.0, imax = 0.0;
+  for (long e = 0; e < n; ++e) {
+    if (std::abs(P[e]) > 1e-8) {
+      vmax = std::max(vmax, std::abs(P[e]));
+      imax = std::max(imax, std::fabs(P[e].imag()));
+    } else {
+      P[e] = 0.0;
+    }
+  }
+  if (imax <= 1e-14 * vmax)
+    for (long e = 0; e < n; ++e) P[e] = std::fabs(P[e].real()) > 1e-8 ? cd(P[e].real(), 0.0) : cd(0.0);
+}
+
 // generateP1 (AFQMCBasePropagator.cpp:85-157) after vMF is known, closed-shell / spin-independent:
 //   H1 = dt*(h + v0) + vHS(vMF)   (getOneBodyPropagatorMatrix, Real3IndexFactorization.hpp:99-124,
 //                                  add_one_body_shifts one_body.hpp:64-85)
@@ -462,6 +481,7 @@ void orc_one_body_propagator(int M, int K, double dt, const double* h, const dou
   if (H1out) std::copy(H1.begin(), H1.end(), H1out);
   for (auto& x : H1) x *= -0.5;
   orc_exp_hermitian(M, H1.data(), P1);
+  orc_sparsify_p1(M, P1);
 }
 
 // ============================= propagate.hpp ==================================================

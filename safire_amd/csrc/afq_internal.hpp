@@ -302,6 +302,9 @@ void launch_population_control(hipStream_t s, int W, double* wbuf, long ldw, lon
 void setup_hamiltonian(afq_ham_s* H, hipStream_t s);
 void launch_v0(hipStream_t s, int M, int K, const double* Likn, double* v0);
 void launch_zero_imag(hipStream_t s, int n, double* v);
+// P1 [M][M] c128 on the host as the reference applies it: |x| <= 1e-8 dropped, imaginary
+// roundoff dropped when P1 is real to 1e-14
+void sparsify_one_body_propagator(int M, double* P);
 void build_one_body_propagator(hipStream_t s, int M, int K, double dt, const double* h,
                                const double* v0, const double* Likn, const double* vMF,
                                double* H1, double* P1);

@@ -129,6 +129,26 @@ void launch_zero_imag(hipStream_t s, int n, double* v) {
   AFQ_HIP(hipGetLastError());
 }
 
+void sparsify_one_body_propagator(int M, double* P) {
+  const long n = (long)M * M;
+  double vmax = 0.0, imax = 0.0;
+  for (long e = 0; e < n; ++e) {
+    const double a = std::hypot(P[2 * e], P[2 * e + 1]);
+    if (a > 1e-8) {
+      vmax = std::max(vmax, a);
+      imax = std::max(imax, std::fabs(P[2 * e + 1]));
+    } else {
+      P[2 * e] = P[2 * e + 1] = 0.0;
+    }
+  }
+  if (imax <= 1e-14 * vmax) {
+    for (long e = 0; e < n; ++e) {
+      P[2 * e + 1] = 0.0;
+      if (!(std::fabs(P[2 * e]) > 1e-8)) P[2 * e] = 0.0;
+    }
+  }
+}
+
 // H1 = dt (h + v0) + vHS(vMF), hermitized, and P1 = exp(-H1/2) (getOneBodyPropagatorMatrix,
 // Real3IndexFactorization.hpp:99-124; generateP1, AFQMCBasePropagator.cpp:141-157) by scaling
 // and squaring of a degree-18 Taylor polynomial. Synchronises the stream (the scaling is chosen
@@ -180,8 +200,15 @@ void build_one_body_propagator(hipStream_t s, int M, int K, double dt, const dou
                          (const cd*)P, (const cd*)P, 0, (cd*)Q);
       std::swap(P, Q);
     }
-    AFQ_HIP(hipMemcpyAsync(P1, P, 2 * (size_t)M * M * sizeof(double), hipMemcpyDeviceToDevice,
-                           s));
+    // the reference keeps P1 sparse (denseP1 = false, parameters.hpp:120): entries with
+    // |x| <= 1e-8 are dropped (to_csr, AFQMCBasePropagator.cpp:165-166), and a P1 whose
+    // imaginary parts are all within 1e-14 of its largest entry is applied in real arithmetic
+    // (AFQMCBasePropagator.cpp:174-211): the same matrix here, with those entries zeroed
+    std::vector<double> hP(2 * (size_t)M * M);
+    AFQ_HIP(hipMemcpyAsync(hP.data(), P, hP.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    AFQ_HIP(hipStreamSynchronize(s));
+    sparsify_one_body_propagator(M, hP.data());
+    AFQ_HIP(hipMemcpyAsync(P1, hP.data(), hP.size() * sizeof(double), hipMemcpyHostToDevice, s));
     AFQ_HIP(hipGetLastError());
     AFQ_HIP(hipStreamSynchronize(s));
   } catch (...) {

@@ -366,3 +366,33 @@ def test_population_control_parity(W):
     _, _, pc = ham.stats()
     assert pc[0] == 5 and pc[1] == 1
     ham.close()
+
+
+def test_one_body_propagator_sparse_truncation_gpu():
+    """The GPU's P1 drops what the reference's sparse P1 drops (|x| <= 1e-8, imaginary roundoff
+    of a real P1; AFQMCBasePropagator.cpp:165-211): on two decoupled blocks of orbitals its
+    off-block entries are exactly zero and it equals the oracle's P1."""
+    rng = np.random.default_rng(13)
+    M, N, K = 12, 3, 10
+    blocks = [np.arange(0, 7), np.arange(7, M)]
+    h = np.zeros((M, M))
+    L = np.zeros((K, M, M))
+    for b in blocks:
+        X = rng.standard_normal((len(b), len(b)))
+        h[np.ix_(b, b)] = 0.3 * (X + X.T)
+        for n in range(K):
+            Y = rng.standard_normal((len(b), len(b)))
+            L[n][np.ix_(b, b)] = 0.4 * (Y + Y.T)
+    Likn = np.ascontiguousarray(L.transpose(1, 2, 0))
+    Q, _ = np.linalg.qr(rand_c(rng, M, M))
+    PsiT = np.ascontiguousarray(Q[:, :N].conj().T)
+    ham = A.HamiltonianOperations(h, Likn, PsiT, 0.0, 0.01)
+    P1g = ham.tensors()["P1"]
+    P1o = O.System(h, Likn, PsiT, 0.0, 0.01).tensors()["P1"]
+    off = np.ones((M, M), bool)
+    for b in blocks:
+        off[np.ix_(b, b)] = False
+    assert np.all(P1g[off] == 0.0) and np.all(P1o[off] == 0.0)
+    assert np.all(P1g.imag == 0.0) == np.all(P1o.imag == 0.0)
+    assert relerr(P1g, P1o) < 1e-12
+    ham.close()

@@ -89,10 +89,11 @@ def test_ph_energy_c5_shape(ndet, max_rank):
 
 @pytest.mark.parametrize("uhf,nmo,nup,ndn,ndet,rank,nw,block", [
     (False, 9, 4, 3, 40, 3, 5, 32), (True, 9, 4, 3, 40, 3, 9, 4), (False, 24, 17, 12, 30, 2, 3, 2),
-    (False, 8, 3, 3, 1, 1, 4, 8)])
+    (False, 8, 3, 3, 1, 1, 4, 8), (False, 20, 14, 13, 30, 2, 3, 2), (True, 16, 10, 10, 25, 2, 3, 4)])
 def test_ph_energy_reference_algorithm_matches_oracle(uhf, nmo, nup, ndn, ndet, rank, nw, block):
     """energy_alg0 (set_energy_algorithm("reference")): one G and energy per unique
-    configuration; 17 electrons of one spin cover pivoting past 16 rows, ndet = 1 the
+    configuration; 17 electrons of one spin cover pivoting past 16 rows, 13 and 14 the kernel
+    for 13..16 electrons, 10 the kernel with the count fixed at compile time, ndet = 1 the
     single-determinant case."""
     rng = np.random.default_rng(60 + uhf + nup)
     s = ph_system(rng, nmo, nup, ndn, 7, ndet, rank, uhf=uhf)

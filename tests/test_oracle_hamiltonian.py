@@ -302,3 +302,36 @@ def test_population_control_placement():
     for slot, s in {0: 1, 1: 1, 2: 2, 3: 1}.items():
         assert buf[slot, 0] == 10 + s
     np.testing.assert_allclose(np.abs(buf[:, P + O.P_WEIGHT]), 1.0)
+
+
+def test_one_body_propagator_sparse_truncation():
+    """P1 as the reference's host path applies it (denseP1 = false): entries with |x| <= 1e-8
+    dropped (to_csr, AFQMCBasePropagator.cpp:165-166) and a real P1 applied in real arithmetic
+    (:174-211). Two decoupled blocks (block-diagonal h and Cholesky vectors) give a P1 whose
+    off-block entries are roundoff: they must be exactly zero, the imaginary parts too, and the
+    kept entries equal expm(-H1/2)."""
+    import scipy.linalg
+    rng = np.random.default_rng(12)
+    M, K, dt = 10, 8, 0.01
+    blocks = [np.arange(0, 6), np.arange(6, M)]
+    h = np.zeros((M, M))
+    L = np.zeros((K, M, M))
+    for b in blocks:
+        X = rng.standard_normal((len(b), len(b)))
+        h[np.ix_(b, b)] = 0.3 * (X + X.T)
+        for n in range(K):
+            Y = rng.standard_normal((len(b), len(b)))
+            L[n][np.ix_(b, b)] = 0.4 * (Y + Y.T)
+    Likn = np.ascontiguousarray(L.transpose(1, 2, 0))
+    vMF = rng.standard_normal(K) * 0.3 + 0j
+    P1, H1 = O.one_body_propagator(h, Likn, vMF, dt)
+    ref = scipy.linalg.expm(-0.5 * H1)
+    off = np.ones((M, M), bool)
+    for b in blocks:
+        off[np.ix_(b, b)] = False
+    assert np.abs(ref[off]).max() < 1e-8           # roundoff in the exact exponential
+    assert np.all(P1[off] == 0.0)                   # dropped as the CSR conversion drops it
+    assert np.all(P1.imag == 0.0)                   # real P1: real arithmetic
+    keep = np.abs(ref.real) > 1e-8
+    np.testing.assert_allclose(P1[keep].real, ref[keep].real, rtol=1e-12, atol=1e-14)
+    assert np.all(P1[~keep] == 0.0)
