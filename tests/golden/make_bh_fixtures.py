@@ -1,5 +1,6 @@
-"""Writes bh_rhf_closed.npz from the reference's BH inputs and statistical reference, and
-bh_rhf_closed_snapshot.json from its seeded 20-step snapshot run, read as data with h5dump (no HDF5 module for Python is available here). Run once, in a tree where the
+"""Writes bh_rhf_closed.npz from the reference's BH inputs and statistical reference,
+bh_rhf_closed_snapshot.json from its seeded 20-step snapshot run, and bh_rcasci_rhf_phmsd.npz /
+bh_rcasci_rhf_phmsd_snapshot.json from the CASCI PHMSD trial and its seeded snapshot run, read as data with h5dump (no HDF5 module for Python is available here). Run once, in a tree where the
 reference exists:
 
     python tests/golden/make_bh_fixtures.py /root/reference
@@ -132,7 +133,59 @@ def snapshot(ref_root):
     print(f"wrote {out}: Energy {obs['Energy']['mean']} +- {obs['Energy']['error']}")
 
 
+def csr_dense(path, grp):
+    shape = raw(path, f"{grp}/shape", "<i4", (2,))
+    rp = raw(path, f"{grp}/row_pointers", "<i4", (shape[0] + 1,))
+    nnz = int(rp[-1])
+    cols = raw(path, f"{grp}/column_indices", "<i4", (nnz,))
+    vals = complex_array(path, f"{grp}/values")
+    assert vals.size == nnz
+    A = np.zeros(tuple(shape), np.complex128)
+    for r in range(shape[0]):
+        for j in range(rp[r], rp[r + 1]):
+            A[r, cols[j]] = vals[j]
+    return A
+
+
+def phmsd_snapshot(ref_root):
+    """The BH CASCI PHMSD trial (tests/functional/afqmc_inputs/BH/afqmc_casci_rhf_phmsd.h5:
+    Wavefunction/PHMSD/PsiT_0 in CSR, occa, occb, ci_coeffs, read as readWfn.cpp:180-215 reads
+    them) and the seeded 20-step COLLINEAR snapshot run with it (snapshot_references/BH/
+    dense_rhf_basis_closed/rcasci_rhf_phmsd/collinear/results.h5): the scalar observables and
+    the run's input. The Hamiltonian is the closed-shell BH one of bh_rhf_closed.npz."""
+    base = os.path.join(ref_root, "tests", "functional")
+    Wf = os.path.join(base, "afqmc_inputs", "BH", "afqmc_casci_rhf_phmsd.h5")
+    R = os.path.join(base, "snapshot_references", "BH", "dense_rhf_basis_closed",
+                     "rcasci_rhf_phmsd", "collinear", "results.h5")
+    PsiT0 = csr_dense(Wf, "/Wavefunction/PHMSD/PsiT_0")
+    hdr = text(Wf, ["-H"])
+    nd, nup = [int(x) for x in re.search(r'DATASET "occa".*?\(\s*(\d+),\s*(\d+)\s*\)', hdr,
+                                         re.S).groups()]
+    ndn = int(re.search(r'DATASET "occb".*?\(\s*\d+,\s*(\d+)\s*\)', hdr, re.S).group(1))
+    occa = raw(Wf, "/Wavefunction/PHMSD/occa", "<i4", (nd, nup))
+    occb = raw(Wf, "/Wavefunction/PHMSD/occb", "<i4", (nd, ndn))
+    ci = complex_array(Wf, "/Wavefunction/PHMSD/ci_coeffs")
+    out = os.path.join(HERE, "bh_rcasci_rhf_phmsd.npz")
+    np.savez_compressed(out, PsiT0=PsiT0, occa=occa, occb=occb, ci=ci)
+    obs = {}
+    for name in ("Energy", "OnebodyEnergy", "ExchangeEnergy", "CoulombEnergy", "Overlap",
+                 "EffectiveNumWalkers", "Phase", "ProjectionTime", "TotalWeight"):
+        obs[name] = {k: scalar_value(R, f"/measurements/Stage0/{name}/{k}")
+                     for k in ("mean", "error")}
+    inp = text(R, ["-d", "/input_file"])
+    run = json.loads(inp[inp.index('"{') + 1:inp.rindex('}"') + 1].replace('\\"', '"'))
+    for k in ("wavefunction", "hamiltonian"):
+        run["execute"][k]["filename"] = os.path.basename(run["execute"][k]["filename"])
+    ranks = int(re.search(r"\(0\):\s*(\d+)", text(R, ["-d", "/num_ranks"])).group(1))
+    snap = os.path.join(HERE, "bh_rcasci_rhf_phmsd_snapshot.json")
+    with open(snap, "w") as f:
+        json.dump({"observables": obs, "run": run, "num_ranks": ranks}, f, indent=1)
+    print(f"wrote {out} ({nd} determinants, {nup}+{ndn} electrons, PsiT_0 {PsiT0.shape}) and "
+          f"{snap}: Energy {obs['Energy']['mean']} +- {obs['Energy']['error']}")
+
+
 if __name__ == "__main__":
     root = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
     main(root)
     snapshot(root)
+    phmsd_snapshot(root)
