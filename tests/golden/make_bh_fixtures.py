@@ -147,17 +147,16 @@ def csr_dense(path, grp):
     return A
 
 
-def phmsd_snapshot(ref_root):
+def phmsd_snapshot(ref_root, kind="rhf"):
     """The BH CASCI PHMSD trial (tests/functional/afqmc_inputs/BH/afqmc_casci_rhf_phmsd.h5:
     Wavefunction/PHMSD/PsiT_0 in CSR, occa, occb, ci_coeffs, read as readWfn.cpp:180-215 reads
     them) and the seeded 20-step COLLINEAR snapshot run with it (snapshot_references/BH/
     dense_rhf_basis_closed/rcasci_rhf_phmsd/collinear/results.h5): the scalar observables and
     the run's input. The Hamiltonian is the closed-shell BH one of bh_rhf_closed.npz."""
     base = os.path.join(ref_root, "tests", "functional")
-    Wf = os.path.join(base, "afqmc_inputs", "BH", "afqmc_casci_rhf_phmsd.h5")
+    Wf = os.path.join(base, "afqmc_inputs", "BH", f"afqmc_casci_{kind}_phmsd.h5")
     R = os.path.join(base, "snapshot_references", "BH", "dense_rhf_basis_closed",
-                     "rcasci_rhf_phmsd", "collinear", "results.h5")
-    PsiT0 = csr_dense(Wf, "/Wavefunction/PHMSD/PsiT_0")
+                     f"rcasci_{kind}_phmsd", "collinear", "results.h5")
     hdr = text(Wf, ["-H"])
     nd, nup = [int(x) for x in re.search(r'DATASET "occa".*?\(\s*(\d+),\s*(\d+)\s*\)', hdr,
                                          re.S).groups()]
@@ -165,8 +164,20 @@ def phmsd_snapshot(ref_root):
     occa = raw(Wf, "/Wavefunction/PHMSD/occa", "<i4", (nd, nup))
     occb = raw(Wf, "/Wavefunction/PHMSD/occb", "<i4", (nd, ndn))
     ci = complex_array(Wf, "/Wavefunction/PHMSD/ci_coeffs")
-    out = os.path.join(HERE, "bh_rcasci_rhf_phmsd.npz")
-    np.savez_compressed(out, PsiT0=PsiT0, occa=occa, occb=occb, ci=ci)
+    if 'GROUP "PsiT_0"' in hdr:
+        PsiT0 = csr_dense(Wf, "/Wavefunction/PHMSD/PsiT_0")
+    else:
+        # occupations of the molecular orbitals only (PHMSDOrbitalType::occ): with the reference
+        # determinant on the lowest orbitals of each spin (build_PsiT_MO_phmsd leaves the
+        # expansion as it is then) the trial orbitals are the orbital basis itself
+        M = int(raw(Wf, "/Wavefunction/PHMSD/occa", "<i4", (nd, nup)).max()) + 1
+        assert list(occa[0]) == list(range(nup)) and list(occb[0]) == list(range(ndn))
+        PsiT0 = None
+    out = os.path.join(HERE, f"bh_rcasci_{kind}_phmsd.npz")
+    if PsiT0 is None:
+        np.savez_compressed(out, occa=occa, occb=occb, ci=ci)
+    else:
+        np.savez_compressed(out, PsiT0=PsiT0, occa=occa, occb=occb, ci=ci)
     obs = {}
     for name in ("Energy", "OnebodyEnergy", "ExchangeEnergy", "CoulombEnergy", "Overlap",
                  "EffectiveNumWalkers", "Phase", "ProjectionTime", "TotalWeight"):
@@ -177,10 +188,10 @@ def phmsd_snapshot(ref_root):
     for k in ("wavefunction", "hamiltonian"):
         run["execute"][k]["filename"] = os.path.basename(run["execute"][k]["filename"])
     ranks = int(re.search(r"\(0\):\s*(\d+)", text(R, ["-d", "/num_ranks"])).group(1))
-    snap = os.path.join(HERE, "bh_rcasci_rhf_phmsd_snapshot.json")
+    snap = os.path.join(HERE, f"bh_rcasci_{kind}_phmsd_snapshot.json")
     with open(snap, "w") as f:
         json.dump({"observables": obs, "run": run, "num_ranks": ranks}, f, indent=1)
-    print(f"wrote {out} ({nd} determinants, {nup}+{ndn} electrons, PsiT_0 {PsiT0.shape}) and "
+    print(f"wrote {out} ({nd} determinants, {nup}+{ndn} electrons) and "
           f"{snap}: Energy {obs['Energy']['mean']} +- {obs['Energy']['error']}")
 
 
@@ -188,4 +199,5 @@ if __name__ == "__main__":
     root = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
     main(root)
     snapshot(root)
-    phmsd_snapshot(root)
+    phmsd_snapshot(root, "rhf")
+    phmsd_snapshot(root, "uhf")

@@ -11,9 +11,10 @@ observables are compared with the reference's exact-mode rule (run_functional.py
 compare_exactly, MACHINE_EPS = 1e-9) on their mean and standard error, for one rebin size as in
 test_gpu_bh_snapshot.py.
 
-Every energy component, the weights, the phase and the effective walker number match. The
-Overlap observable (the weighted mean of exp(log overlap)) matches in magnitude and comes out
-with the opposite sign in every sample: the multi-determinant overlap here carries an extra
+Every energy component, the weights, the phase and the effective walker number match, and for
+the expansion over the orbital basis the Overlap observable too. Over the RHF file's PsiT_0
+orbitals the Overlap (the weighted mean of exp(log overlap)) matches in magnitude and comes out
+with the opposite sign in every sample: the multi-determinant overlap there carries an extra
 global factor -1 (a phase of pi in its logarithm) against the reference's. The trajectory does
 not see it (the phaseless projection uses differences of log overlaps, and the energies ratios
 with the overlap); where the factor arises is not established, so the test holds the Overlap
@@ -98,17 +99,23 @@ class PHEnergyEstimator(ES.EnergyEstimator):
         return self.prop.estimator_sums(wset, E, ov).numpy()
 
 
-def test_bh_phmsd_snapshot_field_for_field():
-    snap = json.load(open(os.path.join(GOLDEN, "bh_rcasci_rhf_phmsd_snapshot.json")))
+@pytest.mark.parametrize("kind", ["rhf", "uhf"])
+def test_bh_phmsd_snapshot_field_for_field(kind):
+    """kind "rhf": the CASCI expansion over the RHF orbitals of the file's PsiT_0 (44
+    determinants); "uhf": an expansion given as occupations of the orbital basis only (34
+    determinants, no PsiT in the file: the trial orbitals are the basis itself, the reference
+    determinant on the lowest orbitals)."""
+    snap = json.load(open(os.path.join(GOLDEN, f"bh_rcasci_{kind}_phmsd_snapshot.json")))
     run = snap["run"]["execute"]
     seed = snap["run"]["seed"]
     assert snap["num_ranks"] == 1 and run["walker_set"]["walker_type"] == "COLLINEAR"
     d = np.load(os.path.join(GOLDEN, "bh_rhf_closed.npz"))
-    f = np.load(os.path.join(GOLDEN, "bh_rcasci_rhf_phmsd.npz"))
+    f = np.load(os.path.join(GOLDEN, f"bh_rcasci_{kind}_phmsd.npz"))
     nmo = d["hcore"].shape[0]
+    orb = f["PsiT0"] if "PsiT0" in f.files else np.eye(nmo, dtype=np.complex128)
     W, steps, dt = run["num_walkers"], run["steps"], run["timestep"]
     occs = np.concatenate([f["occa"], f["occb"] + nmo], axis=1)
-    ph = A.PHMSD(occs, f["ci"], f["PsiT0"][None], d["hcore"], d["Likn"], float(d["E0"]))
+    ph = A.PHMSD(occs, f["ci"], orb[None], d["hcore"], d["Likn"], float(d["E0"]))
     ph.prepare(dt, W)
     assert ph.params.hybrid == 1
     wset = SimpleNamespace(buf=A.to_device(ph.new_walkers(W)), W=W)
@@ -127,13 +134,13 @@ def test_bh_phmsd_snapshot_field_for_field():
     report = {}
     for name, ref in snap["observables"].items():
         x = meas.bins(name).astype(np.complex128).reshape(-1)
-        if name == "Overlap":
-            x = -x   # the global sign described above
         assert len(x) == steps, name
         rm = complex(*ref["mean"])
         re_ = complex(*ref["error"])
-        hits = [B for B in range(1, steps // 2 + 1)
-                if close(rebinned(x, B)[0], rm) and close(rebinned(x, B)[1], re_)]
+        # the Overlap up to the one global sign described above
+        signs = (1.0, -1.0) if name == "Overlap" else (1.0,)
+        hits = [B for sg in signs for B in range(1, steps // 2 + 1)
+                if close(rebinned(sg * x, B)[0], rm) and close(rebinned(sg * x, B)[1], re_)]
         report[name] = (hits, rebinned(x, 1)[0], rm)
     print({k: (v[0][:3], v[1], v[2]) for k, v in report.items()})
     bad = {k: v for k, v in report.items() if not v[0]}
