@@ -764,9 +764,12 @@ int exchange_i8_chunk(int W, int N, int K) {
   const int wpb = N > 16 ? kT2Walkers : kWPB;
   const int cus = device_cus() * (N > 16 ? 1 : kWgPerCu);
   const int wgroups = (W + wpb - 1) / wpb;
+  // N > 16: every chunk re-reads and re-slices G of its walkers, so the chunks may span all
+  // vectors (one workgroup round when the walker groups fill the GPU)
+  const int jmax = N > 16 ? (K + step - 1) / step : 16;
   int best = step;
   double best_eff = -1.0;
-  for (int j = 1; j <= 16; ++j) {
+  for (int j = 1; j <= jmax; ++j) {
     const int chunk = step * j;
     const long nwg = (long)wgroups * ((K + chunk - 1) / chunk);
     const long rounds = (nwg + cus - 1) / cus;

@@ -4,9 +4,10 @@ The mixed forms change only how vbias, vHS and the exchange contraction are comp
 tests hold each form's energies (E1, EXX, EJ) and contractions to the error its arithmetic
 allows, on the same seeded walker states as the oracle:
 
-* FP32: operands rounded to FP32, FP32 MFMA accumulation. The rounding of G is shared by all
-  Cholesky vectors, so the energy error grows with the energy scale: bounded here by
-  5e-6 * (1 + |E1| + |EXX| + |EJ|) per walker (1.3e-6 measured).
+* FP32: operands rounded to FP32, FP32 MFMA accumulation. The rounding of G (u = 2^-24 of each
+  entry) is shared by all Cholesky vectors and adds up coherently in EXX, so the energy error
+  grows with the energy scale, as ~sqrt(N M) u over the shapes here (up to 64 u = 3.8e-6 at
+  N M = 4096, measured up to 1.5e-6): bounded by 5e-6 * (1 + |E1| + |EXX| + |EJ|) per walker.
 * BF16X3: three real products per complex product (Re Re, Im Im, (Re + Im)(Re + Im)), each
   operand split into hi + lo BF16 (~2^-16 relative), each real product hi hi' + hi lo' + lo hi'
   (the dropped lo lo' term is ~2^-16 of it), FP32 accumulation. The FP32 form's energy error is
