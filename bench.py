@@ -48,8 +48,8 @@ GATE_HA = 1e-5
 # scaled by a copy kernel of known size), as recorded in profiles/walker_step_c2_hbm_traffic.md;
 # null for a workload or form without such a measurement
 TRAFFIC_BYTES = {("c2", "int8"): 38.54e6, ("c2", "fp64"): 53.29e6,
-                 ("c3", "int8"): 2939.46e6, ("c3", "fp64"): 21822.41e6,
-                 ("c3", "bf16x3"): 1741.51e6}
+                 ("c3", "int8"): 592.08e6, ("c3", "fp64"): 21822.38e6,
+                 ("c3", "bf16x3"): 1745.07e6}
 TRAFFIC_SOURCE = {"c2": "profiles/walker_step_c2_hbm_traffic.md",
                   "c3": "profiles/walker_step_c3_hbm_traffic.md"}
 
