@@ -88,10 +88,15 @@ def same_bits(torch, a, b):
                             torch.view_as_real(b).view(torch.int64)))
 
 
-def run_form(A, torch, s, W, xi_dev, uniforms, warmup, steps, ortho, precision, hybrid, graph):
+def run_form(A, torch, s, W, xi_dev, uniforms, warmup, steps, ortho, precision, hybrid, graph,
+             tile_cache=None):
     ham = A.HamiltonianOperations(s.h, s.Likn, s.PsiT, 0.0, s.dt)
     ham.set_params(hybrid=int(hybrid))
     ham.set_precision(precision)
+    if tile_cache is not None:
+        # the vbias and vHS GEMM tilings of this precision, timed for this shape and walker
+        # count (every tiling sums k in the same order: the choice changes no result)
+        ham.autotune(W, tile_cache)
     wset = A.WalkerSet(ham, W)
     prop = A.Propagator(ham)
     prop.seed(wset)
@@ -490,7 +495,8 @@ def bench_single(args, A, torch, synth, workload, with_cpu, with_hybrid):
     for f in args.forms.split(","):
         try:
             forms[f] = run_form(A, torch, s, W, xi_dev, uniforms, args.warmup, args.steps, ortho,
-                                FORM_PRECISION[f], hybrid=False, graph=not args.no_graph)
+                                FORM_PRECISION[f], hybrid=False, graph=not args.no_graph,
+                                tile_cache=args.tile_cache)
         except A.AfqError as e:
             if e.code != 5:   # AFQ_EUNSUPPORTED: the form has no kernel for this shape
                 raise
@@ -630,7 +636,8 @@ def bench_single(args, A, torch, synth, workload, with_cpu, with_hybrid):
         hyb = {}
         for f in sorted({"fp64", mixed}):
             r = run_form(A, torch, s, W, xi_dev, uniforms, args.warmup, args.steps, ortho,
-                         FORM_PRECISION[f], hybrid=True, graph=not args.no_graph)
+                         FORM_PRECISION[f], hybrid=True, graph=not args.no_graph,
+                         tile_cache=args.tile_cache)
             hyb[f] = {"walker_steps_per_s": round(W * args.steps / (r["ms"] / 1e3), 1),
                       "ms_per_step": round(r["ms"] / args.steps, 4)}
             r["ham"].close()
@@ -667,7 +674,8 @@ def main(argv=None):
                     help="c4: capture streams the systems are spread over")
     ap.add_argument("--tile-cache", default=os.path.join(tempfile.gettempdir(),
                                                          "safire_amd_tiles.json"),
-                    help="c4: the GEMM autotuner's JSON cache (every system's tilings are tuned)")
+                    help="the GEMM autotuner's JSON cache (every form's vbias and vHS tilings "
+                         "are tuned for the workload's shape)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--forms", default="fp64,fp32,bf16x3,int8")
