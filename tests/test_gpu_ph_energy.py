@@ -234,3 +234,36 @@ def test_ph_step_matches_oracle(hybrid):
     d.imag = (d.imag + np.pi) % (2 * np.pi) - np.pi
     assert np.abs(d).max() < 1e-10
     ph.close()
+
+
+@pytest.mark.parametrize("algorithm", ["woodbury", "reference"])
+def test_ph_energy_c5_workload(algorithm):
+    """The c5 workload itself: the synthetic NMO 64 / 10 + 10 electrons / Nchol 320 Hamiltonian
+    with its 1000-determinant expansion (synth.phmsd_expansion), 3 walkers near the trial, both
+    energy algorithms against the oracle's energy_alg1 at the full Cholesky rank."""
+    from safire_amd import synth
+    s, _, _, _ = synth.workload("c5")
+    occs, ci = synth.phmsd_expansion(s.M, s.N, 1000, s.seed)
+    orb = np.ascontiguousarray(s.meta["orbitals"].conj().T).astype(np.complex128)
+    rng = np.random.default_rng(55)
+    nw = 3
+    Q = s.meta["orbitals"][:, :s.N].astype(np.complex128)
+    Pu = Q[None] + 0.2 * rand_c(rng, nw, s.M, s.N)
+    Pd = Q[None] + 0.2 * rand_c(rng, nw, s.M, s.N)
+    buf = np.zeros((nw, 2 * s.M * s.N + 3), complex)
+    buf[:, :s.M * s.N] = Pu.reshape(nw, -1)
+    buf[:, s.M * s.N:2 * s.M * s.N] = Pd.reshape(nw, -1)
+    ph = A.PHMSD(occs, ci, orb[None], s.h, s.Likn, 0.0)
+    ph.set_energy_algorithm(algorithm)
+    E, ov = ph.energy(A.to_device(buf))
+    A_ = [orb, orb]
+    L = np.ascontiguousarray(s.Likn.transpose(2, 0, 1))
+    Lna = [np.ascontiguousarray(np.einsum("al,nlk->nak", A_[sp], L)) for sp in range(2)]
+    haj = [np.ascontiguousarray(A_[sp] @ s.h) for sp in range(2)]
+    struct = P.build_ph_struct(ci, occs, s.M, s.N, s.N)
+    Eo, ovo = P.energy_alg1(struct, A_, Lna, haj, 0.0, Pu, Pd)
+    np.testing.assert_allclose(E.cpu().numpy(), Eo, rtol=1e-9, atol=1e-10 * np.abs(Eo).max())
+    d = ov.cpu().numpy() - ovo
+    d.imag = (d.imag + np.pi) % (2 * np.pi) - np.pi
+    assert np.abs(d).max() < 1e-9
+    ph.close()
