@@ -41,15 +41,24 @@ METRIC = "walker-steps/s mixed vs FP64 at |ΔE_loc|<1e-5 Ha; % MFMA/HBM peak"
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0, "int8": 5000.0}
 # executed MFMA flops per algorithmic flop of the exchange contraction
 # (BF16X3: 3 real products per complex product x 3 hi/lo terms = 18 flops per complex
-# multiply-add against the 8 counted; int8: 15 slice products)
+# multiply-add against the 8 counted; int8: the slice products per Cholesky vector, 15 for the
+# five-slice kernel of N <= 16, per-vector orders for N > 16 with their mean on the NMO=128 /
+# 32+32 / Nchol=640 workload read from the SQ_INSTS_VALU_MFMA_MOPS_I8 counter)
 EXECUTED_PER_ALGO = {"fp64": 1.0, "fp32": 1.0, "bf16x3": 18.0 / 8.0, "int8": 15.0}
+INT8_PRODUCTS = {"c3": 13.37}
+
+
+def executed_per_algo(form, workload):
+    if form == "int8":
+        return INT8_PRODUCTS.get(workload, EXECUTED_PER_ALGO["int8"])
+    return EXECUTED_PER_ALGO[form]
 GATE_HA = 1e-5
 # HBM bytes per launch of each form's exchange kernel (FETCH_SIZE and WRITE_SIZE counter runs
 # scaled by a copy kernel of known size), as recorded in profiles/walker_step_c2_hbm_traffic.md;
 # null for a workload or form without such a measurement
 TRAFFIC_BYTES = {("c2", "int8"): 38.54e6, ("c2", "fp64"): 53.29e6,
-                 ("c3", "int8"): 592.08e6, ("c3", "fp64"): 21822.38e6,
-                 ("c3", "bf16x3"): 1745.07e6}
+                 ("c3", "int8"): 586.57e6, ("c3", "fp64"): 21822.38e6,
+                 ("c3", "bf16x3"): 1739.17e6}
 TRAFFIC_SOURCE = {"c2": "profiles/walker_step_c2_hbm_traffic.md",
                   "c3": "profiles/walker_step_c3_hbm_traffic.md"}
 
@@ -464,7 +473,7 @@ def bench_batch(args, A, torch, synth):
                      "kernel": f"k_exchange ({head}) over all systems: total algorithmic FLOP / "
                                f"total launch time, one-at-a-time timed run",
                      "executed_frac": round(res[head]["exchange_tflops"] *
-                                            EXECUTED_PER_ALGO[head] / PEAK_TFLOPS[head], 4)},
+                                            executed_per_algo(head, "c4") / PEAK_TFLOPS[head], 4)},
         "cpu_baseline": None if args.no_cpu else batch_cpu_baseline(systems, ortho,
                                                                     args.cpu_seconds),
     }
@@ -590,7 +599,8 @@ def bench_single(args, A, torch, synth, workload, with_cpu, with_hybrid):
                      "kernel": f"k_exchange ({mixed} MFMA)",
                      "flop_per_launch": exchange_flops(W, M, N, K),
                      "mean_launch_ms": round(ex_avg, 5),
-                     "executed_frac": round(achieved * EXECUTED_PER_ALGO[mixed] / peak, 4)},
+                     "executed_frac": round(achieved * executed_per_algo(mixed, workload) / peak,
+                                            4)},
     }
     if ref_form is not None:
         v64 = W * args.steps / (ref_form["ms"] / 1e3)
@@ -616,7 +626,7 @@ def bench_single(args, A, torch, synth, workload, with_cpu, with_hybrid):
         ms_, n_ = r["timers"]["exchange"]
         if n_:
             ex_rate = exchange_flops(W, M, N, K) / (ms_ / n_ / 1e3) / 1e12
-            util[f] = round(ex_rate * EXECUTED_PER_ALGO[f] / PEAK_TFLOPS[f], 4)
+            util[f] = round(ex_rate * executed_per_algo(f, workload) / PEAK_TFLOPS[f], 4)
     out["exchange_mfma_utilisation"] = util
     if unsupported:
         out["forms_unsupported"] = unsupported

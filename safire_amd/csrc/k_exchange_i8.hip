@@ -409,12 +409,12 @@ __global__ __launch_bounds__(64 * kWPB, kWgPerCu * kWPB / 4) void k_exchange_i8(
 //
 // Slices: 6 of G and 6 of each Lnak row (base 254), and per Cholesky vector a slice-product order
 // o_n in 2..5 (all products s + t <= o_n kept; 21, 15, 10 or 6 products) fixed at set-up from
-// the vector's size r_n = max|L_n| / max_m |L_m|: o_n = ceil(5.5 + 2 log_254 r_n). The error a
+// the vector's size r_n = max|L_n| / max_m |L_m|: o_n = ceil(5 + 2 log_254 r_n). The error a
 // vector adds to EXX scales as r_n^2 254^-(o_n + 1) (both its T block and the truncation of its
-// product series are proportional to r_n), so every vector then contributes at most
-// 254^-6.5 of the largest one's scale: on Cholesky vectors whose sizes decay, the large vectors
-// take all 21 products and the small ones few, at about the 15 products per vector of a uniform
-// order 4 and ~100x less error. The real part keeps Re G Re L and Im G Im L in separate int32
+// product series are proportional to r_n), so every vector then contributes at most 254^-6 of
+// the largest one's scale: on Cholesky vectors whose sizes decay, the large vectors take all 21
+// products and the small ones few, fewer products on average than a uniform order 4 (15) for
+// tens of times less error. The real part keeps Re G Re L and Im G Im L in separate int32
 // accumulators (subtracted exactly before the FP64 assembly), so no negated slices are needed.
 //
 // tr(T^2) = tr(T00^2) + tr(T11^2) + 2 tr(T01 T10): wavefront ta = 0 takes the first and the cross
@@ -749,7 +749,7 @@ void exchange_i8_orders(hipStream_t s, int M, int N, int K, const double* Lnak, 
   for (int n = 0; n < K; ++n) {
     if (!(top > 0.0) || !(h[n] <= top)) continue;   // zero or non-finite: keep every product
     const double r = h[n] / top;
-    int on = r > 0.0 ? (int)std::ceil(5.5 + 2.0 * std::log(r) / std::log(kBase)) : 2;
+    int on = r > 0.0 ? (int)std::ceil(5.0 + 2.0 * std::log(r) / std::log(kBase)) : 2;
     o[n] = std::min(kO6 - 1, std::max(2, on));
   }
   AFQ_HIP(hipMemcpyAsync(ord, o.data(), sizeof(int) * K, hipMemcpyHostToDevice, s));
